@@ -1,0 +1,192 @@
+"""Write golden vectors for the mixing hot path from the reference's NumPy path.
+
+Runs ONLY in the build container, where the veros source tree is mounted at
+/root/reference (read-only). The reference is imported with a no-op stand-in
+for the missing ``loguru`` logger (logging only; no numerical code is
+replaced), its NumPy backend and float64. Inputs come from this package's own
+seeded generators (veros_amd.workloads); each fixture stores inputs and the
+reference's outputs as a compressed ``.npz`` next to this script.
+
+Functions covered (reference file:line):
+* solve_tridiagonal_numpy      veros/core/operators.py:60-77
+* solve_implicit               veros/core/utilities.py:51-59
+* create_water_masks           veros/core/utilities.py:38-48
+* isoneutral_diffusion_pre     veros/core/isoneutral/isoneutral.py:18-229
+* isoneutral_diffusion_kernel  veros/core/isoneutral/diffusion.py:210-283 (iso and skew)
+
+Usage: python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import sys
+import tempfile
+from pathlib import Path
+from types import SimpleNamespace
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+REPO = HERE.parent.parent
+REFERENCE = Path("/root/reference")
+
+_LOGURU_STUB = '''
+class _Logger:
+    def __getattr__(self, name):
+        return self._noop
+    def _noop(self, *args, **kwargs):
+        return self
+logger = _Logger()
+'''
+
+
+def _import_reference():
+    stub_dir = Path(tempfile.mkdtemp(prefix="loguru_stub_"))
+    (stub_dir / "loguru").mkdir()
+    (stub_dir / "loguru" / "__init__.py").write_text(_LOGURU_STUB)
+    sys.path.insert(0, str(stub_dir))
+    sys.path.insert(0, str(REFERENCE))
+    os.environ["VEROS_BACKEND"] = "numpy"
+    os.environ["VEROS_FLOAT_TYPE"] = "float64"
+    from veros.core import operators, utilities  # noqa: F401
+    from veros.core.isoneutral import isoneutral, diffusion
+    return SimpleNamespace(operators=operators, utilities=utilities, isoneutral=isoneutral,
+                           diffusion=diffusion)
+
+
+def _ref_state(state):
+    """The reference kernels read state.variables / .settings / .dimensions."""
+    vs = state.variables
+    st = state.settings
+    v = SimpleNamespace(**{k: (np.array(x, copy=True) if isinstance(x, np.ndarray) else x)
+                           for k, x in vars(vs).items()})
+    s = SimpleNamespace(nz=st.nz, dt_tracer=st.dt_tracer, K_iso_steep=st.K_iso_steep,
+                        iso_slopec=st.iso_slopec, iso_dslope=st.iso_dslope,
+                        enable_conserve_energy=st.enable_conserve_energy, grav=st.grav,
+                        rho_0=st.rho_0, eq_of_state_type=1)
+    dims = {"xt": st.nx, "xu": st.nx, "yt": st.ny, "yu": st.ny, "zt": st.nz, "zw": st.nz}
+    return SimpleNamespace(variables=v, settings=s, dimensions=dims)
+
+
+STATE_KEYS = ("temp", "salt", "K_iso", "K_gm", "kbot", "maskT", "maskU", "maskV", "maskW",
+              "dxt", "dxu", "dyt", "dyu", "cost", "cosu", "dzt", "dzw", "zt",
+              "Ai_ez", "Ai_nz", "Ai_bx", "Ai_by", "K_11", "K_22", "K_33",
+              "dtemp_iso", "dsalt_iso", "P_diss_iso", "P_diss_skew", "int_drhodT", "int_drhodS")
+SETTING_KEYS = ("nx", "ny", "nz", "dt_tracer", "K_iso_steep", "iso_slopec", "iso_dslope",
+                "enable_conserve_energy", "enable_cyclic_x", "grav", "rho_0", "drhodT", "drhodS")
+
+
+def tdma_cases(ref):
+    from veros_amd import workloads as W
+    rng = np.random.default_rng(17)
+    cases = {}
+    # 1) diffusion-type systems, independent sub/super diagonals, b_edge and d_edge
+    p = W.diffusion_tdma(rng, 12, 10, 8, dz=W.geometric_dz(8, 10.0, 60.0), land_frac=0.3,
+                         independent_ac=True, with_b_edge=True, with_d_edge=True)
+    cases["diffusion_12x10x8"] = p
+    # 2) every special column: full depth, single wet cell (kbot = nz), land
+    p = W.diffusion_tdma(rng, 9, 7, 5, dz=W.geometric_dz(5, 10.0, 40.0), land_frac=0.0,
+                         independent_ac=True, with_b_edge=True)
+    p.kbot[:] = rng.integers(0, 6, size=p.kbot.shape)
+    p.kbot[0, :] = 5
+    p.kbot[1, :] = 1
+    p.kbot[2, :] = 0
+    cases["special_9x7x5"] = p
+    # 3) no water at all (early return)
+    p = W.diffusion_tdma(rng, 4, 3, 6, dz=np.ones(6), land_frac=0.0)
+    p.kbot[:] = 0
+    cases["allland_4x3x6"] = p
+    # 4) the reference test's system family: standard-normal coefficients
+    shape = (7, 6, 6)
+    a, b, c, d = (rng.standard_normal(shape) for _ in range(4))
+    p = W.TdmaProblem(a, b, c, d, rng.integers(0, 6, size=shape[:2]).astype(np.int32))
+    cases["randn_7x6x6"] = p
+
+    out = {}
+    for name, p in cases.items():
+        nz = p.a.shape[-1]
+        _, water, edge = ref.utilities.create_water_masks(p.kbot, nz)
+        x_tri = ref.operators.solve_tridiagonal_numpy(p.a, p.b, p.c, p.d, water, edge)
+        x_imp = ref.utilities.solve_implicit(p.a, p.b, p.c, p.d, water, edge,
+                                             b_edge=p.b_edge, d_edge=p.d_edge)
+        rec = dict(a=p.a, b=p.b, c=p.c, d=p.d, kbot=p.kbot, water=np.asarray(water),
+                   edge=np.asarray(edge), x_tridiagonal=np.asarray(x_tri), x_implicit=np.asarray(x_imp))
+        if p.b_edge is not None:
+            rec["b_edge"] = p.b_edge
+        if p.d_edge is not None:
+            rec["d_edge"] = p.d_edge
+        out[name] = rec
+    return out
+
+
+def iso_cases(ref):
+    from veros_amd import workloads as W
+    specs = {
+        # random slopes of both signs, random K_iso, jittered grid, noisy in/out fields
+        "iso_12x10x8_cyclic_energy": dict(seed=21, nx=12, ny=10, nz=8, cyclic=True, stratified=False,
+                                          random_k_iso=True, jitter=0.02, random_aux=True, tau=1,
+                                          taup1=2, energy=True, dz=(10.0, 40.0)),
+        # ACC-like stratification, constant K_iso, energy branch off
+        "iso_20x16x12_strat": dict(seed=22, nx=20, ny=16, nz=12, cyclic=True, stratified=True,
+                                   random_k_iso=False, jitter=0.0, random_aux=False, tau=0, taup1=1,
+                                   energy=False, dz=(10.0, 60.0)),
+        # closed domain, single-cell and land columns
+        "iso_9x7x5_closed": dict(seed=23, nx=9, ny=7, nz=5, cyclic=False, stratified=False,
+                                 random_k_iso=True, jitter=0.02, random_aux=True, tau=2, taup1=0,
+                                 energy=True, dz=(10.0, 20.0)),
+    }
+    out = {}
+    for name, sp in specs.items():
+        rng = np.random.default_rng(sp["seed"])
+        settings = W.MixSettings(nx=sp["nx"], ny=sp["ny"], nz=sp["nz"], enable_cyclic_x=sp["cyclic"],
+                                 enable_conserve_energy=sp["energy"], K_iso_steep=1.0 if sp["random_aux"] else 500.0)
+        st = W.iso_state(rng, sp["nx"], sp["ny"], sp["nz"], dx_deg=1.0, dy_deg=1.0, y_origin=-30.0,
+                         dz_top=sp["dz"][0], dz_bottom=sp["dz"][1], land_frac=0.25, cyclic=sp["cyclic"],
+                         settings=settings, tau=sp["tau"], taup1=sp["taup1"], random_k_iso=sp["random_k_iso"],
+                         stratified=sp["stratified"], jitter=sp["jitter"], random_aux=sp["random_aux"])
+        vs = st.variables
+        X, Y, nz = vs.K_iso.shape
+        if name == "iso_9x7x5_closed":
+            vs.kbot[2, 2:-2] = nz          # single wet cell columns
+            vs.kbot[3, 2:-2] = 1           # full-depth columns
+            vs.kbot[4, 3] = 0
+            for k, m in W.masks_from_kbot(vs.kbot, nz, cyclic=False).items():
+                setattr(vs, k, m)
+        vs.K_gm = 300.0 * (0.5 + rng.random((X, Y, nz)))
+        vs.P_diss_skew = rng.standard_normal((X, Y, nz)) if sp["random_aux"] else np.zeros((X, Y, nz))
+
+        rs = _ref_state(st)
+        pre = ref.isoneutral.isoneutral_diffusion_pre(rs)
+        rec = {"in_" + k: np.asarray(getattr(vs, k)) for k in STATE_KEYS}
+        rec.update({"set_" + k: np.asarray(getattr(settings, k)) for k in SETTING_KEYS})
+        rec["set_tau"] = np.asarray(vs.tau)
+        rec["set_taup1"] = np.asarray(vs.taup1)
+        for k, v in pre._asdict().items():
+            rec["pre_" + k] = np.asarray(v)
+        # diffusion runs on the state after pre (its Ai_*/K_* are pre's outputs)
+        for istemp, trn in ((True, "temp"), (False, "salt")):
+            for iso in (True, False):
+                rs2 = _ref_state(st)
+                for k, v in pre._asdict().items():
+                    setattr(rs2.variables, k, np.array(v, copy=True))
+                res = ref.diffusion.isoneutral_diffusion_kernel(rs2, getattr(rs2.variables, trn), istemp, iso)
+                tag = ("iso" if iso else "skew") + "_" + trn
+                for k, v in res._asdict().items():
+                    rec[f"{tag}_{k}"] = np.asarray(v)
+        out[name] = rec
+    return out
+
+
+def main():
+    sys.path.insert(0, str(REPO))
+    ref = _import_reference()
+    for name, rec in tdma_cases(ref).items():
+        np.savez_compressed(HERE / f"tdma_{name}.npz", **rec)
+        print("wrote", f"tdma_{name}.npz")
+    for name, rec in iso_cases(ref).items():
+        np.savez_compressed(HERE / f"{name}.npz", **rec)
+        print("wrote", f"{name}.npz")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,175 @@
+"""Pins the isoneutral oracle: golden vectors from the reference's NumPy path,
+closed forms and conservation/linearity invariants."""
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+
+from conftest import golden_files, load_golden, rel_err, state_from_golden
+from oracle import isoneutral as O
+from veros_amd import workloads as W
+
+PRE_KEYS = ("Ai_ez", "Ai_nz", "Ai_bx", "Ai_by", "K_11", "K_22", "K_33")
+ISO_FILES = golden_files("iso_")
+
+
+@pytest.mark.parametrize("path", ISO_FILES, ids=lambda p: p.stem)
+def test_pre_matches_golden(path):
+    g = load_golden(path)
+    st = state_from_golden(g)
+    out = O.isoneutral_diffusion_pre(st)
+    for k in PRE_KEYS:
+        assert rel_err(getattr(out, k), g["pre_" + k]) <= 1e-15, k
+
+
+@pytest.mark.parametrize("path", ISO_FILES, ids=lambda p: p.stem)
+@pytest.mark.parametrize("iso", [True, False], ids=["iso", "skew"])
+@pytest.mark.parametrize("trn", ["temp", "salt"])
+def test_diffusion_matches_golden(path, iso, trn):
+    g = load_golden(path)
+    st = state_from_golden(g)
+    for k in PRE_KEYS:
+        setattr(st.variables, k, g["pre_" + k])
+    istemp = trn == "temp"
+    out = O.isoneutral_diffusion_kernel(st, getattr(st.variables, trn), istemp, iso=iso)
+    tag = ("iso" if iso else "skew") + "_" + trn
+    keys = [k[len(tag) + 1:] for k in g if k.startswith(tag + "_")]
+    assert set(keys) == set(out._fields)
+    for k in keys:
+        assert rel_err(getattr(out, k), g[f"{tag}_{k}"]) <= 1e-14, k
+
+
+def _cartesian_state(nx=10, ny=8, nz=9, t_fn=None, s_fn=None, k_iso=1000.0):
+    """All-wet, uniform Cartesian grid (cost = cosu = 1)."""
+    st = W.MixSettings(nx=nx, ny=ny, nz=nz, coord_degree=False, enable_cyclic_x=False,
+                       K_iso_steep=50.0)
+    rng = np.random.default_rng(0)
+    state = W.iso_state(rng, nx, ny, nz, dx_deg=2e4, dy_deg=2e4, y_origin=0.0, dz_top=50.0,
+                        dz_bottom=50.0, land_frac=0.0, cyclic=False, settings=st, k_iso=k_iso)
+    vs = state.variables
+    X, Y = nx + 4, ny + 4
+    vs.kbot = np.ones((X, Y), dtype=np.int32)
+    for k, m in W.masks_from_kbot(vs.kbot, nz, cyclic=False).items():
+        setattr(vs, k, m)
+    xt = np.cumsum(vs.dxt)[:, None, None]
+    yt = np.cumsum(vs.dyt)[None, :, None]
+    zt = vs.zt[None, None, :]
+    T = t_fn(xt, yt, zt) * np.ones((X, Y, nz))
+    S = s_fn(xt, yt, zt) * np.ones((X, Y, nz))
+    vs.temp = np.repeat(T[..., None], 3, axis=-1)
+    vs.salt = np.repeat(S[..., None], 3, axis=-1)
+    return state
+
+
+def test_pre_uniform_tracers_closed_form():
+    state = _cartesian_state(t_fn=lambda x, y, z: 12.0 + 0 * x, s_fn=lambda x, y, z: 35.0 + 0 * x)
+    vs, st = state.variables, state.settings
+    out = O.isoneutral_diffusion_pre(state)
+    nz = vs.K_iso.shape[2]
+    for k in ("Ai_ez", "Ai_nz"):
+        assert np.all(getattr(out, k)[1:-2, 1:-2, 1:] == 0)
+    assert np.all(out.Ai_bx[2:-2, 2:-2, :-1] == 0)
+    assert np.all(out.K_33 == 0)
+    t0 = 0.5 * (1 + np.tanh(st.iso_slopec / st.iso_dslope))
+    dzw = vs.dzw
+    wsum = 2 * dzw + 2 * np.concatenate([[0.0], dzw[:-1]])
+    expect = wsum * max(st.K_iso_steep, 1000.0 * t0) / (4 * vs.dzt)
+    np.testing.assert_allclose(out.K_11[2:-2, 2:-2, :], np.broadcast_to(expect, (vs.K_iso.shape[0] - 4,
+                                                                                  vs.K_iso.shape[1] - 4, nz)),
+                               rtol=1e-14)
+    np.testing.assert_allclose(out.K_22[2:-2, 2:-3, :], out.K_11[2:-2, 2:-3, :], rtol=1e-14)
+
+
+def test_pre_linear_stratification_closed_form():
+    ax, ay, az = 2e-6, -1e-6, 4e-3
+    state = _cartesian_state(t_fn=lambda x, y, z: 10 + ax * x + ay * y + az * z,
+                             s_fn=lambda x, y, z: 35.0 + 0 * x)
+    vs, st = state.variables, state.settings
+    out = O.isoneutral_diffusion_pre(state)
+    sx = -ax / az
+    sy = -ay / az
+    tx = 0.5 * (1 + np.tanh((st.iso_slopec - abs(sx)) / st.iso_dslope))
+    ty = 0.5 * (1 + np.tanh((st.iso_slopec - abs(sy)) / st.iso_dslope))
+    inner = (slice(3, -3), slice(3, -3), slice(1, -1))
+    for ip in (0, 1):
+        for kr in (0, 1):
+            np.testing.assert_allclose(out.Ai_ez[inner + (ip, kr)], tx * sx, rtol=1e-9)
+            np.testing.assert_allclose(out.Ai_bx[inner + (ip, kr)], tx * sx, rtol=1e-9)
+            np.testing.assert_allclose(out.Ai_nz[inner + (ip, kr)], ty * sy, rtol=1e-9)
+            np.testing.assert_allclose(out.Ai_by[inner + (ip, kr)], ty * sy, rtol=1e-9)
+    k33 = 1000.0 * (tx * sx ** 2 + ty * sy ** 2)
+    np.testing.assert_allclose(out.K_33[inner], k33, rtol=1e-9)
+
+
+def test_pre_unstable_stratification_tapers_to_zero():
+    """drho/dz >= 0: the denominator is -1e-20, slopes are huge and the taper is
+    exactly 0, so Ai_* vanish and K_11 falls back to K_iso_steep."""
+    state = _cartesian_state(t_fn=lambda x, y, z: 10 + 1e-6 * x - 4e-3 * z,
+                             s_fn=lambda x, y, z: 35.0 + 0 * x)
+    vs, st = state.variables, state.settings
+    out = O.isoneutral_diffusion_pre(state)
+    inner = (slice(3, -3), slice(3, -3), slice(1, -1))
+    assert np.all(out.Ai_ez[inner] == 0)
+    assert np.all(out.Ai_bx[inner] == 0)
+    wsum = 2 * vs.dzw + 2 * np.concatenate([[0.0], vs.dzw[:-1]])
+    np.testing.assert_allclose(out.K_11[inner], np.broadcast_to(
+        (wsum * st.K_iso_steep / (4 * vs.dzt))[1:-1], out.K_11[inner].shape), rtol=1e-14)
+
+
+def _closed_random_state(seed=4, nx=11, ny=9, nz=7):
+    rng = np.random.default_rng(seed)
+    st = W.MixSettings(nx=nx, ny=ny, nz=nz, enable_cyclic_x=False)
+    state = W.iso_state(rng, nx, ny, nz, dx_deg=1.0, dy_deg=1.0, y_origin=-40.0, dz_top=10.0,
+                        dz_bottom=30.0, land_frac=0.2, cyclic=False, settings=st, stratified=False,
+                        random_k_iso=True, jitter=0.02, tau=0, taup1=1)
+    pre = O.isoneutral_diffusion_pre(state)
+    for k, v in pre._asdict().items():
+        setattr(state.variables, k, v)
+    return state
+
+
+def test_uniform_tracer_is_a_fixed_point():
+    state = _closed_random_state()
+    vs = state.variables
+    vs.temp = np.full(vs.temp.shape, 7.5)
+    out = O.isoneutral_diffusion_kernel(state, vs.temp, True)
+    np.testing.assert_allclose(out.temp[2:-2, 2:-2, :, vs.taup1], 7.5, rtol=1e-14)
+    np.testing.assert_allclose(out.dtemp_iso, 0.0, atol=1e-17)
+
+
+def test_explicit_step_conserves_content():
+    state = _closed_random_state()
+    vs = state.variables
+    fe, fn, ft = O.calc_tracer_fluxes(state, vs.temp, vs.K_iso, 0.0)
+    dtr = O.calc_explicit_part(state, fe, fn, ft)
+    vol = vs.cost[None, :, None] * vs.dxt[:, None, None] * vs.dyt[None, :, None] * vs.dzt[None, None, :]
+    total = np.sum(dtr * vol * vs.maskT)
+    scale = np.sum(np.abs(dtr) * vol)
+    assert abs(total) <= 1e-12 * scale
+
+
+def test_implicit_step_conserves_column_content():
+    state = _closed_random_state()
+    vs = state.variables
+    tr = vs.temp
+    new = O.calc_implicit_part(state, tr)
+    old = tr[2:-2, 2:-2, :, vs.taup1]
+    mT = vs.maskT[2:-2, 2:-2]
+    lhs = np.sum(new * mT * vs.dzt, axis=-1)
+    rhs = np.sum(old * mT * vs.dzt, axis=-1)
+    np.testing.assert_allclose(lhs, rhs, rtol=1e-13)
+
+
+def test_step_is_linear_in_tracer():
+    state = _closed_random_state()
+    vs = state.variables
+    vs.dtemp_iso = np.zeros_like(vs.dtemp_iso)
+    state.settings.enable_conserve_energy = False
+    rng = np.random.default_rng(9)
+    u = rng.standard_normal(vs.temp.shape)
+    v = rng.standard_normal(vs.temp.shape)
+    fu = O.isoneutral_diffusion_kernel(state, u, True)
+    fv = O.isoneutral_diffusion_kernel(state, v, True)
+    fw = O.isoneutral_diffusion_kernel(state, 2.0 * u - 3.0 * v, True)
+    np.testing.assert_allclose(fw.temp, 2.0 * fu.temp - 3.0 * fv.temp, atol=1e-11)
+    np.testing.assert_allclose(fw.dtemp_iso, 2.0 * fu.dtemp_iso - 3.0 * fv.dtemp_iso, atol=1e-11)
