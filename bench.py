@@ -1,0 +1,270 @@
+"""Benchmark of the veros mixing hot path on one MI355X.
+
+One step = the kbot-masked TDMA solve (solve_implicit on 1440x720x80
+vertical-diffusion systems) + isoneutral_diffusion_pre + isoneutral_diffusion
+for temp and for salt on the 1440x720x80 quarter-degree grid, inputs resident
+in HBM. Timed with HIP events on the stream the kernels run on.
+
+Prints ONE JSON line (rank 0). ``value`` is the step throughput in Mcells/s
+(interior cells / step time) of the configured precision form; TDMA
+Mcolumns/s and isoneutral Mcells/s are reported beside it for both forms.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nx", type=int, default=1440)
+    ap.add_argument("--ny", type=int, default=720)
+    ap.add_argument("--nz", type=int, default=80)
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--refine", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
+    return ap.parse_args()
+
+
+def build_workload(nx, ny, nz, seed=1002):
+    """Config-2 inputs built on the device (same recipe as
+    veros_amd.workloads.iso_config2 / tdma_config2, drawn with torch RNG)."""
+    import math
+
+    import numpy as np
+    import torch
+
+    from veros_amd import workloads as W
+    from veros_amd.state import to_device_state
+
+    dev = torch.device("cuda", 0)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    X, Y = nx + 4, ny + 4
+    rng = np.random.default_rng(seed)
+    settings = W.config2_settings(nx, ny, nz)
+    grid = W.make_grid(nx, ny, nz, dx_deg=0.25, dy_deg=0.25, y_origin=-89.75, dz_top=10.0, dz_bottom=250.0)
+    kbot = W.make_kbot(rng, nx, ny, nz, kmin=min(5, nz), land_frac=0.35, cyclic=True)
+    f64 = dict(dtype=torch.float64, device=dev)
+    zt = torch.tensor(grid["zt"], **f64)
+    T1 = 20.0 * torch.exp(zt / 800.0)[None, None, :] + 0.1 * torch.randn((X, Y, nz), generator=g, **f64)
+    S1 = 35.0 + 0.3 * torch.randn((X, Y, nz), generator=g, **f64)
+    for F in (T1, S1):
+        F[-2:] = F[2:4].clone()
+        F[:2] = F[-4:-2].clone()
+    vs = dict(tau=0, taup1=1, kbot=torch.from_numpy(kbot).to(dev))
+    vs["temp"] = T1[..., None].repeat(1, 1, 1, 3).contiguous()
+    vs["salt"] = S1[..., None].repeat(1, 1, 1, 3).contiguous()
+    del T1, S1
+    absz = zt.abs()[None, None, :, None]
+    vs["int_drhodT"] = (absz * settings.drhodT).expand(X, Y, nz, 3).contiguous()
+    vs["int_drhodS"] = (absz * settings.drhodS).expand(X, Y, nz, 3).contiguous()
+    vs["K_iso"] = torch.full((X, Y, nz), 1000.0, **f64)
+    for k in ("Ai_ez", "Ai_nz", "Ai_bx", "Ai_by"):
+        vs[k] = torch.zeros((X, Y, nz, 2, 2), **f64)
+    for k in ("K_11", "K_22", "K_33", "dtemp_iso", "dsalt_iso", "P_diss_iso"):
+        vs[k] = torch.zeros((X, Y, nz), **f64)
+    for k, v in grid.items():
+        vs[k] = torch.tensor(v, **f64)
+    from types import SimpleNamespace
+    state = SimpleNamespace(variables=SimpleNamespace(**vs), settings=settings)
+
+    # TDMA systems: a = c = -dt*kappa/dz^2, kappa ~ LogU(1e-5, 1e-2), b = 1 - a - c, d ~ N(0, 1)
+    dz = torch.tensor(W.geometric_dz(nz, 10.0, 250.0), **f64)
+    lo, hi = math.log(1e-5), math.log(1e-2)
+    kappa = torch.exp(lo + (hi - lo) * torch.rand((nx, ny, nz), generator=g, **f64))
+    a = -3600.0 * kappa / dz[None, None, :] ** 2
+    del kappa
+    c = a.clone()
+    b = 1.0 - a - c
+    d = torch.randn((nx, ny, nz), generator=g, **f64)
+    tk = torch.from_numpy(rng.integers(min(5, nz), nz + 1, size=(nx, ny)).astype(np.int32))
+    tk[torch.from_numpy(rng.random((nx, ny)) < 0.35)] = 0
+    tdma = dict(a=a, b=b, c=c, d=d, kbot=tk.to(dev), x=torch.empty_like(d))
+    return state, tdma
+
+
+def tdma_bytes(kbot, nz):
+    import torch
+    kb = kbot.to(torch.int64)
+    wet = torch.where((kb >= 1) & (kb <= nz), nz - kb + 1, torch.zeros_like(kb))
+    W = int(wet.sum().item())
+    C = kb.numel()
+    return 32 * W + 8 * C * nz + 4 * C, W
+
+
+def cpu_baseline(nx, ny, nz, budget_s):
+    """The oracle (NumPy float64; dgtsv for the TDMA) on a bounded sample of
+    the same workload: a 1440 x ny_s x 80 TDMA block and a 360 x ny_s x 80
+    isoneutral block, sized from a short calibration run."""
+    import numpy as np
+
+    from oracle import isoneutral as OI
+    from oracle import tdma as OT
+    from veros_amd import workloads as W
+
+    def time_iso(nys):
+        st = W.iso_config2(nx=360, ny=nys, nz=nz)
+        st.settings = W.config2_settings(360, nys, nz)
+        t0 = time.perf_counter()
+        pre = OI.isoneutral_diffusion_pre(st)
+        for k, v in pre._asdict().items():
+            setattr(st.variables, k, v)
+        OI.isoneutral_diffusion(st, "temp", True)
+        OI.isoneutral_diffusion(st, "salt", False)
+        return time.perf_counter() - t0, 360 * nys * nz
+
+    def time_tdma(nys):
+        p = W.tdma_config2(nx=nx, ny=nys, nz=nz)
+        t0 = time.perf_counter()
+        OT.solve_implicit_kbot(p.a, p.b, p.c, p.d, p.kbot)
+        return time.perf_counter() - t0, nx * nys
+
+    t_iso, n_iso = time_iso(4)
+    t_td, n_td = time_tdma(8)
+    scale_iso = max(1, int((budget_s * 0.7) / max(t_iso, 1e-3) * 4))
+    scale_td = max(1, int((budget_s * 0.3) / max(t_td, 1e-3) * 8))
+    t_iso, n_iso = time_iso(min(scale_iso, 720))
+    t_td, n_td = time_tdma(min(scale_td, 720))
+    iso_rate = n_iso / t_iso / 1e6
+    td_rate = n_td / t_td / 1e6
+    # step-equivalent: per interior cell, the TDMA costs (1/nz) columns
+    N = nx * ny * nz
+    t_step = N / (iso_rate * 1e6) + (nx * ny) / (td_rate * 1e6)
+    return dict(value=N / t_step / 1e6, unit="Mcells/s", cores=1, kind="port",
+                sample=f"oracle (NumPy float64, LAPACK dgtsv) on a 360x{min(scale_iso, 720)}x{nz} isoneutral "
+                       f"block ({t_iso:.1f} s) and a {nx}x{min(scale_td, 720)}x{nz} TDMA block ({t_td:.1f} s)",
+                tdma_mcolumns_per_s=td_rate, iso_mcells_per_s=iso_rate,
+                host=dict(cpu_count=os.cpu_count(), affinity=len(os.sched_getaffinity(0)),
+                          cpu=platform.processor() or platform.machine(), numpy=np.__version__))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build targets one GPU; --gpus above 1 is not supported "
+                                   "(columns shard with no communication, see DESIGN.md)"}))
+        sys.exit(2)
+    import torch
+
+    from veros_amd import isoneutral, utilities
+
+    torch.cuda.set_device(0)
+    nx, ny, nz = args.nx, args.ny, args.nz
+    N = nx * ny * nz
+    print(f"[bench] building {nx}x{ny}x{nz} workload", file=sys.stderr, flush=True)
+    state, td = build_workload(nx, ny, nz)
+    torch.cuda.synchronize()
+    vs = state.variables
+    stream = torch.cuda.current_stream()
+
+    def run_tdma(prec):
+        utilities.solve_implicit_kbot(td["a"], td["b"], td["c"], td["d"], td["kbot"], precision=prec,
+                                      refine_steps=args.refine, out=td["x"])
+
+    def run_pre():
+        isoneutral.isoneutral_diffusion_pre(state)
+
+    def run_diff():
+        isoneutral.isoneutral_diffusion(state, vs.temp, True)
+        isoneutral.isoneutral_diffusion(state, vs.salt, False)
+
+    def timed(fn, reps):
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+        for s, e in evs:
+            s.record(stream)
+            fn()
+            e.record(stream)
+        torch.cuda.synchronize()
+        ts = sorted(s.elapsed_time(e) for s, e in evs)
+        return ts[len(ts) // 2], ts[0]
+
+    # per-part timings (median, min) in ms
+    parts = {}
+    for name, fn in (("tdma_fp64", lambda: run_tdma("fp64")), ("tdma_mixed", lambda: run_tdma("mixed")),
+                     ("iso_pre", run_pre), ("iso_diffusion_TS", run_diff)):
+        for _ in range(args.warmup):
+            fn()
+        parts[name] = timed(fn, max(5, args.steps))
+        print(f"[bench] {name}: median {parts[name][0]:.3f} ms, min {parts[name][1]:.3f} ms",
+              file=sys.stderr, flush=True)
+
+    prec = args.precision
+
+    def step():
+        run_tdma(prec)
+        run_pre()
+        run_diff()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    for _ in range(args.steps):
+        step()
+    t1.record(stream)
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / args.steps
+
+    B_td, W_td = tdma_bytes(td["kbot"], nz)
+    C = nx * ny
+    B_pre = 176 * N + 4 * (nx + 4) * (ny + 4)
+    B_diff = 2 * (224 * N + 4 * (nx + 4) * (ny + 4))
+    rates = {
+        "tdma_mcolumns_per_s": {p: C / (parts[f"tdma_{p}"][0] * 1e-3) / 1e6 for p in ("fp64", "mixed")},
+        "iso_mcells_per_s": N / ((parts["iso_pre"][0] + parts["iso_diffusion_TS"][0]) * 1e-3) / 1e6,
+    }
+    kern = {
+        "tdma_fp64": (B_td, parts["tdma_fp64"][0]),
+        "tdma_mixed": (B_td, parts["tdma_mixed"][0]),
+        "iso_pre": (B_pre, parts["iso_pre"][0]),
+        "iso_diffusion_TS": (B_diff, parts["iso_diffusion_TS"][0]),
+    }
+    gbs = {k: b / (t * 1e-3) / 1e9 for k, (b, t) in kern.items()}
+    dom = max(("iso_pre", "iso_diffusion_TS", f"tdma_{prec}"), key=lambda k: kern[k][1])
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(gbs[dom], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs[dom] / HBM_PEAK_GBS, 4), "traffic": None,
+            "algorithmic_bytes_per_launch": kern[dom][0], "avg_launch_ms": kern[dom][1]}
+
+    out = {
+        "metric": "TDMA Mcolumns/s + isoneutral Mcells/s; mixed vs FP64 GPU at <=1e-12 rel err",
+        "value": round(N / (ms * 1e-3) / 1e6, 2),
+        "unit": "Mcells/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if prec == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"mixed-precision TDMA + isoneutral step, {nx}x{ny}x{nz} quarter-degree grid "
+                               f"(35% land, kbot U{{5..80}})", "nx": nx, "ny": ny, "nz": nz,
+                   "precision": prec, "refine_steps": args.refine, "tdma_wet_cells": W_td},
+        "tdma_mcolumns_per_s": {k: round(v, 1) for k, v in rates["tdma_mcolumns_per_s"].items()},
+        "iso_mcells_per_s": round(rates["iso_mcells_per_s"], 1),
+        "mixed_over_fp64_tdma": round(parts["tdma_fp64"][0] / parts["tdma_mixed"][0], 3),
+        "kernel_ms_median": {k: round(v[0], 4) for k, v in parts.items()},
+        "kernel_algorithmic_GBps": {k: round(v, 1) for k, v in gbs.items()},
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        print("[bench] CPU baseline (oracle on host cores)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(nx, ny, nz, args.cpu_budget)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,148 @@
+/*
+ * vxmix.h — C ABI of libvxmix.so, the gfx950 (MI355X) implementation of the
+ * veros per-timestep mixing hot path:
+ *   - the kbot-masked batched tridiagonal solve behind solve_implicit, and
+ *   - the isoneutral triad-tensor stencils isoneutral_diffusion_pre /
+ *     isoneutral_diffusion.
+ *
+ * Conventions shared by every entry point
+ *   - All array pointers are DEVICE pointers (hipMalloc'd or equivalent).
+ *   - 3-D fields are (x, y, z) row-major with z contiguous (stride 1); the
+ *     x and y strides are given in ELEMENTS. Level k = 0 is the deepest level.
+ *   - kbot is int32, 1-based index of the deepest wet level; 0 means land.
+ *     A column's wet cells are the suffix k in [kbot-1, nz).
+ *   - `stream` is a hipStream_t passed as void* (NULL = the default stream).
+ *     Work is enqueued asynchronously; no entry point synchronises the host.
+ *   - Return value: VX_OK (0) or a VX_ERR_* code; vx_last_error() returns a
+ *     message for the calling thread's most recent failure. No entry point
+ *     exits the process or throws across the ABI.
+ *   - Entry points allocate nothing unless stated; workspaces are
+ *     caller-provided.
+ */
+#ifndef VXMIX_H
+#define VXMIX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    VX_OK = 0,
+    VX_ERR_INVALID_ARGUMENT = 1, /* null pointer, bad shape/stride, kbot out of range */
+    VX_ERR_UNSUPPORTED = 2,      /* e.g. nz above the compiled maximum */
+    VX_ERR_LAUNCH = 3,           /* the HIP runtime reported an error */
+};
+
+/* Library version and limits. */
+const char* vx_version(void);
+int32_t vx_max_nz(void);
+/* Message for this thread's most recent non-zero return. */
+const char* vx_last_error(void);
+
+/*
+ * Batched kbot-masked tridiagonal solve, float64 throughout.
+ *
+ * Replaces solve_implicit(a, b, c, d, water_mask, edge_mask, b_edge, d_edge)
+ * (veros/core/utilities.py:51-59) with its solver
+ * solve_tridiagonal_numpy (veros/core/operators.py:60-77) /
+ * solve_tridiagonal_jax + the tdma custom call (veros/core/operators.py:102-156,
+ * veros/core/special/tdma_.py:49-68), i.e. the XLA targets tdma_cython_double
+ * (veros/core/special/tdma_cython_.pyx:49-83) and tdma_cuda_double
+ * (veros/core/special/cuda_tdma_kernels.cu:81-101).
+ *
+ * Per column (i, j) with kb = kbot[i*kbot_sx + j*kbot_sy] > 0, the rows
+ * k = kb-1 .. nz-1 form one system: the sub-diagonal is taken as 0 at
+ * k = kb-1 (the edge), the super-diagonal as 0 at k = nz-1; at the edge b and
+ * d are replaced by b_edge / d_edge when those are non-NULL. x is written on
+ * every level of every column, 0 outside the wet suffix. a, b, c, d, b_edge,
+ * d_edge and x share the strides (sx, sy, 1) and shape (nx, ny, nz). x may
+ * alias d (in-place solve).
+ */
+int vx_tdma_f64(void* stream,
+                const double* a, const double* b, const double* c, const double* d,
+                const double* b_edge, const double* d_edge,
+                const int32_t* kbot, int64_t kbot_sx, int64_t kbot_sy,
+                double* x,
+                int64_t nx, int64_t ny, int32_t nz, int64_t sx, int64_t sy);
+
+/*
+ * Same system and layout as vx_tdma_f64, solved by float32 Thomas sweeps with
+ * `refine_steps` (0..2) steps of iterative refinement whose residual
+ * r = d - A x is formed in float64 and whose correction is accumulated into a
+ * float64 solution. Inputs and output are float64.
+ */
+int vx_tdma_mixed(void* stream,
+                  const double* a, const double* b, const double* c, const double* d,
+                  const double* b_edge, const double* d_edge,
+                  const int32_t* kbot, int64_t kbot_sx, int64_t kbot_sy,
+                  double* x,
+                  int64_t nx, int64_t ny, int32_t nz, int64_t sx, int64_t sy,
+                  int32_t refine_steps);
+
+/* ------------------------------------------------------------------------
+ * Isoneutral stencils. Arrays are full veros state arrays including the
+ * 2-cell halos: X = nx + 4, Y = ny + 4, each C-contiguous.
+ *   temp/salt/tracer/int_drhodX : (X, Y, nz, 3)  time level innermost
+ *   K_iso, K_11, K_22, K_33, dtr_iso, P_diss : (X, Y, nz)
+ *   Ai_ez, Ai_nz, Ai_bx, Ai_by   : (X, Y, nz, 2, 2)
+ *   kbot : (X, Y) int32;  dxt, dxu : (X);  dyt, dyu, cost, cosu : (Y);
+ *   dzt, dzw : (nz)
+ * Masks are not passed: maskT/U/V/W are derived from kbot in the kernels the
+ * way calc_topo_kernel builds them (veros/core/numerics.py:195-221), which
+ * requires the halo columns of kbot to be consistent (cyclic copy or 0).
+ * ------------------------------------------------------------------------ */
+
+typedef struct vx_grid {
+    const int32_t* kbot;
+    const double *dxt, *dxu, *dyt, *dyu, *cost, *cosu, *dzt, *dzw;
+    int64_t nx, ny;
+    int32_t nz;
+    int32_t _pad;
+} vx_grid;
+
+typedef struct vx_iso_params {
+    double drhodT, drhodS;   /* linear-EOS derivatives (eq_of_state_type 1) */
+    double K_iso_steep, iso_slopec, iso_dslope;
+    double dt_tracer, grav, rho_0;
+} vx_iso_params;
+
+/*
+ * isoneutral_diffusion_pre (veros/core/isoneutral/isoneutral.py:18-229).
+ * Reads temp/salt at time level `tau` and K_iso; writes Ai_ez, Ai_nz, Ai_bx,
+ * Ai_by, K_11, K_22, K_33 in place over exactly the index ranges the
+ * reference writes (all other entries keep their values), including
+ * K_33[..., nz-1] = 0 over the whole plane.
+ */
+int vx_iso_pre_f64(void* stream, const vx_grid* grid, const vx_iso_params* prm,
+                   const double* temp, const double* salt, int32_t tau,
+                   const double* K_iso,
+                   double* Ai_ez, double* Ai_nz, double* Ai_bx, double* Ai_by,
+                   double* K_11, double* K_22, double* K_33);
+
+/*
+ * isoneutral_diffusion for one tracer (veros/core/isoneutral/diffusion.py:172-295,
+ * iso=True), in place: tracer[..., taup1] receives the explicit flux
+ * divergence times dt_tracer followed by the implicit K_33 vertical solve,
+ * dtr_iso accumulates both tendencies, and, when P_diss is non-NULL
+ * (enable_conserve_energy), P_diss accumulates the dissipation of dynamic
+ * enthalpy computed from int_drhodX[..., tau] (veros/core/diffusion.py:9-62).
+ * `workspace` must hold at least vx_iso_workspace_bytes(grid) bytes.
+ */
+int64_t vx_iso_workspace_bytes(const vx_grid* grid);
+int vx_iso_diffusion_f64(void* stream, const vx_grid* grid, const vx_iso_params* prm,
+                         double* tracer, int32_t tau, int32_t taup1,
+                         const double* K_iso,
+                         const double* Ai_ez, const double* Ai_nz,
+                         const double* Ai_bx, const double* Ai_by,
+                         const double* K_11, const double* K_22, const double* K_33,
+                         double* dtr_iso,
+                         const double* int_drhodX, double* P_diss,
+                         void* workspace);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VXMIX_H */

@@ -1,0 +1,73 @@
+"""CPU checks of the C ABI: the library loads, exports every symbol the header
+declares, and rejects bad arguments before touching the GPU."""
+import re
+from pathlib import Path
+
+import pytest
+
+from veros_amd import _native
+
+HEADER = Path(__file__).resolve().parent.parent / "include" / "vxmix.h"
+
+
+def declared_symbols():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(vx_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_loads_and_reports():
+    L = _native.lib()
+    assert _native.version().startswith("vxmix")
+    assert _native.max_nz() >= 100
+    assert L is _native.lib()
+
+
+def test_every_declared_symbol_is_exported_and_bound():
+    L = _native.lib()
+    names = declared_symbols()
+    assert len(names) >= 8
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(_native.exported_symbols())
+
+
+def test_tdma_argument_errors_without_gpu():
+    L = _native.lib()
+    args = dict(a=1, b=1, c=1, d=1, be=None, de=None, kb=1, kx=1, ky=1, x=1, nx=2, ny=2, nz=4, sx=8, sy=4)
+
+    def call(**kw):
+        v = {**args, **kw}
+        return L.vx_tdma_f64(None, v["a"], v["b"], v["c"], v["d"], v["be"], v["de"], v["kb"], v["kx"], v["ky"],
+                             v["x"], v["nx"], v["ny"], v["nz"], v["sx"], v["sy"])
+
+    assert call(nz=0) == _native.VX_ERR_INVALID_ARGUMENT
+    assert call(nz=_native.max_nz() + 1) == _native.VX_ERR_UNSUPPORTED
+    assert call(nx=-1) == _native.VX_ERR_INVALID_ARGUMENT
+    assert call(a=None) == _native.VX_ERR_INVALID_ARGUMENT
+    assert call(sy=2) == _native.VX_ERR_INVALID_ARGUMENT      # columns would overlap
+    assert "vx_tdma_f64" in L.vx_last_error().decode()
+    assert call(nx=0) == _native.VX_OK                        # nothing to do: no launch
+    with pytest.raises(ValueError):
+        _native.check(call(nz=0), "t")
+    with pytest.raises(NotImplementedError):
+        _native.check(call(nz=999), "t")
+    v = {**args}
+    rc = L.vx_tdma_mixed(None, v["a"], v["b"], v["c"], v["d"], None, None, v["kb"], 1, 1, v["x"], 2, 2, 4, 8, 4, 3)
+    assert rc == _native.VX_ERR_INVALID_ARGUMENT
+
+
+def test_iso_argument_errors_without_gpu():
+    L = _native.lib()
+    g = _native.VxGrid()
+    p = _native.VxIsoParams()
+    g.nx, g.ny, g.nz = 4, 4, 1
+    assert L.vx_iso_pre_f64(None, g, p, 1, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1) == _native.VX_ERR_INVALID_ARGUMENT
+    g.nz = 8
+    assert L.vx_iso_pre_f64(None, g, p, 1, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1) == _native.VX_ERR_INVALID_ARGUMENT  # null grid arrays
+    for name in ("kbot", "dxt", "dxu", "dyt", "dyu", "cost", "cosu", "dzt", "dzw"):
+        setattr(g, name, 16)
+    assert L.vx_iso_pre_f64(None, g, p, 1, 1, 5, 1, 1, 1, 1, 1, 1, 1, 1) == _native.VX_ERR_INVALID_ARGUMENT  # tau
+    assert L.vx_iso_workspace_bytes(g) == 3 * 8 * 8 * 8 * 8
+    rc = L.vx_iso_diffusion_f64(None, g, p, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, None, 1, 1)
+    assert rc == _native.VX_ERR_INVALID_ARGUMENT  # P_diss without int_drhodX

@@ -1,0 +1,128 @@
+"""GPU parity of the isoneutral entry points against the oracle and the
+reference golden vectors (bar: 1e-12 relative per output,
+max|x - x_ref| / max|x_ref| over the whole array, halos included)."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import golden_files, load_golden, rel_err, state_from_golden
+from oracle import isoneutral as O
+from veros_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+PRE_KEYS = ("Ai_ez", "Ai_nz", "Ai_bx", "Ai_by", "K_11", "K_22", "K_33")
+
+
+def _dev_state(st):
+    from veros_amd.state import to_device_state
+    return to_device_state(st)
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.mark.parametrize("path", golden_files("iso_"), ids=lambda p: p.stem)
+def test_pre_golden(path):
+    from veros_amd import isoneutral
+    g = load_golden(path)
+    ds = _dev_state(state_from_golden(g))
+    out = isoneutral.isoneutral_diffusion_pre(ds)
+    for k in PRE_KEYS:
+        assert rel_err(_np(getattr(out, k)), g["pre_" + k]) <= TOL, k
+
+
+@pytest.mark.parametrize("path", golden_files("iso_"), ids=lambda p: p.stem)
+@pytest.mark.parametrize("trn", ["temp", "salt"])
+def test_diffusion_golden(path, trn):
+    from veros_amd import isoneutral
+    g = load_golden(path)
+    st = state_from_golden(g)
+    for k in PRE_KEYS:
+        setattr(st.variables, k, g["pre_" + k])
+    ds = _dev_state(st)
+    istemp = trn == "temp"
+    out = isoneutral.isoneutral_diffusion_kernel(ds, getattr(ds.variables, trn), istemp)
+    tag = "iso_" + trn
+    keys = [k[len(tag) + 1:] for k in g if k.startswith(tag + "_")]
+    assert set(keys) == set(out._fields)
+    for k in keys:
+        assert rel_err(_np(getattr(out, k)), g[f"{tag}_{k}"]) <= TOL, k
+
+
+def _random_state(seed, nx, ny, nz, **kw):
+    rng = np.random.default_rng(seed)
+    opts = dict(dx_deg=1.0, dy_deg=1.0, y_origin=-50.0, dz_top=10.0, dz_bottom=80.0, land_frac=0.25,
+                stratified=False, random_k_iso=True, jitter=0.02, random_aux=True, tau=1, taup1=0)
+    opts.update(kw)
+    st = W.iso_state(rng, nx, ny, nz, **opts)
+    st.settings.K_iso_steep = 20.0
+    return st
+
+
+@pytest.mark.parametrize("shape,cyclic", [((37, 29, 17), True), ((25, 41, 33), False), ((9, 8, 2), True),
+                                          ((18, 11, 96), True)])
+def test_pre_and_diffusion_random(shape, cyclic):
+    from veros_amd import isoneutral
+    st = _random_state(sum(shape), *shape, cyclic=cyclic)
+    ref = O.isoneutral_diffusion_pre(st)
+    ds = _dev_state(st)
+    out = isoneutral.isoneutral_diffusion_pre(ds)
+    for k in PRE_KEYS:
+        assert rel_err(_np(getattr(out, k)), getattr(ref, k)) <= TOL, k
+    for k, v in ref._asdict().items():
+        setattr(st.variables, k, v)
+    for energy in (True, False):
+        st.settings.enable_conserve_energy = energy
+        for istemp, trn in ((True, "temp"), (False, "salt")):
+            r = O.isoneutral_diffusion_kernel(st, getattr(st.variables, trn), istemp)
+            ds = _dev_state(st)
+            o = isoneutral.isoneutral_diffusion_kernel(ds, getattr(ds.variables, trn), istemp)
+            assert set(o._fields) == set(r._fields)
+            for k in r._fields:
+                assert rel_err(_np(getattr(o, k)), getattr(r, k)) <= TOL, (energy, trn, k)
+
+
+def test_config1_step_against_oracle():
+    """The 360x160x60 ACC-like channel: pre, then T and S diffusion, as in
+    thermodynamics (veros/core/thermodynamics.py:426-433)."""
+    from veros_amd import isoneutral
+    st = W.iso_config1()
+    ds = _dev_state(st)
+    ref = O.isoneutral_diffusion_pre(st)
+    out = isoneutral.isoneutral_diffusion_pre(ds)
+    for k in PRE_KEYS:
+        assert rel_err(_np(getattr(out, k)), getattr(ref, k)) <= TOL, k
+        setattr(st.variables, k, getattr(ref, k))
+    O.isoneutral_diffusion(st, "temp", True)
+    O.isoneutral_diffusion(st, "salt", False)
+    isoneutral.isoneutral_diffusion(ds, ds.variables.temp, True)
+    isoneutral.isoneutral_diffusion(ds, ds.variables.salt, False)
+    for k in ("temp", "salt", "dtemp_iso", "dsalt_iso", "P_diss_iso"):
+        assert rel_err(_np(getattr(ds.variables, k)), getattr(st.variables, k)) <= TOL, k
+
+
+def test_deterministic_pre():
+    from veros_amd import isoneutral
+    st = _random_state(3, 30, 20, 12)
+    a = [_np(t).copy() for t in isoneutral.isoneutral_diffusion_pre(_dev_state(st))]
+    b = [_np(t).copy() for t in isoneutral.isoneutral_diffusion_pre(_dev_state(st))]
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_shape_and_dtype_errors():
+    from veros_amd import isoneutral
+    ds = _dev_state(_random_state(5, 10, 8, 6))
+    ds.variables.K_11 = ds.variables.K_11[:, :, :5].contiguous()
+    with pytest.raises(ValueError):
+        isoneutral.isoneutral_diffusion_pre(ds)
+    ds = _dev_state(_random_state(5, 10, 8, 6))
+    ds.variables.K_iso = ds.variables.K_iso.float()
+    with pytest.raises(TypeError):
+        isoneutral.isoneutral_diffusion_pre(ds)
+    ds = _dev_state(_random_state(5, 10, 8, 6))
+    ds.settings.eq_of_state_type = 3
+    with pytest.raises(NotImplementedError):
+        isoneutral.isoneutral_diffusion_pre(ds)

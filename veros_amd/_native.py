@@ -1,0 +1,101 @@
+"""ctypes binding of libvxmix.so (the C ABI declared in include/vxmix.h).
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libvxmix.so"
+
+VX_OK = 0
+VX_ERR_INVALID_ARGUMENT = 1
+VX_ERR_UNSUPPORTED = 2
+VX_ERR_LAUNCH = 3
+
+c_void_p = ctypes.c_void_p
+c_i32 = ctypes.c_int32
+c_i64 = ctypes.c_int64
+c_f64 = ctypes.c_double
+
+
+class VxGrid(ctypes.Structure):
+    _fields_ = [
+        ("kbot", c_void_p),
+        ("dxt", c_void_p), ("dxu", c_void_p), ("dyt", c_void_p), ("dyu", c_void_p),
+        ("cost", c_void_p), ("cosu", c_void_p), ("dzt", c_void_p), ("dzw", c_void_p),
+        ("nx", c_i64), ("ny", c_i64), ("nz", c_i32), ("_pad", c_i32),
+    ]
+
+
+class VxIsoParams(ctypes.Structure):
+    _fields_ = [(name, c_f64) for name in (
+        "drhodT", "drhodS", "K_iso_steep", "iso_slopec", "iso_dslope", "dt_tracer", "grav", "rho_0")]
+
+
+# name -> (restype, argtypes)
+_SIGNATURES = {
+    "vx_version": (ctypes.c_char_p, []),
+    "vx_max_nz": (c_i32, []),
+    "vx_last_error": (ctypes.c_char_p, []),
+    "vx_tdma_f64": (ctypes.c_int, [c_void_p] + [c_void_p] * 6 + [c_void_p, c_i64, c_i64, c_void_p,
+                                                                c_i64, c_i64, c_i32, c_i64, c_i64]),
+    "vx_tdma_mixed": (ctypes.c_int, [c_void_p] + [c_void_p] * 6 + [c_void_p, c_i64, c_i64, c_void_p,
+                                                                  c_i64, c_i64, c_i32, c_i64, c_i64, c_i32]),
+    "vx_iso_pre_f64": (ctypes.c_int, [c_void_p, ctypes.POINTER(VxGrid), ctypes.POINTER(VxIsoParams),
+                                      c_void_p, c_void_p, c_i32, c_void_p] + [c_void_p] * 7),
+    "vx_iso_workspace_bytes": (c_i64, [ctypes.POINTER(VxGrid)]),
+    "vx_iso_diffusion_f64": (ctypes.c_int, [c_void_p, ctypes.POINTER(VxGrid), ctypes.POINTER(VxIsoParams),
+                                            c_void_p, c_i32, c_i32, c_void_p] + [c_void_p] * 7
+                             + [c_void_p, c_void_p, c_void_p, c_void_p]),
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+def exported_symbols():
+    return tuple(_SIGNATURES)
+
+
+def lib():
+    """The loaded library (raises RuntimeError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise RuntimeError(
+                    f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+                    "or `python veros_amd/build.py`; there is no CPU fallback")
+            L = ctypes.CDLL(str(LIB_PATH))
+            for name, (res, args) in _SIGNATURES.items():
+                fn = getattr(L, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = L
+    return _lib
+
+
+def check(rc: int, what: str):
+    """Raise the exception class the reference raises for the same condition."""
+    if rc == VX_OK:
+        return
+    msg = lib().vx_last_error().decode(errors="replace")
+    if rc == VX_ERR_INVALID_ARGUMENT:
+        raise ValueError(f"{what}: {msg}")
+    if rc == VX_ERR_UNSUPPORTED:
+        raise NotImplementedError(f"{what}: {msg}")
+    raise RuntimeError(f"{what}: {msg}")
+
+
+def max_nz() -> int:
+    return int(lib().vx_max_nz())
+
+
+def version() -> str:
+    return lib().vx_version().decode()

@@ -1,0 +1,158 @@
+"""Drop-in for veros.core.isoneutral's hot pair on the GPU.
+
+* ``isoneutral_diffusion_pre(state) -> KernelOutput(Ai_ez, Ai_nz, Ai_bx, Ai_by,
+  K_11, K_22, K_33)``  (veros/core/isoneutral/isoneutral.py:18-229)
+* ``isoneutral_diffusion_kernel(state, tr, istemp, iso=True) -> KernelOutput(temp|salt,
+  dtemp_iso|dsalt_iso[, P_diss_iso])``  (veros/core/isoneutral/diffusion.py:210-283)
+* ``isoneutral_diffusion(state, tr, istemp)``, the routine that stores the
+  kernel's outputs in ``state.variables`` (veros/core/isoneutral/diffusion.py:286-295)
+
+``state`` is any object with ``.variables`` and ``.settings`` holding the veros
+variables and settings these functions read (a ``VerosState`` whose arrays
+are on the GPU, or ``veros_amd.state.to_device_state(...)``). Arrays are the
+full veros arrays with 2-cell halos, C-contiguous, float64, on the GPU.
+
+The outputs are written IN PLACE into the state's arrays over exactly the
+index ranges the reference writes; entries outside those ranges keep their
+values, which is what the reference's copy-and-update yields. The returned
+KernelOutput holds the same tensors.
+
+The equation of state must be the linear one (``eq_of_state_type == 1``,
+whose drho/dT and drho/dS are constants, veros/core/density/get_rho.py:93-132);
+the constants are taken from ``settings.drhodT`` / ``settings.drhodS`` when
+present, else from veros's linear EOS module values
+(veros/core/density/linear_eq.py:10-15,36-43).
+"""
+from __future__ import annotations
+
+from collections import namedtuple
+
+import torch
+
+from . import _native
+from ._arrays import ptr, require_contiguous, stream_handle, workspace
+
+# veros/core/density/linear_eq.py:10-15
+LINEAR_EOS_RHO0 = 1024.0
+LINEAR_EOS_BETA_T = 1.67e-4
+LINEAR_EOS_BETA_S = 0.78e-3
+
+
+def KernelOutput(**kw):
+    """namedtuple built from keyword arguments (veros/state.py:16-20)."""
+    return namedtuple("KernelOutput", list(kw.keys()))(*kw.values())
+
+
+def eos_derivatives(settings):
+    if getattr(settings, "eq_of_state_type", 1) != 1:
+        raise NotImplementedError("only the linear equation of state (eq_of_state_type=1) is supported")
+    drhodT = getattr(settings, "drhodT", -LINEAR_EOS_BETA_T * LINEAR_EOS_RHO0)
+    drhodS = getattr(settings, "drhodS", LINEAR_EOS_BETA_S * LINEAR_EOS_RHO0)
+    return float(drhodT), float(drhodS)
+
+
+def _check_array(name, t, shape, dtype=torch.float64):
+    if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
+        raise TypeError(f"{name} must be a torch tensor on the GPU")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype} (got {t.dtype})")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"unexpected shape for {name} (expected {tuple(shape)}, got {tuple(t.shape)})")
+    require_contiguous(name, t)
+    return t
+
+
+def grid_struct(vs, keep):
+    """vx_grid for the state's grid; ``keep`` collects the tensors it points into."""
+    kbot = vs.kbot
+    X, Y = kbot.shape
+    nz = vs.K_iso.shape[2]
+    kb = kbot if (kbot.dtype == torch.int32 and kbot.is_contiguous()) else kbot.to(torch.int32).contiguous()
+    _check_array("kbot", kb, (X, Y), torch.int32)
+    g = _native.VxGrid()
+    g.kbot = kb.data_ptr()
+    for name, n in (("dxt", X), ("dxu", X), ("dyt", Y), ("dyu", Y), ("cost", Y), ("cosu", Y),
+                    ("dzt", nz), ("dzw", nz)):
+        t = _check_array(name, getattr(vs, name), (n,))
+        setattr(g, name, t.data_ptr())
+        keep.append(t)
+    keep.append(kb)
+    g.nx, g.ny, g.nz = X - 4, Y - 4, nz
+    return g
+
+
+def params_struct(settings):
+    p = _native.VxIsoParams()
+    p.drhodT, p.drhodS = eos_derivatives(settings)
+    p.K_iso_steep = float(settings.K_iso_steep)
+    p.iso_slopec = float(settings.iso_slopec)
+    p.iso_dslope = float(settings.iso_dslope)
+    p.dt_tracer = float(settings.dt_tracer)
+    p.grav = float(settings.grav)
+    p.rho_0 = float(settings.rho_0)
+    return p
+
+
+def isoneutral_diffusion_pre(state):
+    """Triad slopes, tapers and the isoneutral tensor, in place."""
+    vs, st = state.variables, state.settings
+    X, Y, nz = vs.K_iso.shape
+    keep = []
+    g = grid_struct(vs, keep)
+    p = params_struct(st)
+    T = _check_array("temp", vs.temp, (X, Y, nz, 3))
+    S = _check_array("salt", vs.salt, (X, Y, nz, 3))
+    K = _check_array("K_iso", vs.K_iso, (X, Y, nz))
+    outs = {}
+    for name in ("Ai_ez", "Ai_nz", "Ai_bx", "Ai_by"):
+        outs[name] = _check_array(name, getattr(vs, name), (X, Y, nz, 2, 2))
+    for name in ("K_11", "K_22", "K_33"):
+        outs[name] = _check_array(name, getattr(vs, name), (X, Y, nz))
+    rc = _native.lib().vx_iso_pre_f64(stream_handle(), g, p, ptr(T), ptr(S), int(vs.tau), ptr(K),
+                                      *(ptr(outs[n]) for n in ("Ai_ez", "Ai_nz", "Ai_bx", "Ai_by",
+                                                               "K_11", "K_22", "K_33")))
+    _native.check(rc, "isoneutral_diffusion_pre")
+    return KernelOutput(**outs)
+
+
+def isoneutral_diffusion_kernel(state, tr, istemp, iso=True):
+    """One tracer's isoneutral step, in place on ``tr`` and the state's
+    dtemp_iso/dsalt_iso and P_diss_iso."""
+    if not iso:
+        raise NotImplementedError("the skew-diffusion variant (iso=False) is not built yet")
+    vs, st = state.variables, state.settings
+    X, Y, nz = vs.K_iso.shape
+    keep = []
+    g = grid_struct(vs, keep)
+    p = params_struct(st)
+    tr = _check_array("tr", tr, (X, Y, nz, 3))
+    dname = "dtemp_iso" if istemp else "dsalt_iso"
+    dtr = _check_array(dname, getattr(vs, dname), (X, Y, nz))
+    arrays = [_check_array("K_iso", vs.K_iso, (X, Y, nz))]
+    arrays += [_check_array(n, getattr(vs, n), (X, Y, nz, 2, 2)) for n in ("Ai_ez", "Ai_nz", "Ai_bx", "Ai_by")]
+    arrays += [_check_array(n, getattr(vs, n), (X, Y, nz)) for n in ("K_11", "K_22", "K_33")]
+    energy = bool(st.enable_conserve_energy)
+    if energy:
+        I = _check_array("int_drhodT" if istemp else "int_drhodS",
+                         vs.int_drhodT if istemp else vs.int_drhodS, (X, Y, nz, 3))
+        P = _check_array("P_diss_iso", vs.P_diss_iso, (X, Y, nz))
+    else:
+        I = P = None
+    L = _native.lib()
+    nbytes = L.vx_iso_workspace_bytes(g)
+    ws = workspace(nbytes, tr.device)
+    rc = L.vx_iso_diffusion_f64(stream_handle(), g, p, ptr(tr), int(vs.tau), int(vs.taup1),
+                                *(ptr(a) for a in arrays), ptr(dtr), ptr(I), ptr(P), ptr(ws))
+    _native.check(rc, "isoneutral_diffusion")
+    out = {("temp" if istemp else "salt"): tr, dname: dtr}
+    if energy:
+        out["P_diss_iso"] = P
+    return KernelOutput(**out)
+
+
+def isoneutral_diffusion(state, tr, istemp):
+    """The routine: run the kernel and store its outputs in state.variables
+    (veros/core/isoneutral/diffusion.py:286-295)."""
+    out = isoneutral_diffusion_kernel(state, tr, istemp, iso=True)
+    for k, v in out._asdict().items():
+        setattr(state.variables, k, v)
