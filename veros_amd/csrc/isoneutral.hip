@@ -1,10 +1,12 @@
 // Isoneutral triad-tensor stencils for gfx950 (float64).
 //
-//   iso_pre_kernel      isoneutral_diffusion_pre   veros/core/isoneutral/isoneutral.py:18-229
-//   iso_flux_kernel     _calc_tracer_fluxes        veros/core/isoneutral/diffusion.py:9-113
-//   iso_column_kernel   _calc_explicit_part + _calc_implicit_part + tracer update +
-//                       the energy branch          veros/core/isoneutral/diffusion.py:116-283,
+//   iso_pre_kernel       isoneutral_diffusion_pre  veros/core/isoneutral/isoneutral.py:18-229
+//   iso_flux_kernel      _calc_tracer_fluxes       veros/core/isoneutral/diffusion.py:9-113
+//   iso_explicit_kernel  _calc_explicit_part, the explicit tracer update and the lateral
+//                        dissipation               veros/core/isoneutral/diffusion.py:116-139,196-197,
 //                                                  veros/core/diffusion.py:9-62
+//   iso_implicit_kernel  _calc_implicit_part       veros/core/isoneutral/diffusion.py:142-169,202-204
+//   iso_post_kernel      vertical-mixing dissipation veros/core/isoneutral/diffusion.py:254-279
 //
 // Layout: veros state arrays with 2-cell halos, z contiguous, k = 0 deepest.
 // Cell-parallel kernels put consecutive lanes on consecutive levels of one
@@ -45,162 +47,313 @@ struct Fields {
     int tau;
 };
 
-__device__ __forceinline__ void grad_x(const Grid& g, const Fields& f, int i, int j, int k, double& gT, double& gS) {
+// Blocks are dispatched round-robin over the 8 XCDs; remap so that each XCD
+// walks a contiguous range of the domain and neighbouring rows share its L2.
+// `nb` must be a multiple of 8.
+__device__ __forceinline__ int64_t xcd_block(int64_t nb) {
+    const int64_t b = blockIdx.x;
+    return (b & 7) * (nb >> 3) + (b >> 3);
+}
+
+// 1/x from v_rcp_f64 refined by two Newton steps (about 1 ulp; finite
+// non-zero x only).
+__device__ __forceinline__ double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// Masked face gradients (veros/core/isoneutral/isoneutral.py:46-95); `rd` is
+// the reciprocal of the metric factor (dxu*cost, dyu or dzw).
+struct Grad {
+    double T, S;
+};
+
+__device__ __forceinline__ Grad grad_x(const Grid& g, const Fields& f, int i, int j, int k, double rd) {
     const double m = mT(kb_of(g, i, j), k) * mT(kb_of(g, i + 1, j), k);
-    const double den = g.dxu[i] * g.cost[j];
-    gT = m * (tr3(f.T, g, i + 1, j, k, f.tau) - tr3(f.T, g, i, j, k, f.tau)) / den;
-    gS = m * (tr3(f.S, g, i + 1, j, k, f.tau) - tr3(f.S, g, i, j, k, f.tau)) / den;
+    return {m * (tr3(f.T, g, i + 1, j, k, f.tau) - tr3(f.T, g, i, j, k, f.tau)) * rd,
+            m * (tr3(f.S, g, i + 1, j, k, f.tau) - tr3(f.S, g, i, j, k, f.tau)) * rd};
 }
 
-__device__ __forceinline__ void grad_y(const Grid& g, const Fields& f, int i, int j, int k, double& gT, double& gS) {
+__device__ __forceinline__ Grad grad_y(const Grid& g, const Fields& f, int i, int j, int k, double rd) {
     const double m = mT(kb_of(g, i, j), k) * mT(kb_of(g, i, j + 1), k);
-    const double den = g.dyu[j];
-    gT = m * (tr3(f.T, g, i, j + 1, k, f.tau) - tr3(f.T, g, i, j, k, f.tau)) / den;
-    gS = m * (tr3(f.S, g, i, j + 1, k, f.tau) - tr3(f.S, g, i, j, k, f.tau)) / den;
+    return {m * (tr3(f.T, g, i, j + 1, k, f.tau) - tr3(f.T, g, i, j, k, f.tau)) * rd,
+            m * (tr3(f.S, g, i, j + 1, k, f.tau) - tr3(f.S, g, i, j, k, f.tau)) * rd};
 }
 
-// dT/dz at the top face of (i, j, k); 0 at the top level.
-__device__ __forceinline__ void grad_z(const Grid& g, const Fields& f, int i, int j, int k, double& gT, double& gS) {
-    if (k >= g.nz - 1) {
-        gT = 0.0;
-        gS = 0.0;
-        return;
-    }
+// dT/dz at the top face of (i, j, k); 0 at the top level and below k = 0.
+__device__ __forceinline__ Grad grad_z(const Grid& g, const Fields& f, int i, int j, int k, double rd) {
+    if (k < 0 || k >= g.nz - 1) return {0.0, 0.0};
     const int kb = kb_of(g, i, j);
     const double m = mT(kb, k) * mT(kb, k + 1);
-    const double den = g.dzw[k];
-    gT = m * (tr3(f.T, g, i, j, k + 1, f.tau) - tr3(f.T, g, i, j, k, f.tau)) / den;
-    gS = m * (tr3(f.S, g, i, j, k + 1, f.tau) - tr3(f.S, g, i, j, k, f.tau)) / den;
+    return {m * (tr3(f.T, g, i, j, k + 1, f.tau) - tr3(f.T, g, i, j, k, f.tau)) * rd,
+            m * (tr3(f.S, g, i, j, k + 1, f.tau) - tr3(f.S, g, i, j, k, f.tau)) * rd};
 }
 
-__device__ __forceinline__ double taper(const Prm& p, double s) {
-    // dm_taper (veros/core/isoneutral/isoneutral.py:10-15)
-    return 0.5 * (1.0 + tanh((-fabs(s) + p.iso_slopec) / p.iso_dslope));
+// exp(x) for |x| <= 64 by Cody-Waite reduction: x = n ln2 + r with ln2 split
+// into a 32-bit-exact high part and a low part, |r| <= ln2/2, e^r from its
+// Taylor series to degree 13 (truncation < 5e-18 relative), then 2^n.
+// The coefficients are read once per kernel into an ExpCoef (uniform, so
+// they stay in scalar registers and feed v_fma_f64 directly).
+__constant__ double kExpTable[16] = {
+    1.4426950408889634,                       // log2(e)
+    6.93147180369123816490e-01,               // ln2, high part
+    1.90821492927058770002e-10,               // ln2, low part
+    1.0 / 6227020800.0, 1.0 / 479001600.0, 1.0 / 39916800.0, 1.0 / 3628800.0, 1.0 / 362880.0,
+    1.0 / 40320.0, 1.0 / 5040.0, 1.0 / 720.0, 1.0 / 120.0, 1.0 / 24.0, 1.0 / 6.0, 0.5, 1.0};
+
+struct ExpCoef {
+    double c[16];
+};
+
+__device__ __forceinline__ ExpCoef load_exp_coef() {
+    ExpCoef e;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) e.c[q] = kExpTable[q];
+    return e;
 }
 
-// One triad: slope from a horizontal density gradient and a vertical one,
-// both weighted by the same drho/dX at a T point (mask folded in).
-__device__ __forceinline__ double slope(const Prm& p, double mt, double gxT, double gxS, double gzT, double gzS) {
-    const double drdT = mt * p.drhodT;
-    const double drdS = mt * p.drhodS;
-    const double num = drdT * gxT + drdS * gxS;
-    const double den = drdT * gzT + drdS * gzS;
-    return -num / (fmin(0.0, den) - kEpsln);
+__device__ __forceinline__ double exp_reduced(const ExpCoef& e, double x) {
+    const double n = rint(x * e.c[0]);
+    double r = fma(-n, e.c[1], x);
+    r = fma(-n, e.c[2], r);
+    double q = e.c[3];
+#pragma unroll
+    for (int t = 4; t < 16; ++t) q = fma(q, r, e.c[t]);
+    q = fma(q, r, 1.0);
+    return ldexp(q, (int)n);
 }
+
+// dm_taper (veros/core/isoneutral/isoneutral.py:10-15):
+//   0.5*(1 + tanh(u)) == 1/(1 + exp(-2u)),  u = (slopec - |s|)/dslope.
+// In float64 tanh(u) rounds to exactly -1 / +1 for |u| > 55*ln(2)/2 (~19.06),
+// which makes the reference taper exactly 0 / 1 there; those branches are
+// taken without evaluating the exponential. `two_over_dslope` = 2/dslope.
+__device__ __forceinline__ double taper(const Prm& p, const ExpCoef& ec, double two_over_dslope, double s) {
+    const double x = (fabs(s) - p.iso_slopec) * two_over_dslope;  // -2u
+    if (x > 38.12) return 0.0;
+    if (x < -38.12) return 1.0;
+    return rcp_nr(1.0 + exp_reduced(ec, x));
+}
+
+// Horizontal and vertical density gradients of one triad, weighted by the
+// drho/dX of one T point (the 0/1 mask is folded into the weights).
+struct Dens {
+    double drdT, drdS;
+};
+__device__ __forceinline__ Dens dens(const Prm& p, double mt) { return {mt * p.drhodT, mt * p.drhodS}; }
+__device__ __forceinline__ double dot(const Dens& w, const Grad& gr) { return w.drdT * gr.T + w.drdS * gr.S; }
+// 1 / (min(0, drho/dz) - epsln): the slope denominator
+__device__ __forceinline__ double rden(double drodz) { return rcp_nr(fmin(0.0, drodz) - kEpsln); }
 
 // ---------------------------------------------------------------------------
-// isoneutral_diffusion_pre: thread per cell (i, j, k), i in [1, X-2), j in [1, Y-2)
+// Cell-parallel launch geometry: block = (nz levels) x (cpb columns along j);
+// consecutive lanes are consecutive levels of one column, so every read of a
+// neighbouring column is a contiguous run. The 1-D grid holds ni x nbx
+// blocks (rounded up to a multiple of 8) and is remapped XCD-contiguously.
+// ---------------------------------------------------------------------------
+struct CellIdx {
+    int i, j, k;
+    bool valid;
+};
+
+__device__ __forceinline__ CellIdx cell_of(int i0, int ni, int j0, int nj) {
+    const int cpb = blockDim.y;
+    const int nbx = (nj + cpb - 1) / cpb;
+    const int L = (int)xcd_block(gridDim.x);
+    CellIdx r;
+    const int ii = L / nbx;
+    const int bx = L - ii * nbx;
+    r.i = i0 + ii;
+    r.j = j0 + bx * cpb + (int)threadIdx.y;
+    r.k = (int)threadIdx.x;
+    r.valid = ii < ni && r.j < j0 + nj;
+    return r;
+}
+
+__device__ __forceinline__ double m01(int kb, int k) { return (kb > 0 && kb - 1 <= k) ? 1.0 : 0.0; }
+
+// ---------------------------------------------------------------------------
+// isoneutral_diffusion_pre: thread per cell (i, j, k), i in [1, X-2), j in [1, Y-2).
+// The 13 tracer points of the stencil are loaded once per field; all face
+// gradients are formed from them in registers. Offsets are 32-bit element
+// offsets (the host checks the arrays fit).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void iso_pre_kernel(Grid g, Prm p, Fields f, const double* __restrict__ K,
                                                       double* __restrict__ Aez, double* __restrict__ Anz,
                                                       double* __restrict__ Abx, double* __restrict__ Aby,
                                                       double* __restrict__ K11, double* __restrict__ K22,
                                                       double* __restrict__ K33) {
+    const CellIdx ci = cell_of(1, g.X - 3, 1, g.Y - 3);
+    if (!ci.valid) return;
+    const int i = ci.i, j = ci.j, k = ci.k;
     const int nz = g.nz;
-    const int ny3 = g.Y - 3;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t col = t / nz;
-    if (col >= (int64_t)(g.X - 3) * ny3) return;
-    const int k = (int)(t - col * nz);
-    const int i = 1 + (int)(col / ny3);
-    const int j = 1 + (int)(col - (int64_t)(i - 1) * ny3);
-    const int64_t c = idx3(g, i, j, k);
-    const int kbc = kb_of(g, i, j);
+    const uint32_t sj = (uint32_t)nz, si = (uint32_t)g.Y * (uint32_t)nz;
+    const uint32_t c = (uint32_t)i * si + (uint32_t)j * sj + (uint32_t)k;
+    const uint32_t ce = c + si, cw = c - si, cn = c + sj, cs = c - sj;
+    const bool hp = k < nz - 1, hm = k >= 1;
+    const double* __restrict__ T = f.T + f.tau;
+    const double* __restrict__ S = f.S + f.tau;
+    const int kb_c = g.kbot[i * g.Y + j], kb_e = g.kbot[(i + 1) * g.Y + j], kb_w = g.kbot[(i - 1) * g.Y + j];
+    const int kb_n = g.kbot[i * g.Y + j + 1], kb_s = g.kbot[i * g.Y + j - 1];
 
-    // gradients at this cell's faces, shared by the three face blocks
-    double gzT_c, gzS_c, gzT_m, gzS_m;  // top faces of (i,j,k) and (i,j,k-1)
-    grad_z(g, f, i, j, k, gzT_c, gzS_c);
-    if (k >= 1) grad_z(g, f, i, j, k - 1, gzT_m, gzS_m);
-    else { gzT_m = 0.0; gzS_m = 0.0; }
+    // tracer values: c(enter), e(ast), w(est), n(orth), s(outh); p = k+1, m = k-1
+    auto L2 = [&](uint32_t o, bool ok, double& t, double& sv) {
+        t = ok ? T[3u * o] : 0.0;
+        sv = ok ? S[3u * o] : 0.0;
+    };
+    double Tc, Sc, Tcp, Scp, Tcm, Scm, Te, Se, Tep, Sep, Tem, Sem, Tw, Sw, Twp, Swp;
+    double Tn, Sn, Tnp, Snp, Tnm, Snm, Ts, Ss, Tsp, Ssp;
+    L2(c, true, Tc, Sc);
+    L2(c + 1, hp, Tcp, Scp);
+    L2(c - 1, hm, Tcm, Scm);
+    L2(ce, true, Te, Se);
+    L2(ce + 1, hp, Tep, Sep);
+    L2(ce - 1, hm, Tem, Sem);
+    L2(cw, true, Tw, Sw);
+    L2(cw + 1, hp, Twp, Swp);
+    L2(cn, true, Tn, Sn);
+    L2(cn + 1, hp, Tnp, Snp);
+    L2(cn - 1, hm, Tnm, Snm);
+    L2(cs, true, Ts, Ss);
+    L2(cs + 1, hp, Tsp, Ssp);
+
+    // reciprocal metric factors
+    const double rz_c = hp ? rcp_nr(g.dzw[k]) : 0.0;
+    const double rz_m = hm ? rcp_nr(g.dzw[k - 1]) : 0.0;
+    const double rx_c = rcp_nr(g.dxu[i] * g.cost[j]);
+    const double ry_c = rcp_nr(g.dyu[j]);
+
+    // masked face gradients (veros/core/isoneutral/isoneutral.py:46-95)
+    auto G = [](double m, double a1, double a0, double b1, double b0, double rd) -> Grad {
+        return {m * (a1 - a0) * rd, m * (b1 - b0) * rd};
+    };
+    const double mc = m01(kb_c, k), mcp = m01(kb_c, k + 1), mcm = m01(kb_c, k - 1);
+    const double me = m01(kb_e, k), mep = m01(kb_e, k + 1), mem = m01(kb_e, k - 1);
+    const double mn = m01(kb_n, k), mnp = m01(kb_n, k + 1), mnm = m01(kb_n, k - 1);
+    const Grad gz_c = hp ? G(mc * mcp, Tcp, Tc, Scp, Sc, rz_c) : Grad{0.0, 0.0};
+    const Grad gz_m = hm ? G(mcm * mc, Tc, Tcm, Sc, Scm, rz_m) : Grad{0.0, 0.0};
+
+    const double tds = 2.0 / p.iso_dslope;
+    const ExpCoef ec = load_exp_coef();
 
     // --- east face (i+1/2, j, k), j in [2, Y-2)   (:97-132)
     if (j >= 2) {
-        const int kbe = kb_of(g, i + 1, j);
-        const double mU = mT(kbc, k) * mT(kbe, k);
-        double gxT, gxS;
-        grad_x(g, f, i, j, k, gxT, gxS);
-        double gzT_e, gzS_e, gzT_em = 0.0, gzS_em = 0.0;
-        grad_z(g, f, i + 1, j, k, gzT_e, gzS_e);
-        if (k >= 1) grad_z(g, f, i + 1, j, k - 1, gzT_em, gzS_em);
-        const double diffloc = (k >= 1) ? 0.25 * (K[idx3(g, i, j, k)] + K[idx3(g, i, j, k - 1)] +
-                                                  K[idx3(g, i + 1, j, k)] + K[idx3(g, i + 1, j, k - 1)])
-                                        : 0.5 * (K[idx3(g, i, j, 0)] + K[idx3(g, i + 1, j, 0)]);
+        const double mU = mc * me;
+        const Grad gx = G(mU, Te, Tc, Se, Sc, rx_c);
+        const Grad gz_e = hp ? G(me * mep, Tep, Te, Sep, Se, rz_c) : Grad{0.0, 0.0};
+        const Grad gz_em = hm ? G(mem * me, Te, Tem, Se, Sem, rz_m) : Grad{0.0, 0.0};
+        const double diffloc = hm ? 0.25 * (K[c] + K[c - 1] + K[ce] + K[ce - 1]) : 0.5 * (K[c] + K[ce]);
         double sumz = 0.0;
+        double out[4];
 #pragma unroll
         for (int kr = 0; kr < 2; ++kr) {
-            if (k < 1 - kr) continue;  // levels k >= 1 - kr
 #pragma unroll
             for (int ip = 0; ip < 2; ++ip) {
-                const double mt = mT(ip ? kbe : kbc, k);
-                const double zT = ip ? (kr ? gzT_e : gzT_em) : (kr ? gzT_c : gzT_m);
-                const double zS = ip ? (kr ? gzS_e : gzS_em) : (kr ? gzS_c : gzS_m);
-                const double s = slope(p, mt, gxT, gxS, zT, zS);
-                const double tp = taper(p, s);
+                out[ip * 2 + kr] = 0.0;
+                if (kr == 0 && !hm) continue;  // levels k >= 1 - kr
+                const Dens w = dens(p, ip ? me : mc);
+                const Grad& gz = ip ? (kr ? gz_e : gz_em) : (kr ? gz_c : gz_m);
+                const double sl = -dot(w, gx) * rden(dot(w, gz));
+                const double tp = taper(p, ec, tds, sl);
                 sumz += g.dzw[k - 1 + kr] * mU * fmax(p.K_iso_steep, diffloc * tp);
-                Aez[c * 4 + ip * 2 + kr] = tp * s * mU;
+                out[ip * 2 + kr] = tp * sl * mU;
             }
+        }
+        if (hm) {
+            *reinterpret_cast<double2*>(Aez + 4u * c) = make_double2(out[0], out[1]);
+        } else {
+            Aez[4u * c + 1] = out[1];  // (ip=0, kr=0) keeps its value at k = 0
+        }
+        if (hm) {
+            *reinterpret_cast<double2*>(Aez + 4u * c + 2) = make_double2(out[2], out[3]);
+        } else {
+            Aez[4u * c + 3] = out[3];
         }
         K11[c] = sumz / (4.0 * g.dzt[k]);
     }
 
     // --- north face (i, j+1/2, k), i in [2, X-2)   (:134-168)
     if (i >= 2) {
-        const int kbn = kb_of(g, i, j + 1);
-        const double mV = mT(kbc, k) * mT(kbn, k);
-        double gyT, gyS;
-        grad_y(g, f, i, j, k, gyT, gyS);
-        double gzT_n, gzS_n, gzT_nm = 0.0, gzS_nm = 0.0;
-        grad_z(g, f, i, j + 1, k, gzT_n, gzS_n);
-        if (k >= 1) grad_z(g, f, i, j + 1, k - 1, gzT_nm, gzS_nm);
-        const double diffloc = (k >= 1) ? 0.25 * (K[idx3(g, i, j, k)] + K[idx3(g, i, j, k - 1)] +
-                                                  K[idx3(g, i, j + 1, k)] + K[idx3(g, i, j + 1, k - 1)])
-                                        : 0.5 * (K[idx3(g, i, j, 0)] + K[idx3(g, i, j + 1, 0)]);
+        const double mV = mc * mn;
+        const Grad gy = G(mV, Tn, Tc, Sn, Sc, ry_c);
+        const Grad gz_n = hp ? G(mn * mnp, Tnp, Tn, Snp, Sn, rz_c) : Grad{0.0, 0.0};
+        const Grad gz_nm = hm ? G(mnm * mn, Tn, Tnm, Sn, Snm, rz_m) : Grad{0.0, 0.0};
+        const double diffloc = hm ? 0.25 * (K[c] + K[c - 1] + K[cn] + K[cn - 1]) : 0.5 * (K[c] + K[cn]);
         double sumz = 0.0;
+        double out[4];
 #pragma unroll
         for (int kr = 0; kr < 2; ++kr) {
-            if (k < 1 - kr) continue;
 #pragma unroll
             for (int jp = 0; jp < 2; ++jp) {
-                const double mt = mT(jp ? kbn : kbc, k);
-                const double zT = jp ? (kr ? gzT_n : gzT_nm) : (kr ? gzT_c : gzT_m);
-                const double zS = jp ? (kr ? gzS_n : gzS_nm) : (kr ? gzS_c : gzS_m);
-                const double s = slope(p, mt, gyT, gyS, zT, zS);
-                const double tp = taper(p, s);
+                out[jp * 2 + kr] = 0.0;
+                if (kr == 0 && !hm) continue;
+                const Dens w = dens(p, jp ? mn : mc);
+                const Grad& gz = jp ? (kr ? gz_n : gz_nm) : (kr ? gz_c : gz_m);
+                const double sl = -dot(w, gy) * rden(dot(w, gz));
+                const double tp = taper(p, ec, tds, sl);
                 sumz += g.dzw[k - 1 + kr] * mV * fmax(p.K_iso_steep, diffloc * tp);
-                Anz[c * 4 + jp * 2 + kr] = tp * s * mV;
+                out[jp * 2 + kr] = tp * sl * mV;
             }
+        }
+        if (hm) {
+            *reinterpret_cast<double2*>(Anz + 4u * c) = make_double2(out[0], out[1]);
+            *reinterpret_cast<double2*>(Anz + 4u * c + 2) = make_double2(out[2], out[3]);
+        } else {
+            Anz[4u * c + 1] = out[1];
+            Anz[4u * c + 3] = out[3];
         }
         K22[c] = sumz / (4.0 * g.dzt[k]);
     }
 
     // --- top face (i, j, k+1/2), i, j in [2, X/Y-2), k < nz-1   (:170-225)
-    if (i >= 2 && j >= 2 && k < nz - 1) {
-        const double mW = mT(kbc, k) * mT(kbc, k + 1);
+    if (i >= 2 && j >= 2 && hp) {
+        const int kb_w2 = kb_w, kb_s2 = kb_s;
+        const double mW = mc * mcp;
         const double Kc = K[c];
+        const double rx_w = rcp_nr(g.dxu[i - 1] * g.cost[j]);
+        const double ry_s = rcp_nr(g.dyu[j - 1]);
+        const double mw = m01(kb_w2, k), mwp = m01(kb_w2, k + 1);
+        const double ms = m01(kb_s2, k), msp = m01(kb_s2, k + 1);
+        // horizontal gradients at (i-1+ip, k+kr) and (j-1+jp, k+kr)
+        const Grad gx_w = G(mw * mc, Tc, Tw, Sc, Sw, rx_w);
+        const Grad gx_wp = G(mwp * mcp, Tcp, Twp, Scp, Swp, rx_w);
+        const Grad gx_c = G(mc * me, Te, Tc, Se, Sc, rx_c);
+        const Grad gx_cp = G(mcp * mep, Tep, Tcp, Sep, Scp, rx_c);
+        const Grad gy_s = G(ms * mc, Tc, Ts, Sc, Ss, ry_s);
+        const Grad gy_sp = G(msp * mcp, Tcp, Tsp, Scp, Ssp, ry_s);
+        const Grad gy_c = G(mc * mn, Tn, Tc, Sn, Sc, ry_c);
+        const Grad gy_cp = G(mcp * mnp, Tnp, Tcp, Snp, Scp, ry_c);
         double sumx = 0.0, sumy = 0.0;
+        double ox[4], oy[4];
 #pragma unroll
         for (int kr = 0; kr < 2; ++kr) {
-            const double mt = mT(kbc, k + kr);
+            const Dens w = dens(p, kr ? mcp : mc);
+            const double rd = rden(dot(w, gz_c));
 #pragma unroll
             for (int ip = 0; ip < 2; ++ip) {
-                double gxT, gxS;
-                grad_x(g, f, i - 1 + ip, j, k + kr, gxT, gxS);
-                const double s = slope(p, mt, gxT, gxS, gzT_c, gzS_c);
-                const double tp = taper(p, s);
-                sumx += g.dxu[i - 1 + ip] * Kc * tp * (s * s) * mW;
-                Abx[c * 4 + ip * 2 + kr] = tp * s * mW;
+                const Grad& gx = ip ? (kr ? gx_cp : gx_c) : (kr ? gx_wp : gx_w);
+                const double sl = -dot(w, gx) * rd;
+                const double tp = taper(p, ec, tds, sl);
+                sumx += g.dxu[i - 1 + ip] * Kc * tp * (sl * sl) * mW;
+                ox[ip * 2 + kr] = tp * sl * mW;
             }
 #pragma unroll
             for (int jp = 0; jp < 2; ++jp) {
-                double gyT, gyS;
-                grad_y(g, f, i, j - 1 + jp, k + kr, gyT, gyS);
-                const double s = slope(p, mt, gyT, gyS, gzT_c, gzS_c);
-                const double tp = taper(p, s);
+                const Grad& gy = jp ? (kr ? gy_cp : gy_c) : (kr ? gy_sp : gy_s);
+                const double sl = -dot(w, gy) * rd;
+                const double tp = taper(p, ec, tds, sl);
                 const double facty = g.cosu[j - 1 + jp] * g.dyu[j - 1 + jp];
-                sumy += facty * Kc * tp * (s * s) * mW;
-                Aby[c * 4 + jp * 2 + kr] = tp * s * mW;
+                sumy += facty * Kc * tp * (sl * sl) * mW;
+                oy[jp * 2 + kr] = tp * sl * mW;
             }
         }
+        *reinterpret_cast<double2*>(Abx + 4u * c) = make_double2(ox[0], ox[1]);
+        *reinterpret_cast<double2*>(Abx + 4u * c + 2) = make_double2(ox[2], ox[3]);
+        *reinterpret_cast<double2*>(Aby + 4u * c) = make_double2(oy[0], oy[1]);
+        *reinterpret_cast<double2*>(Aby + 4u * c + 2) = make_double2(oy[2], oy[3]);
         K33[c] = sumx / (4 * g.dxt[i]) + sumy / (4 * g.dyt[j] * g.cost[j]);
     }
 }
@@ -308,11 +461,11 @@ __global__ __launch_bounds__(256) void iso_flux_kernel(Grid g, FluxArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// column kernel: thread per column (i, j) in [1, X-1) x [1, Y-1).
-//  - every such column: P_diss += dissipation_on_wgrid(compute_dissipation(...))
-//  - interior columns additionally: explicit divergence, tracer and dtr_iso
-//    update, implicit K_33 solve (Thomas in registers), the implicit part of
-//    dtr_iso, and the vertical-mixing dissipation term.
+// explicit part: thread per cell (i, j, k), (i, j) in [1, X-1) x [1, Y-1)
+//  - all such cells (energy on): P_diss += dissipation_on_wgrid(compute_dissipation(...))
+//    (veros/core/diffusion.py:9-62; diss is evaluated at k and k+1 here)
+//  - interior cells: dtr = flux divergence (veros/core/isoneutral/diffusion.py:116-139),
+//    dtr_iso += dtr, tr[taup1] += dt * dtr (:196-197)
 // ---------------------------------------------------------------------------
 struct ColArgs {
     double* tr;
@@ -322,7 +475,6 @@ struct ColArgs {
     const double* I;   // int_drhodX (X, Y, nz, 3), nullable
     double* P;         // P_diss, nullable
     const double *fe, *fn, *ft;
-    int implicit;      // 1: iso (implicit K33 solve), 0: skew (explicit only)
 };
 
 // Face fluxes with their valid index ranges (0 elsewhere).
@@ -333,127 +485,160 @@ __device__ __forceinline__ double fn_at(const Grid& g, const ColArgs& a, int i, 
     return (i >= 2 && i <= g.X - 3 && j >= 1 && j <= g.Y - 3) ? a.fn[idx3(g, i, j, k)] : 0.0;
 }
 
-template <int NZB>
-__global__ __launch_bounds__(256) void iso_column_kernel(Grid g, Prm p, ColArgs a) {
+// compute_dissipation at one T point (veros/core/diffusion.py:13-33)
+__device__ __forceinline__ double diss_at(const Grid& g, const ColArgs& a, double g05, int i, int j, int k) {
+    const double Ic = tr3(a.I, g, i, j, k, a.tau);
+    const double de = (tr3(a.I, g, i + 1, j, k, a.tau) - Ic) * fe_at(g, a, i, j, k) +
+                      (Ic - tr3(a.I, g, i - 1, j, k, a.tau)) * fe_at(g, a, i - 1, j, k);
+    const double dn = (tr3(a.I, g, i, j + 1, k, a.tau) - Ic) * fn_at(g, a, i, j, k) +
+                      (Ic - tr3(a.I, g, i, j - 1, k, a.tau)) * fn_at(g, a, i, j - 1, k);
+    return g05 * de / (g.dxt[i] * g.cost[j]) + g05 * dn / (g.dyt[j] * g.cost[j]);
+}
+
+__global__ __launch_bounds__(256) void iso_explicit_kernel(Grid g, Prm p, ColArgs a) {
     const int nz = g.nz;
     const int ny2 = g.Y - 2;
-    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    const int64_t col = t / nz;
     if (col >= (int64_t)(g.X - 2) * ny2) return;
+    const int k = (int)(t - col * nz);
     const int i = 1 + (int)(col / ny2);
     const int j = 1 + (int)(col - (int64_t)(i - 1) * ny2);
-    const bool interior = i >= 2 && i <= g.X - 3 && j >= 2 && j <= g.Y - 3;
+    const int64_t c = idx3(g, i, j, k);
     const int kb = kb_of(g, i, j);
-    const bool energy = a.P != nullptr;
 
-    // ---- dissipation of lateral fluxes on W points (veros/core/diffusion.py:9-62)
-    if (energy) {
+    if (a.P) {
         const double g05 = 0.5 * p.grav / p.rho_0;
-        const double rx = g.dxt[i] * g.cost[j];
-        const double ry = g.dyt[j] * g.cost[j];
-        const bool land = kb >= 1;          // create_water_masks(kbot): ks = kbot - 1 >= 0
-        double diss_next = 0.0;             // diss at k+1
-        for (int k = nz - 1; k >= 0; --k) {
-            const double Ic = tr3(a.I, g, i, j, k, a.tau);
-            const double de = (tr3(a.I, g, i + 1, j, k, a.tau) - Ic) * fe_at(g, a, i, j, k) +
-                              (Ic - tr3(a.I, g, i - 1, j, k, a.tau)) * fe_at(g, a, i - 1, j, k);
-            const double dn = (tr3(a.I, g, i, j + 1, k, a.tau) - Ic) * fn_at(g, a, i, j, k) +
-                              (Ic - tr3(a.I, g, i, j - 1, k, a.tau)) * fn_at(g, a, i, j - 1, k);
-            const double diss = g05 * de / rx + g05 * dn / ry;
-            double dw;
-            if (k == nz - 1) {
-                dw = diss * (land ? 1.0 : 0.0);
-            } else {
-                const double avg = 0.5 * (diss + diss_next);
-                const bool edge = land && k == kb - 1;
-                const bool water = land && k > kb - 1;
-                const double dzw_lo = g.dzw[k >= 1 ? k - 1 : 0];  // pad_z_edges(dzw)[k]
-                dw = (avg + 0.5 * (diss * dzw_lo / g.dzw[k])) * (edge ? 1.0 : 0.0) + avg * (water ? 1.0 : 0.0);
-            }
-            double* Pk = a.P + idx3(g, i, j, k);
-            *Pk = *Pk + dw;
-            diss_next = diss;
+        const double d0 = diss_at(g, a, g05, i, j, k);
+        const bool land = kb >= 1;  // create_water_masks(kbot): ks = kbot - 1 >= 0
+        double dw;
+        if (k == nz - 1) {
+            dw = d0 * (land ? 1.0 : 0.0);
+        } else {
+            const double d1 = diss_at(g, a, g05, i, j, k + 1);
+            const double avg = 0.5 * (d0 + d1);
+            const bool edge = land && k == kb - 1;
+            const bool water = land && k > kb - 1;
+            const double dzw_lo = g.dzw[k >= 1 ? k - 1 : 0];  // pad_z_edges(dzw)[k]
+            dw = (avg + 0.5 * (d0 * dzw_lo / g.dzw[k])) * (edge ? 1.0 : 0.0) + avg * (water ? 1.0 : 0.0);
         }
+        a.P[c] = a.P[c] + dw;
     }
-    if (!interior) return;
+    if (i < 2 || i > g.X - 3 || j < 2 || j > g.Y - 3) return;
 
+    const double mt = mT(kb, k);
+    const double ftk = a.ft[c];
+    const double hor = (a.fe[c] - fe_at(g, a, i - 1, j, k)) / (g.cost[j] * g.dxt[i]) +
+                       (a.fn[c] - fn_at(g, a, i, j - 1, k)) / (g.cost[j] * g.dyt[j]);
+    const double vert = (k == 0) ? ftk / g.dzt[0] : (ftk - a.ft[c - 1]) / g.dzt[k];
+    const double dtrk = mt * hor + mt * vert;
+    a.dtr[c] = a.dtr[c] + dtrk;
+    double* trp = a.tr + c * 3 + a.taup1;
+    *trp = *trp + p.dt_tracer * dtrk;
+}
+
+// ---------------------------------------------------------------------------
+// implicit K_33 solve (veros/core/isoneutral/diffusion.py:142-169,202-204):
+// lane per interior column, Thomas in registers over the wet suffix;
+// tr[taup1] <- solution, dtr_iso += (solution - rhs) / dt on wet cells
+// (dry cells keep rhs, so their increment is 0).
+// ---------------------------------------------------------------------------
+template <int NZB, bool VEC>
+__global__ __launch_bounds__(256) void iso_implicit_kernel(Grid g, Prm p, ColArgs a) {
+    const int nz = g.nz;
+    const int ny4 = g.Y - 4;
+    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= (int64_t)(g.X - 4) * ny4) return;
+    const int i = 2 + (int)(col / ny4);
+    const int j = 2 + (int)(col - (int64_t)(i - 2) * ny4);
+    const int kb = kb_of(g, i, j);
+    if (kb < 1 || kb > nz) return;
+    const int k0 = kb - 1;
+    const int64_t base = idx3(g, i, j, 0);
+    const double* __restrict__ K33 = a.K33 + base;
+    double* __restrict__ trc = a.tr + base * 3 + a.taup1;
+    double* __restrict__ dtr = a.dtr + base;
     const double dt = p.dt_tracer;
-    const double cx = g.cost[j] * g.dxt[i];
-    const double cy = g.cost[j] * g.dyt[j];
-    const int k0 = (kb >= 1 && kb <= nz) ? kb - 1 : nz;  // first wet level of the implicit system
-    double* __restrict__ trc = a.tr;
 
-    // ---- explicit part + forward elimination of the implicit system
     double cp[NZB], dp[NZB];
-    double cpp = 0.0, dpp = 0.0;
-    double delta_m = 0.0;   // delta at k-1
-    double ft_m = 0.0;      // flux_top at k-1
+    double cpp = 0.0, dpp = 0.0, delta_m = 0.0;
 #pragma unroll
-    for (int k = 0; k < NZB; ++k) {
-        if (k < nz) {
-            const int64_t ck = idx3(g, i, j, k);
-            const double mt = mT(kb, k);
-            const double ftk = a.ft[ck];
-            const double hor = (a.fe[ck] - fe_at(g, a, i - 1, j, k)) / cx + (a.fn[ck] - fn_at(g, a, i, j - 1, k)) / cy;
-            const double vert = (k == 0) ? ftk / g.dzt[0] : (ftk - ft_m) / g.dzt[k];
-            const double dtrk = mt * hor + mt * vert;
-            ft_m = ftk;
-            a.dtr[ck] = a.dtr[ck] + dtrk;
-            const double rhs = trc[ck * 3 + a.taup1] + dt * dtrk;
-            trc[ck * 3 + a.taup1] = rhs;
-            if (a.implicit) {
-                const double delta = (k < nz - 1) ? dt / g.dzw[k] * a.K33[ck] : 0.0;
-                if (k >= k0) {
+    for (int q = 0; q < NZB; q += 8) {
+        if (q < nz && q + 8 > k0) {
+            double kv[8] = {}, rv[8] = {};
+            if constexpr (VEC) {
+#pragma unroll
+                for (int t = 0; t < 8; t += 2) {
+                    if (q + t < nz) {
+                        const double2 w = *reinterpret_cast<const double2*>(K33 + q + t);
+                        kv[t] = w.x;
+                        kv[t + 1] = w.y;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < 8; ++t)
+                    if (q + t < nz) kv[t] = K33[q + t];
+            }
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                if (q + t < nz && q + t >= k0) rv[t] = trc[(q + t) * 3];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int k = q + t;
+                if (k < nz && k >= k0) {
+                    const double delta = (k < nz - 1) ? dt / g.dzw[k] * kv[t] : 0.0;
                     const double dz = g.dzt[k];
                     const bool edge = k == k0;
                     const double ak = edge ? 0.0 : -delta_m / dz;
                     const double bk = edge ? 1 + delta / dz : 1 + (delta + delta_m) / dz;
-                    const double ck2 = (k < nz - 1) ? -delta / dz : 0.0;
+                    const double ck = (k < nz - 1) ? -delta / dz : 0.0;
                     const double r = 1.0 / (bk - ak * cpp);
-                    cpp = ck2 * r;
-                    dpp = (rhs - ak * dpp) * r;
+                    cpp = ck * r;
+                    dpp = (rv[t] - ak * dpp) * r;
+                    delta_m = delta;
                 }
-                delta_m = delta;
-            }
-            cp[k] = cpp;
-            dp[k] = dpp;
-        }
-    }
-    if (!a.implicit) {
-        if (energy) {
-            const double gr = -p.grav / p.rho_0;
-            for (int k = 0; k < nz - 1; ++k) {
-                const int64_t ck = idx3(g, i, j, k);
-                const double mw = mT(kb, k) * mT(kb, k + 1);
-                const double fxa = (-tr3(a.I, g, i, j, k + 1, a.tau) + tr3(a.I, g, i, j, k, a.tau)) / g.dzw[k];
-                a.P[ck] = a.P[ck] + gr * fxa * a.ft[ck] * mw;
+                cp[k] = cpp;
+                dp[k] = dpp;
             }
         }
-        return;
     }
-
-    // ---- back substitution, tracer/dtr_iso update, vertical dissipation term
-    const double gr = -p.grav / p.rho_0;
-    double xn = 0.0;       // new tracer at k+1
+    double xn = 0.0;
 #pragma unroll
     for (int k = NZB - 1; k >= 0; --k) {
-        if (k < nz) {
-            const int64_t ck = idx3(g, i, j, k);
-            double xk;
-            if (k >= k0) {
-                xk = dp[k] - cp[k] * xn;
-                const double rhs = trc[ck * 3 + a.taup1];
-                trc[ck * 3 + a.taup1] = xk;
-                a.dtr[ck] = a.dtr[ck] + (xk - rhs) / dt;
-            } else {
-                xk = trc[ck * 3 + a.taup1];
-            }
-            if (energy && k < nz - 1) {
-                const double mw = mT(kb, k) * mT(kb, k + 1);
-                const double fxa = (-tr3(a.I, g, i, j, k + 1, a.tau) + tr3(a.I, g, i, j, k, a.tau)) / g.dzw[k];
-                a.P[ck] = a.P[ck] + gr * fxa * (a.ft[ck] * mw + a.K33[ck] * (xn - xk) / g.dzw[k] * mw);
-            }
+        if (k < nz && k >= k0) {
+            const double xk = dp[k] - cp[k] * xn;
+            const double rhs = trc[k * 3];
+            trc[k * 3] = xk;
+            dtr[k] = dtr[k] + (xk - rhs) / dt;
             xn = xk;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// vertical-mixing dissipation (veros/core/isoneutral/diffusion.py:254-279):
+// thread per interior cell, k < nz-1, after the implicit solve.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void iso_post_kernel(Grid g, Prm p, ColArgs a, int implicit) {
+    const int nz1 = g.nz - 1;
+    const int ny4 = g.Y - 4;
+    const int64_t t = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    const int64_t col = t / nz1;
+    if (col >= (int64_t)(g.X - 4) * ny4) return;
+    const int k = (int)(t - col * nz1);
+    const int i = 2 + (int)(col / ny4);
+    const int j = 2 + (int)(col - (int64_t)(i - 2) * ny4);
+    const int64_t c = idx3(g, i, j, k);
+    const int kb = kb_of(g, i, j);
+    const double mw = mT(kb, k) * mT(kb, k + 1);
+    const double fxa = (-tr3(a.I, g, i, j, k + 1, a.tau) + tr3(a.I, g, i, j, k, a.tau)) / g.dzw[k];
+    const double gr = -p.grav / p.rho_0;
+    if (implicit) {
+        const double dtrz = a.tr[(c + 1) * 3 + a.taup1] - a.tr[c * 3 + a.taup1];
+        a.P[c] = a.P[c] + gr * fxa * (a.ft[c] * mw + a.K33[c] * dtrz / g.dzw[k] * mw);
+    } else {
+        a.P[c] = a.P[c] + gr * fxa * a.ft[c] * mw;
     }
 }
 
@@ -501,6 +686,15 @@ Prm make_prm(const vx_iso_params* q) {
 }
 
 unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+unsigned blocks_xcd(int64_t n, int bs) { return (unsigned)(((n + bs - 1) / bs + 7) / 8 * 8); }
+
+// block of nz levels x cpb columns (about 256 threads); 1-D grid of ni row
+// groups, rounded up to a multiple of 8 for the XCD remap
+dim3 cell_block(int nz) { return dim3(nz, nz >= 256 ? 1 : 256 / nz); }
+dim3 cell_grid(int ni, int nj, dim3 blk) {
+    const int64_t nbx = (nj + blk.y - 1) / blk.y;
+    return dim3((unsigned)(((int64_t)ni * nbx + 7) / 8 * 8));
+}
 
 }  // namespace
 }  // namespace vx
@@ -519,8 +713,10 @@ extern "C" int vx_iso_pre_f64(void* stream, const vx_grid* grid, const vx_iso_pa
     hipStream_t s = (hipStream_t)stream;
     Fields f{temp, salt, tau};
     const Prm p = make_prm(prm);
-    const int64_t n = (int64_t)(g.X - 3) * (g.Y - 3) * g.nz;
-    hipLaunchKernelGGL(iso_pre_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, g, p, f, K_iso, Ai_ez, Ai_nz,
+    if ((int64_t)g.X * g.Y * g.nz * 4 >= ((int64_t)1 << 32))
+        return fail(VX_ERR_UNSUPPORTED, "vx_iso_pre_f64: grid too large for 32-bit offsets");
+    const dim3 blk = cell_block(g.nz);
+    hipLaunchKernelGGL(iso_pre_kernel, cell_grid(g.X - 3, g.Y - 3, blk), blk, 0, s, g, p, f, K_iso, Ai_ez, Ai_nz,
                        Ai_bx, Ai_by, K_11, K_22, K_33);
     if ((rc = check_launch("vx_iso_pre_f64"))) return rc;
     hipLaunchKernelGGL(k33_top_zero_kernel, dim3(blocks_for((int64_t)g.X * g.Y, 256)), dim3(256), 0, s, g, K_33);
@@ -534,25 +730,30 @@ extern "C" int64_t vx_iso_workspace_bytes(const vx_grid* grid) {
 
 namespace vx {
 namespace {
-int launch_column(const Grid& g, const Prm& p, const ColArgs& a, hipStream_t s) {
-    const int64_t ncol = (int64_t)(g.X - 2) * (g.Y - 2);
+
+int launch_implicit(const Grid& g, const Prm& p, const ColArgs& a, hipStream_t s) {
+    const int64_t ncol = (int64_t)(g.X - 4) * (g.Y - 4);
     const dim3 grid(blocks_for(ncol, 256)), block(256);
     const int nzb = ((g.nz + 15) / 16) * 16;
-#define VX_COL_CASE(N) \
-    case N: hipLaunchKernelGGL(iso_column_kernel<N>, grid, block, 0, s, g, p, a); break;
+    const bool vec = (g.nz % 2 == 0) && aligned16(a.K33);
+#define VX_IMPL_CASE(N)                                                                   \
+    case N:                                                                               \
+        if (vec) hipLaunchKernelGGL((iso_implicit_kernel<N, true>), grid, block, 0, s, g, p, a); \
+        else hipLaunchKernelGGL((iso_implicit_kernel<N, false>), grid, block, 0, s, g, p, a);    \
+        break;
     switch (nzb) {
-        VX_COL_CASE(16)
-        VX_COL_CASE(32)
-        VX_COL_CASE(48)
-        VX_COL_CASE(64)
-        VX_COL_CASE(80)
-        VX_COL_CASE(96)
-        VX_COL_CASE(112)
-        VX_COL_CASE(128)
-        default: return fail(VX_ERR_UNSUPPORTED, "iso column kernel: nz=%d", g.nz);
+        VX_IMPL_CASE(16)
+        VX_IMPL_CASE(32)
+        VX_IMPL_CASE(48)
+        VX_IMPL_CASE(64)
+        VX_IMPL_CASE(80)
+        VX_IMPL_CASE(96)
+        VX_IMPL_CASE(112)
+        VX_IMPL_CASE(128)
+        default: return fail(VX_ERR_UNSUPPORTED, "iso implicit kernel: nz=%d", g.nz);
     }
-#undef VX_COL_CASE
-    return check_launch("iso_column_kernel");
+#undef VX_IMPL_CASE
+    return check_launch("iso_implicit_kernel");
 }
 
 int iso_diffusion_common(void* stream, const vx_grid* grid, const vx_iso_params* prm, double* tracer, int32_t tau,
@@ -579,8 +780,17 @@ int iso_diffusion_common(void* stream, const vx_grid* grid, const vx_iso_params*
     const int64_t n = (int64_t)(g.X - 3) * (g.Y - 3) * g.nz;
     hipLaunchKernelGGL(iso_flux_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, g, fa);
     if ((rc = check_launch(who))) return rc;
-    ColArgs ca{tracer, tau, taup1, K_33, dtr_iso, int_drhodX, P_diss, ws, ws + n3, ws + 2 * n3, implicit};
-    return launch_column(g, p, ca, s);
+    ColArgs ca{tracer, tau, taup1, K_33, dtr_iso, int_drhodX, P_diss, ws, ws + n3, ws + 2 * n3};
+    const int64_t ne = (int64_t)(g.X - 2) * (g.Y - 2) * g.nz;
+    hipLaunchKernelGGL(iso_explicit_kernel, dim3(blocks_xcd(ne, 256)), dim3(256), 0, s, g, p, ca);
+    if ((rc = check_launch(who))) return rc;
+    if (implicit && (rc = launch_implicit(g, p, ca, s))) return rc;
+    if (P_diss) {
+        const int64_t np_ = (int64_t)(g.X - 4) * (g.Y - 4) * (g.nz - 1);
+        hipLaunchKernelGGL(iso_post_kernel, dim3(blocks_xcd(np_, 256)), dim3(256), 0, s, g, p, ca, implicit);
+        if ((rc = check_launch(who))) return rc;
+    }
+    return VX_OK;
 }
 }  // namespace
 }  // namespace vx
