@@ -141,6 +141,24 @@ int vx_iso_diffusion_f64(void* stream, const vx_grid* grid, const vx_iso_params*
                          const double* int_drhodX, double* P_diss,
                          void* workspace);
 
+/*
+ * isoneutral_skew_diffusion for one tracer (veros/core/isoneutral/diffusion.py:172-207,
+ * 210-283, 298-307, iso=False): the same flux stencil with K_iso = 0 and
+ * K_skew = K_gm, the explicit update of tracer[..., taup1] and dtr_iso, no
+ * implicit part; P_diss_skew (when non-NULL, with int_drhodX) accumulates the
+ * lateral dissipation and the explicit vertical term. Same workspace size
+ * as vx_iso_diffusion_f64.
+ */
+int vx_iso_skew_diffusion_f64(void* stream, const vx_grid* grid, const vx_iso_params* prm,
+                              double* tracer, int32_t tau, int32_t taup1,
+                              const double* K_gm,
+                              const double* Ai_ez, const double* Ai_nz,
+                              const double* Ai_bx, const double* Ai_by,
+                              const double* K_11, const double* K_22,
+                              double* dtr_iso,
+                              const double* int_drhodX, double* P_diss_skew,
+                              void* workspace);
+
 #ifdef __cplusplus
 }
 #endif

@@ -35,7 +35,8 @@ def test_pre_golden(path):
 
 @pytest.mark.parametrize("path", golden_files("iso_"), ids=lambda p: p.stem)
 @pytest.mark.parametrize("trn", ["temp", "salt"])
-def test_diffusion_golden(path, trn):
+@pytest.mark.parametrize("iso", [True, False], ids=["iso", "skew"])
+def test_diffusion_golden(path, trn, iso):
     from veros_amd import isoneutral
     g = load_golden(path)
     st = state_from_golden(g)
@@ -43,8 +44,8 @@ def test_diffusion_golden(path, trn):
         setattr(st.variables, k, g["pre_" + k])
     ds = _dev_state(st)
     istemp = trn == "temp"
-    out = isoneutral.isoneutral_diffusion_kernel(ds, getattr(ds.variables, trn), istemp)
-    tag = "iso_" + trn
+    out = isoneutral.isoneutral_diffusion_kernel(ds, getattr(ds.variables, trn), istemp, iso=iso)
+    tag = ("iso_" if iso else "skew_") + trn
     keys = [k[len(tag) + 1:] for k in g if k.startswith(tag + "_")]
     assert set(keys) == set(out._fields)
     for k in keys:
@@ -73,15 +74,18 @@ def test_pre_and_diffusion_random(shape, cyclic):
         assert rel_err(_np(getattr(out, k)), getattr(ref, k)) <= TOL, k
     for k, v in ref._asdict().items():
         setattr(st.variables, k, v)
+    st.variables.K_gm = 300.0 * (0.5 + np.random.default_rng(1).random(st.variables.K_iso.shape))
+    st.variables.P_diss_skew = np.zeros(st.variables.K_iso.shape)
     for energy in (True, False):
         st.settings.enable_conserve_energy = energy
         for istemp, trn in ((True, "temp"), (False, "salt")):
-            r = O.isoneutral_diffusion_kernel(st, getattr(st.variables, trn), istemp)
-            ds = _dev_state(st)
-            o = isoneutral.isoneutral_diffusion_kernel(ds, getattr(ds.variables, trn), istemp)
-            assert set(o._fields) == set(r._fields)
-            for k in r._fields:
-                assert rel_err(_np(getattr(o, k)), getattr(r, k)) <= TOL, (energy, trn, k)
+            for iso in (True, False):
+                r = O.isoneutral_diffusion_kernel(st, getattr(st.variables, trn), istemp, iso=iso)
+                ds = _dev_state(st)
+                o = isoneutral.isoneutral_diffusion_kernel(ds, getattr(ds.variables, trn), istemp, iso=iso)
+                assert set(o._fields) == set(r._fields)
+                for k in r._fields:
+                    assert rel_err(_np(getattr(o, k)), getattr(r, k)) <= TOL, (energy, trn, iso, k)
 
 
 def test_config1_step_against_oracle():

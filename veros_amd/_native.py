@@ -51,6 +51,9 @@ _SIGNATURES = {
     "vx_iso_diffusion_f64": (ctypes.c_int, [c_void_p, ctypes.POINTER(VxGrid), ctypes.POINTER(VxIsoParams),
                                             c_void_p, c_i32, c_i32, c_void_p] + [c_void_p] * 7
                              + [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "vx_iso_skew_diffusion_f64": (ctypes.c_int, [c_void_p, ctypes.POINTER(VxGrid), ctypes.POINTER(VxIsoParams),
+                                                 c_void_p, c_i32, c_i32, c_void_p] + [c_void_p] * 6
+                                  + [c_void_p, c_void_p, c_void_p, c_void_p]),
 }
 
 _lock = threading.Lock()

@@ -2,11 +2,10 @@
 //
 //   iso_pre_kernel       isoneutral_diffusion_pre  veros/core/isoneutral/isoneutral.py:18-229
 //   iso_flux_kernel      _calc_tracer_fluxes       veros/core/isoneutral/diffusion.py:9-113
-//   iso_explicit_kernel  _calc_explicit_part, the explicit tracer update and the lateral
-//                        dissipation               veros/core/isoneutral/diffusion.py:116-139,196-197,
+//   iso_column_tile_kernel  _calc_explicit_part, _calc_implicit_part, the tracer and
+//                        dtr_iso updates and the energy branch
+//                                                  veros/core/isoneutral/diffusion.py:116-283,
 //                                                  veros/core/diffusion.py:9-62
-//   iso_implicit_kernel  _calc_implicit_part       veros/core/isoneutral/diffusion.py:142-169,202-204
-//   iso_post_kernel      vertical-mixing dissipation veros/core/isoneutral/diffusion.py:254-279
 //
 // Layout: veros state arrays with 2-cell halos, z contiguous, k = 0 deepest.
 // Cell-parallel kernels put consecutive lanes on consecutive levels of one
@@ -373,99 +372,108 @@ __global__ void k33_top_zero_kernel(Grid g, double* __restrict__ K33) {
 struct FluxArgs {
     const double* __restrict__ tr;
     int tau;
-    const double* __restrict__ K1;   // K_iso - K_skew (east/north)
-    const double* __restrict__ K2;   // K_iso + K_skew (top)
+    // K1 = K_iso - K_skew (east/north faces), K2 = K_iso + K_skew (top faces),
+    // either term absent when its pointer is null (veros/core/isoneutral/diffusion.py:15-16,180-188)
+    const double* __restrict__ Kiso;
+    const double* __restrict__ Kskew;
     const double *Aez, *Anz, *Abx, *Aby, *K11, *K22;
     double *fe, *fn, *ft;
 };
 
+__device__ __forceinline__ double k1_at(const FluxArgs& a, uint32_t o) {
+    const double ki = a.Kiso ? a.Kiso[o] : 0.0;
+    return a.Kskew ? ki - a.Kskew[o] : ki;
+}
+__device__ __forceinline__ double k2_at(const FluxArgs& a, uint32_t o) {
+    const double ki = a.Kiso ? a.Kiso[o] : 0.0;
+    return a.Kskew ? ki + a.Kskew[o] : ki;
+}
+
 __global__ __launch_bounds__(256) void iso_flux_kernel(Grid g, FluxArgs a) {
+    const CellIdx ci = cell_of(1, g.X - 3, 1, g.Y - 3);
+    if (!ci.valid) return;
+    const int i = ci.i, j = ci.j, k = ci.k;
     const int nz = g.nz;
-    const int ny3 = g.Y - 3;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t col = t / nz;
-    if (col >= (int64_t)(g.X - 3) * ny3) return;
-    const int k = (int)(t - col * nz);
-    const int i = 1 + (int)(col / ny3);
-    const int j = 1 + (int)(col - (int64_t)(i - 1) * ny3);
-    const int64_t c = idx3(g, i, j, k);
-    const double* __restrict__ tr = a.tr;
-    const int lev = a.tau;
-    const int kup = (k + 1 < nz) ? k + 1 : nz - 1;  // pad_z_edges clamping
-    const int kdn = (k >= 1) ? k - 1 : 0;
-    const double trc = tr3(tr, g, i, j, k, lev);
+    const uint32_t sj = (uint32_t)nz, si = (uint32_t)g.Y * (uint32_t)nz;
+    const uint32_t c = (uint32_t)i * si + (uint32_t)j * sj + (uint32_t)k;
+    const uint32_t ce = c + si, cn = c + sj;
+    const double* __restrict__ tr = a.tr + a.tau;
+    const bool hm = k >= 1, hp = k + 1 < nz;
+    // pad_z_edges clamping: tr[min(k+1, nz-1)], tr[max(k-1, 0)]
+    const uint32_t up = hp ? 1u : 0u, dn = hm ? 1u : 0u;
+    const double trc = tr[3u * c];
 
     if (j >= 2) {  // east face (:22-47)
-        const double* K1 = a.K1;
-        const double diffloc = (k >= 1) ? 0.25 * (K1[idx3(g, i, j, k)] + K1[idx3(g, i, j, k - 1)] +
-                                                  K1[idx3(g, i + 1, j, k)] + K1[idx3(g, i + 1, j, k - 1)])
-                                        : 0.5 * (K1[idx3(g, i, j, 0)] + K1[idx3(g, i + 1, j, 0)]);
+        const double diffloc = hm ? 0.25 * (k1_at(a, c) + k1_at(a, c - 1) + k1_at(a, ce) + k1_at(a, ce - 1))
+                                  : 0.5 * (k1_at(a, c) + k1_at(a, ce));
+        const double2 A01 = *reinterpret_cast<const double2*>(a.Aez + 4u * c);      // (ip=0, kr=0), (ip=0, kr=1)
+        const double2 A23 = *reinterpret_cast<const double2*>(a.Aez + 4u * c + 2);  // (ip=1, kr=0), (ip=1, kr=1)
+        const double t0 = trc, t0u = tr[3u * (c + up)], t0d = tr[3u * (c - dn)];
+        const double t1 = tr[3u * ce], t1u = tr[3u * (ce + up)], t1d = tr[3u * (ce - dn)];
         double sumz = 0.0;
-#pragma unroll
-        for (int kr = 0; kr < 2; ++kr) {
-#pragma unroll
-            for (int ip = 0; ip < 2; ++ip) {
-                // tr[min(k+kr, nz-1)] - tr[max(k+kr-1, 0)]
-                const double up = tr3(tr, g, i + ip, j, kr ? kup : k, lev);
-                const double dn = tr3(tr, g, i + ip, j, kr ? k : kdn, lev);
-                sumz = sumz + diffloc * a.Aez[c * 4 + ip * 2 + kr] * (up - dn);
-            }
-        }
-        const double tre = tr3(tr, g, i + 1, j, k, lev);
-        a.fe[c] = sumz / (4.0 * g.dzt[k]) + (tre - trc) / (g.cost[j] * g.dxu[i]) * a.K11[c];
+        sumz = sumz + diffloc * A01.x * (t0 - t0d);   // kr=0, ip=0
+        sumz = sumz + diffloc * A23.x * (t1 - t1d);   // kr=0, ip=1
+        sumz = sumz + diffloc * A01.y * (t0u - t0);   // kr=1, ip=0
+        sumz = sumz + diffloc * A23.y * (t1u - t1);   // kr=1, ip=1
+        a.fe[c] = sumz / (4.0 * g.dzt[k]) + (t1 - t0) / (g.cost[j] * g.dxu[i]) * a.K11[c];
     }
     if (i >= 2) {  // north face (:49-77)
-        const double* K1 = a.K1;
-        const double diffloc = (k >= 1) ? 0.25 * (K1[idx3(g, i, j, k)] + K1[idx3(g, i, j, k - 1)] +
-                                                  K1[idx3(g, i, j + 1, k)] + K1[idx3(g, i, j + 1, k - 1)])
-                                        : 0.5 * (K1[idx3(g, i, j, 0)] + K1[idx3(g, i, j + 1, 0)]);
+        const double diffloc = hm ? 0.25 * (k1_at(a, c) + k1_at(a, c - 1) + k1_at(a, cn) + k1_at(a, cn - 1))
+                                  : 0.5 * (k1_at(a, c) + k1_at(a, cn));
+        const double2 A01 = *reinterpret_cast<const double2*>(a.Anz + 4u * c);
+        const double2 A23 = *reinterpret_cast<const double2*>(a.Anz + 4u * c + 2);
+        const double t0 = trc, t0u = tr[3u * (c + up)], t0d = tr[3u * (c - dn)];
+        const double t1 = tr[3u * cn], t1u = tr[3u * (cn + up)], t1d = tr[3u * (cn - dn)];
         double sumz = 0.0;
-#pragma unroll
-        for (int kr = 0; kr < 2; ++kr) {
-#pragma unroll
-            for (int jp = 0; jp < 2; ++jp) {
-                const double up = tr3(tr, g, i, j + jp, kr ? kup : k, lev);
-                const double dn = tr3(tr, g, i, j + jp, kr ? k : kdn, lev);
-                sumz = sumz + diffloc * a.Anz[c * 4 + jp * 2 + kr] * (up - dn);
-            }
-        }
-        const double trn = tr3(tr, g, i, j + 1, k, lev);
-        a.fn[c] = g.cosu[j] * (sumz / (4.0 * g.dzt[k]) + (trn - trc) / g.dyu[j] * a.K22[c]);
+        sumz = sumz + diffloc * A01.x * (t0 - t0d);
+        sumz = sumz + diffloc * A23.x * (t1 - t1d);
+        sumz = sumz + diffloc * A01.y * (t0u - t0);
+        sumz = sumz + diffloc * A23.y * (t1u - t1);
+        a.fn[c] = g.cosu[j] * (sumz / (4.0 * g.dzt[k]) + (t1 - t0) / g.dyu[j] * a.K22[c]);
     }
     if (i >= 2 && j >= 2) {  // top face (:79-111)
-        if (k == nz - 1) {
+        if (!hp) {
             a.ft[c] = 0.0;
         } else {
-            const double diffloc = a.K2[c];
+            const double diffloc = k2_at(a, c);
+            const uint32_t cw = c - si, cs = c - sj;
+            const double2 B01 = *reinterpret_cast<const double2*>(a.Abx + 4u * c);      // ip=0: kr=0,1
+            const double2 B23 = *reinterpret_cast<const double2*>(a.Abx + 4u * c + 2);  // ip=1: kr=0,1
+            const double2 C01 = *reinterpret_cast<const double2*>(a.Aby + 4u * c);
+            const double2 C23 = *reinterpret_cast<const double2*>(a.Aby + 4u * c + 2);
+            const double tc0 = trc, tc1 = tr[3u * (c + 1)];
+            const double tw0 = tr[3u * cw], tw1 = tr[3u * (cw + 1)];
+            const double te0 = tr[3u * ce], te1 = tr[3u * (ce + 1)];
+            const double ts0 = tr[3u * cs], ts1 = tr[3u * (cs + 1)];
+            const double tn0 = tr[3u * cn], tn1 = tr[3u * (cn + 1)];
             double sumx = 0.0;
-#pragma unroll
-            for (int ip = 0; ip < 2; ++ip) {
-#pragma unroll
-                for (int kr = 0; kr < 2; ++kr) {
-                    const double d = tr3(tr, g, i + ip, j, k + kr, lev) - tr3(tr, g, i - 1 + ip, j, k + kr, lev);
-                    sumx = sumx + diffloc * a.Abx[c * 4 + ip * 2 + kr] / g.cost[j] * d;
-                }
-            }
+            sumx = sumx + diffloc * B01.x / g.cost[j] * (tc0 - tw0);   // ip=0, kr=0
+            sumx = sumx + diffloc * B01.y / g.cost[j] * (tc1 - tw1);   // ip=0, kr=1
+            sumx = sumx + diffloc * B23.x / g.cost[j] * (te0 - tc0);   // ip=1, kr=0
+            sumx = sumx + diffloc * B23.y / g.cost[j] * (te1 - tc1);   // ip=1, kr=1
             double sumy = 0.0;
-#pragma unroll
-            for (int jp = 0; jp < 2; ++jp) {
-#pragma unroll
-                for (int kr = 0; kr < 2; ++kr) {
-                    const double d = tr3(tr, g, i, j + jp, k + kr, lev) - tr3(tr, g, i, j - 1 + jp, k + kr, lev);
-                    sumy = sumy + diffloc * a.Aby[c * 4 + jp * 2 + kr] * g.cosu[j - 1 + jp] * d;
-                }
-            }
+            sumy = sumy + diffloc * C01.x * g.cosu[j - 1] * (tc0 - ts0);
+            sumy = sumy + diffloc * C01.y * g.cosu[j - 1] * (tc1 - ts1);
+            sumy = sumy + diffloc * C23.x * g.cosu[j] * (tn0 - tc0);
+            sumy = sumy + diffloc * C23.y * g.cosu[j] * (tn1 - tc1);
             a.ft[c] = sumx / (4 * g.dxt[i]) + sumy / (4 * g.dyt[j] * g.cost[j]);
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// explicit part: thread per cell (i, j, k), (i, j) in [1, X-1) x [1, Y-1)
-//  - all such cells (energy on): P_diss += dissipation_on_wgrid(compute_dissipation(...))
-//    (veros/core/diffusion.py:9-62; diss is evaluated at k and k+1 here)
-//  - interior cells: dtr = flux divergence (veros/core/isoneutral/diffusion.py:116-139),
-//    dtr_iso += dtr, tr[taup1] += dt * dtr (:196-197)
+// Column-tile kernel: everything of the tracer step after the fluxes.
+// Block = nz levels x TC columns (consecutive j of one i row), covering
+// (i, j) in [1, X-1) x [1, Y-1).
+//   phase 1 (thread per cell): lateral dissipation on T points
+//     (veros/core/diffusion.py:9-36), flux divergence dtr
+//     (veros/core/isoneutral/diffusion.py:116-139), rhs = tr[taup1] + dt*dtr
+//     (:196-197) and delta = dt/dzw*K_33 (:154-157) -> LDS
+//   phase 2 (lane per interior column): Thomas on the wet suffix, in LDS
+//     (:158-169); the rhs slots become the new tracer (dry slots keep rhs)
+//   phase 3 (thread per cell): tr[taup1] = new, dtr_iso += dtr + (new-rhs)/dt
+//     (:196,204), P += dissipation_on_wgrid(...) (veros/core/diffusion.py:39-62)
+//     + the vertical-mixing term (veros/core/isoneutral/diffusion.py:254-279)
 // ---------------------------------------------------------------------------
 struct ColArgs {
     double* tr;
@@ -478,167 +486,138 @@ struct ColArgs {
 };
 
 // Face fluxes with their valid index ranges (0 elsewhere).
-__device__ __forceinline__ double fe_at(const Grid& g, const ColArgs& a, int i, int j, int k) {
-    return (i >= 1 && i <= g.X - 3 && j >= 2 && j <= g.Y - 3) ? a.fe[idx3(g, i, j, k)] : 0.0;
+__device__ __forceinline__ double fe_at(const Grid& g, const ColArgs& a, int i, int j, uint32_t o) {
+    return (i >= 1 && i <= g.X - 3 && j >= 2 && j <= g.Y - 3) ? a.fe[o] : 0.0;
 }
-__device__ __forceinline__ double fn_at(const Grid& g, const ColArgs& a, int i, int j, int k) {
-    return (i >= 2 && i <= g.X - 3 && j >= 1 && j <= g.Y - 3) ? a.fn[idx3(g, i, j, k)] : 0.0;
-}
-
-// compute_dissipation at one T point (veros/core/diffusion.py:13-33)
-__device__ __forceinline__ double diss_at(const Grid& g, const ColArgs& a, double g05, int i, int j, int k) {
-    const double Ic = tr3(a.I, g, i, j, k, a.tau);
-    const double de = (tr3(a.I, g, i + 1, j, k, a.tau) - Ic) * fe_at(g, a, i, j, k) +
-                      (Ic - tr3(a.I, g, i - 1, j, k, a.tau)) * fe_at(g, a, i - 1, j, k);
-    const double dn = (tr3(a.I, g, i, j + 1, k, a.tau) - Ic) * fn_at(g, a, i, j, k) +
-                      (Ic - tr3(a.I, g, i, j - 1, k, a.tau)) * fn_at(g, a, i, j - 1, k);
-    return g05 * de / (g.dxt[i] * g.cost[j]) + g05 * dn / (g.dyt[j] * g.cost[j]);
+__device__ __forceinline__ double fn_at(const Grid& g, const ColArgs& a, int i, int j, uint32_t o) {
+    return (i >= 2 && i <= g.X - 3 && j >= 1 && j <= g.Y - 3) ? a.fn[o] : 0.0;
 }
 
-__global__ __launch_bounds__(256) void iso_explicit_kernel(Grid g, Prm p, ColArgs a) {
+template <int TC>
+__global__ __launch_bounds__(512) void iso_column_tile_kernel(Grid g, Prm p, ColArgs a, int implicit) {
+    extern __shared__ double lds[];
     const int nz = g.nz;
-    const int ny2 = g.Y - 2;
-    const int64_t t = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
-    const int64_t col = t / nz;
-    if (col >= (int64_t)(g.X - 2) * ny2) return;
-    const int k = (int)(t - col * nz);
-    const int i = 1 + (int)(col / ny2);
-    const int j = 1 + (int)(col - (int64_t)(i - 1) * ny2);
-    const int64_t c = idx3(g, i, j, k);
-    const int kb = kb_of(g, i, j);
+    double* sD = lds;                 // [TC][nz] lateral dissipation on T points
+    double* sR = lds + TC * nz;       // [TC][nz] rhs -> forward sweep -> new tracer
+    double* sA = lds + 2 * TC * nz;   // [TC][nz] sub-diagonal
+    double* sB = lds + 3 * TC * nz;   // [TC][nz] diagonal
+    double* sC = lds + 4 * TC * nz;   // [TC][nz] super-diagonal -> cp
 
-    if (a.P) {
+    const CellIdx ci = cell_of(1, g.X - 2, 1, g.Y - 2);
+    const int i = ci.i, j = ci.j, k = ci.k;
+    const int tc = (int)threadIdx.y;
+    const bool valid = ci.valid;
+    const bool interior = valid && i >= 2 && i <= g.X - 3 && j >= 2 && j <= g.Y - 3;
+    const uint32_t sj = (uint32_t)nz, si = (uint32_t)g.Y * (uint32_t)nz;
+    const uint32_t c = (uint32_t)i * si + (uint32_t)j * sj + (uint32_t)k;
+    const int kb = valid ? g.kbot[i * g.Y + j] : 0;
+    const bool energy = a.P != nullptr;
+    const double dt = p.dt_tracer;
+
+    // ---- phase 1
+    double dtrk = 0.0, rhs = 0.0, ftk = 0.0, K33k = 0.0, Ic = 0.0;
+    if (valid && energy) {
+        const double* __restrict__ I = a.I + a.tau;
+        Ic = I[3u * c];
+        const double de = (I[3u * (c + si)] - Ic) * fe_at(g, a, i, j, c) +
+                          (Ic - I[3u * (c - si)]) * fe_at(g, a, i - 1, j, c - si);
+        const double dn = (I[3u * (c + sj)] - Ic) * fn_at(g, a, i, j, c) +
+                          (Ic - I[3u * (c - sj)]) * fn_at(g, a, i, j - 1, c - sj);
         const double g05 = 0.5 * p.grav / p.rho_0;
-        const double d0 = diss_at(g, a, g05, i, j, k);
+        sD[tc * nz + k] = g05 * de / (g.dxt[i] * g.cost[j]) + g05 * dn / (g.dyt[j] * g.cost[j]);
+    }
+    if (interior) {
+        const double mt = m01(kb, k);
+        ftk = a.ft[c];
+        const double hor = (a.fe[c] - a.fe[c - si]) / (g.cost[j] * g.dxt[i]) +
+                           (a.fn[c] - a.fn[c - sj]) / (g.cost[j] * g.dyt[j]);
+        const double vert = (k == 0) ? ftk / g.dzt[0] : (ftk - a.ft[c - 1]) / g.dzt[k];
+        dtrk = mt * hor + mt * vert;
+        rhs = a.tr[3u * c + a.taup1] + dt * dtrk;
+        sR[tc * nz + k] = rhs;
+        if (implicit) {
+            // rows of the K33 system (veros/core/isoneutral/diffusion.py:154-164); the
+            // edge row k = kbot-1 takes b_edge and a = 0 (veros/core/operators.py:72)
+            K33k = a.K33[c];
+            const double delta = (k < nz - 1) ? dt / g.dzw[k] * K33k : 0.0;
+            const double delta_m = (k >= 1) ? dt / g.dzw[k - 1] * a.K33[c - 1] : 0.0;
+            const double dz = g.dzt[k];
+            const bool edge = k == kb - 1;
+            sA[tc * nz + k] = edge ? 0.0 : -delta_m / dz;
+            sB[tc * nz + k] = edge ? 1 + delta / dz : 1 + (delta + delta_m) / dz;
+            sC[tc * nz + k] = (k < nz - 1) ? -delta / dz : 0.0;
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 2: threads 0..TC-1 (first wave) sweep one column each
+    if (implicit) {
+        const int lin = (int)(threadIdx.y * blockDim.x + threadIdx.x);
+        if (lin < TC) {
+            const int cj = j - tc + lin;  // j - tc is the tile's first column
+            const bool col_ok = i >= 2 && i <= g.X - 3 && cj >= 2 && cj <= g.Y - 3;
+            const int kbl = col_ok ? g.kbot[i * g.Y + cj] : 0;
+            if (col_ok && kbl >= 1 && kbl <= nz) {
+                const int k0 = kbl - 1;
+                double* R = sR + lin * nz;
+                const double* Ar = sA + lin * nz;
+                const double* Br = sB + lin * nz;
+                double* Cc = sC + lin * nz;
+                double cpp = 0.0, dpp = 0.0;
+#pragma unroll 8
+                for (int kk = k0; kk < nz; ++kk) {
+                    const double ak = Ar[kk];
+                    const double r = rcp_nr(Br[kk] - ak * cpp);
+                    cpp = Cc[kk] * r;
+                    dpp = (R[kk] - ak * dpp) * r;
+                    Cc[kk] = cpp;
+                    R[kk] = dpp;
+                }
+                double xn = 0.0;
+#pragma unroll 8
+                for (int kk = nz - 1; kk >= k0; --kk) {
+                    xn = R[kk] - Cc[kk] * xn;
+                    R[kk] = xn;
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- phase 3
+    if (!valid) return;
+    if (interior) {
+        const double newk = sR[tc * nz + k];
+        a.tr[3u * c + a.taup1] = newk;
+        a.dtr[c] = (a.dtr[c] + dtrk) + (newk - rhs) / dt;
+    }
+    if (energy) {
+        const double d0 = sD[tc * nz + k];
         const bool land = kb >= 1;  // create_water_masks(kbot): ks = kbot - 1 >= 0
         double dw;
         if (k == nz - 1) {
             dw = d0 * (land ? 1.0 : 0.0);
         } else {
-            const double d1 = diss_at(g, a, g05, i, j, k + 1);
+            const double d1 = sD[tc * nz + k + 1];
             const double avg = 0.5 * (d0 + d1);
             const bool edge = land && k == kb - 1;
             const bool water = land && k > kb - 1;
             const double dzw_lo = g.dzw[k >= 1 ? k - 1 : 0];  // pad_z_edges(dzw)[k]
             dw = (avg + 0.5 * (d0 * dzw_lo / g.dzw[k])) * (edge ? 1.0 : 0.0) + avg * (water ? 1.0 : 0.0);
         }
-        a.P[c] = a.P[c] + dw;
-    }
-    if (i < 2 || i > g.X - 3 || j < 2 || j > g.Y - 3) return;
-
-    const double mt = mT(kb, k);
-    const double ftk = a.ft[c];
-    const double hor = (a.fe[c] - fe_at(g, a, i - 1, j, k)) / (g.cost[j] * g.dxt[i]) +
-                       (a.fn[c] - fn_at(g, a, i, j - 1, k)) / (g.cost[j] * g.dyt[j]);
-    const double vert = (k == 0) ? ftk / g.dzt[0] : (ftk - a.ft[c - 1]) / g.dzt[k];
-    const double dtrk = mt * hor + mt * vert;
-    a.dtr[c] = a.dtr[c] + dtrk;
-    double* trp = a.tr + c * 3 + a.taup1;
-    *trp = *trp + p.dt_tracer * dtrk;
-}
-
-// ---------------------------------------------------------------------------
-// implicit K_33 solve (veros/core/isoneutral/diffusion.py:142-169,202-204):
-// lane per interior column, Thomas in registers over the wet suffix;
-// tr[taup1] <- solution, dtr_iso += (solution - rhs) / dt on wet cells
-// (dry cells keep rhs, so their increment is 0).
-// ---------------------------------------------------------------------------
-template <int NZB, bool VEC>
-__global__ __launch_bounds__(256) void iso_implicit_kernel(Grid g, Prm p, ColArgs a) {
-    const int nz = g.nz;
-    const int ny4 = g.Y - 4;
-    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= (int64_t)(g.X - 4) * ny4) return;
-    const int i = 2 + (int)(col / ny4);
-    const int j = 2 + (int)(col - (int64_t)(i - 2) * ny4);
-    const int kb = kb_of(g, i, j);
-    if (kb < 1 || kb > nz) return;
-    const int k0 = kb - 1;
-    const int64_t base = idx3(g, i, j, 0);
-    const double* __restrict__ K33 = a.K33 + base;
-    double* __restrict__ trc = a.tr + base * 3 + a.taup1;
-    double* __restrict__ dtr = a.dtr + base;
-    const double dt = p.dt_tracer;
-
-    double cp[NZB], dp[NZB];
-    double cpp = 0.0, dpp = 0.0, delta_m = 0.0;
-#pragma unroll
-    for (int q = 0; q < NZB; q += 8) {
-        if (q < nz && q + 8 > k0) {
-            double kv[8] = {}, rv[8] = {};
-            if constexpr (VEC) {
-#pragma unroll
-                for (int t = 0; t < 8; t += 2) {
-                    if (q + t < nz) {
-                        const double2 w = *reinterpret_cast<const double2*>(K33 + q + t);
-                        kv[t] = w.x;
-                        kv[t + 1] = w.y;
-                    }
-                }
+        double Pk = a.P[c] + dw;
+        if (interior && k < nz - 1) {
+            const double mw = m01(kb, k) * m01(kb, k + 1);
+            const double fxa = (-a.I[3u * (c + 1) + a.tau] + Ic) / g.dzw[k];
+            const double gr = -p.grav / p.rho_0;
+            if (implicit) {
+                const double dtrz = sR[tc * nz + k + 1] - sR[tc * nz + k];
+                Pk = Pk + gr * fxa * (ftk * mw + K33k * dtrz / g.dzw[k] * mw);
             } else {
-#pragma unroll
-                for (int t = 0; t < 8; ++t)
-                    if (q + t < nz) kv[t] = K33[q + t];
-            }
-#pragma unroll
-            for (int t = 0; t < 8; ++t)
-                if (q + t < nz && q + t >= k0) rv[t] = trc[(q + t) * 3];
-#pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                const int k = q + t;
-                if (k < nz && k >= k0) {
-                    const double delta = (k < nz - 1) ? dt / g.dzw[k] * kv[t] : 0.0;
-                    const double dz = g.dzt[k];
-                    const bool edge = k == k0;
-                    const double ak = edge ? 0.0 : -delta_m / dz;
-                    const double bk = edge ? 1 + delta / dz : 1 + (delta + delta_m) / dz;
-                    const double ck = (k < nz - 1) ? -delta / dz : 0.0;
-                    const double r = 1.0 / (bk - ak * cpp);
-                    cpp = ck * r;
-                    dpp = (rv[t] - ak * dpp) * r;
-                    delta_m = delta;
-                }
-                cp[k] = cpp;
-                dp[k] = dpp;
+                Pk = Pk + gr * fxa * ftk * mw;
             }
         }
-    }
-    double xn = 0.0;
-#pragma unroll
-    for (int k = NZB - 1; k >= 0; --k) {
-        if (k < nz && k >= k0) {
-            const double xk = dp[k] - cp[k] * xn;
-            const double rhs = trc[k * 3];
-            trc[k * 3] = xk;
-            dtr[k] = dtr[k] + (xk - rhs) / dt;
-            xn = xk;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// vertical-mixing dissipation (veros/core/isoneutral/diffusion.py:254-279):
-// thread per interior cell, k < nz-1, after the implicit solve.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void iso_post_kernel(Grid g, Prm p, ColArgs a, int implicit) {
-    const int nz1 = g.nz - 1;
-    const int ny4 = g.Y - 4;
-    const int64_t t = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
-    const int64_t col = t / nz1;
-    if (col >= (int64_t)(g.X - 4) * ny4) return;
-    const int k = (int)(t - col * nz1);
-    const int i = 2 + (int)(col / ny4);
-    const int j = 2 + (int)(col - (int64_t)(i - 2) * ny4);
-    const int64_t c = idx3(g, i, j, k);
-    const int kb = kb_of(g, i, j);
-    const double mw = mT(kb, k) * mT(kb, k + 1);
-    const double fxa = (-tr3(a.I, g, i, j, k + 1, a.tau) + tr3(a.I, g, i, j, k, a.tau)) / g.dzw[k];
-    const double gr = -p.grav / p.rho_0;
-    if (implicit) {
-        const double dtrz = a.tr[(c + 1) * 3 + a.taup1] - a.tr[c * 3 + a.taup1];
-        a.P[c] = a.P[c] + gr * fxa * (a.ft[c] * mw + a.K33[c] * dtrz / g.dzw[k] * mw);
-    } else {
-        a.P[c] = a.P[c] + gr * fxa * a.ft[c] * mw;
+        a.P[c] = Pk;
     }
 }
 
@@ -731,33 +710,19 @@ extern "C" int64_t vx_iso_workspace_bytes(const vx_grid* grid) {
 namespace vx {
 namespace {
 
-int launch_implicit(const Grid& g, const Prm& p, const ColArgs& a, hipStream_t s) {
-    const int64_t ncol = (int64_t)(g.X - 4) * (g.Y - 4);
-    const dim3 grid(blocks_for(ncol, 256)), block(256);
-    const int nzb = ((g.nz + 15) / 16) * 16;
-    const bool vec = (g.nz % 2 == 0) && aligned16(a.K33);
-#define VX_IMPL_CASE(N)                                                                   \
-    case N:                                                                               \
-        if (vec) hipLaunchKernelGGL((iso_implicit_kernel<N, true>), grid, block, 0, s, g, p, a); \
-        else hipLaunchKernelGGL((iso_implicit_kernel<N, false>), grid, block, 0, s, g, p, a);    \
-        break;
-    switch (nzb) {
-        VX_IMPL_CASE(16)
-        VX_IMPL_CASE(32)
-        VX_IMPL_CASE(48)
-        VX_IMPL_CASE(64)
-        VX_IMPL_CASE(80)
-        VX_IMPL_CASE(96)
-        VX_IMPL_CASE(112)
-        VX_IMPL_CASE(128)
-        default: return fail(VX_ERR_UNSUPPORTED, "iso implicit kernel: nz=%d", g.nz);
-    }
-#undef VX_IMPL_CASE
-    return check_launch("iso_implicit_kernel");
+constexpr int kTileColumns = 4;
+
+int launch_tile(const Grid& g, const Prm& p, const ColArgs& a, int implicit, hipStream_t s) {
+    constexpr int TC = kTileColumns;
+    if (g.nz * TC > 512) return fail(VX_ERR_UNSUPPORTED, "iso column tile: nz=%d too deep", g.nz);
+    const dim3 blk(g.nz, TC);
+    const size_t lds = (size_t)5 * TC * g.nz * sizeof(double);
+    hipLaunchKernelGGL(iso_column_tile_kernel<TC>, cell_grid(g.X - 2, g.Y - 2, blk), blk, lds, s, g, p, a, implicit);
+    return check_launch("iso_column_tile_kernel");
 }
 
 int iso_diffusion_common(void* stream, const vx_grid* grid, const vx_iso_params* prm, double* tracer, int32_t tau,
-                         int32_t taup1, const double* K1, const double* K2, const double* Ai_ez, const double* Ai_nz,
+                         int32_t taup1, const double* Kiso, const double* Kskew, const double* Ai_ez, const double* Ai_nz,
                          const double* Ai_bx, const double* Ai_by, const double* K_11, const double* K_22,
                          const double* K_33, double* dtr_iso, const double* int_drhodX, double* P_diss,
                          void* workspace, int implicit, const char* who) {
@@ -767,7 +732,7 @@ int iso_diffusion_common(void* stream, const vx_grid* grid, const vx_iso_params*
     if (!prm) return fail(VX_ERR_INVALID_ARGUMENT, "%s: null params", who);
     if (tau < 0 || tau > 2 || taup1 < 0 || taup1 > 2)
         return fail(VX_ERR_INVALID_ARGUMENT, "%s: time levels must be 0..2", who);
-    if (!tracer || !K1 || !K2 || !Ai_ez || !Ai_nz || !Ai_bx || !Ai_by || !K_11 || !K_22 || !dtr_iso || !workspace ||
+    if (!tracer || (!Kiso && !Kskew) || !Ai_ez || !Ai_nz || !Ai_bx || !Ai_by || !K_11 || !K_22 || !dtr_iso || !workspace ||
         (implicit && !K_33))
         return fail(VX_ERR_INVALID_ARGUMENT, "%s: null array pointer", who);
     if ((P_diss == nullptr) != (int_drhodX == nullptr))
@@ -776,20 +741,13 @@ int iso_diffusion_common(void* stream, const vx_grid* grid, const vx_iso_params*
     const Prm p = make_prm(prm);
     const int64_t n3 = (int64_t)g.X * g.Y * g.nz;
     double* ws = static_cast<double*>(workspace);
-    FluxArgs fa{tracer, tau, K1, K2, Ai_ez, Ai_nz, Ai_bx, Ai_by, K_11, K_22, ws, ws + n3, ws + 2 * n3};
-    const int64_t n = (int64_t)(g.X - 3) * (g.Y - 3) * g.nz;
-    hipLaunchKernelGGL(iso_flux_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, g, fa);
+    FluxArgs fa{tracer, tau, Kiso, Kskew, Ai_ez, Ai_nz, Ai_bx, Ai_by, K_11, K_22, ws, ws + n3, ws + 2 * n3};
+    if (n3 * 4 >= ((int64_t)1 << 32)) return fail(VX_ERR_UNSUPPORTED, "%s: grid too large for 32-bit offsets", who);
+    const dim3 blk = cell_block(g.nz);
+    hipLaunchKernelGGL(iso_flux_kernel, cell_grid(g.X - 3, g.Y - 3, blk), blk, 0, s, g, fa);
     if ((rc = check_launch(who))) return rc;
     ColArgs ca{tracer, tau, taup1, K_33, dtr_iso, int_drhodX, P_diss, ws, ws + n3, ws + 2 * n3};
-    const int64_t ne = (int64_t)(g.X - 2) * (g.Y - 2) * g.nz;
-    hipLaunchKernelGGL(iso_explicit_kernel, dim3(blocks_xcd(ne, 256)), dim3(256), 0, s, g, p, ca);
-    if ((rc = check_launch(who))) return rc;
-    if (implicit && (rc = launch_implicit(g, p, ca, s))) return rc;
-    if (P_diss) {
-        const int64_t np_ = (int64_t)(g.X - 4) * (g.Y - 4) * (g.nz - 1);
-        hipLaunchKernelGGL(iso_post_kernel, dim3(blocks_xcd(np_, 256)), dim3(256), 0, s, g, p, ca, implicit);
-        if ((rc = check_launch(who))) return rc;
-    }
+    if ((rc = launch_tile(g, p, ca, implicit, s))) return rc;
     return VX_OK;
 }
 }  // namespace
@@ -800,7 +758,20 @@ extern "C" int vx_iso_diffusion_f64(void* stream, const vx_grid* grid, const vx_
                                     const double* Ai_nz, const double* Ai_bx, const double* Ai_by,
                                     const double* K_11, const double* K_22, const double* K_33, double* dtr_iso,
                                     const double* int_drhodX, double* P_diss, void* workspace) {
-    return vx::iso_diffusion_common(stream, grid, prm, tracer, tau, taup1, K_iso, K_iso, Ai_ez, Ai_nz, Ai_bx, Ai_by,
+    if (!K_iso) return vx::fail(VX_ERR_INVALID_ARGUMENT, "vx_iso_diffusion_f64: null K_iso");
+    return vx::iso_diffusion_common(stream, grid, prm, tracer, tau, taup1, K_iso, nullptr, Ai_ez, Ai_nz, Ai_bx, Ai_by,
                                     K_11, K_22, K_33, dtr_iso, int_drhodX, P_diss, workspace, 1,
                                     "vx_iso_diffusion_f64");
+}
+
+extern "C" int vx_iso_skew_diffusion_f64(void* stream, const vx_grid* grid, const vx_iso_params* prm,
+                                         double* tracer, int32_t tau, int32_t taup1, const double* K_gm,
+                                         const double* Ai_ez, const double* Ai_nz, const double* Ai_bx,
+                                         const double* Ai_by, const double* K_11, const double* K_22,
+                                         double* dtr_iso, const double* int_drhodX, double* P_diss_skew,
+                                         void* workspace) {
+    if (!K_gm) return vx::fail(VX_ERR_INVALID_ARGUMENT, "vx_iso_skew_diffusion_f64: null K_gm");
+    return vx::iso_diffusion_common(stream, grid, prm, tracer, tau, taup1, nullptr, K_gm, Ai_ez, Ai_nz, Ai_bx,
+                                    Ai_by, K_11, K_22, nullptr, dtr_iso, int_drhodX, P_diss_skew, workspace, 0,
+                                    "vx_iso_skew_diffusion_f64");
 }

@@ -6,6 +6,7 @@
   dtemp_iso|dsalt_iso[, P_diss_iso])``  (veros/core/isoneutral/diffusion.py:210-283)
 * ``isoneutral_diffusion(state, tr, istemp)``, the routine that stores the
   kernel's outputs in ``state.variables`` (veros/core/isoneutral/diffusion.py:286-295)
+* ``isoneutral_skew_diffusion(state, tr, istemp)`` (veros/core/isoneutral/diffusion.py:298-307)
 
 ``state`` is any object with ``.variables`` and ``.settings`` holding the veros
 variables and settings these functions read (a ``VerosState`` whose arrays
@@ -116,43 +117,55 @@ def isoneutral_diffusion_pre(state):
 
 
 def isoneutral_diffusion_kernel(state, tr, istemp, iso=True):
-    """One tracer's isoneutral step, in place on ``tr`` and the state's
-    dtemp_iso/dsalt_iso and P_diss_iso."""
-    if not iso:
-        raise NotImplementedError("the skew-diffusion variant (iso=False) is not built yet")
+    """One tracer's isoneutral (``iso=True``) or skew (``iso=False``, K_skew =
+    K_gm) step, in place on ``tr`` and the state's dtemp_iso/dsalt_iso and
+    P_diss_iso / P_diss_skew (veros/core/isoneutral/diffusion.py:210-283)."""
     vs, st = state.variables, state.settings
-    X, Y, nz = vs.K_iso.shape
+    X, Y, nz = vs.K_iso.shape if iso else vs.K_gm.shape
     keep = []
     g = grid_struct(vs, keep)
     p = params_struct(st)
     tr = _check_array("tr", tr, (X, Y, nz, 3))
     dname = "dtemp_iso" if istemp else "dsalt_iso"
     dtr = _check_array(dname, getattr(vs, dname), (X, Y, nz))
-    arrays = [_check_array("K_iso", vs.K_iso, (X, Y, nz))]
+    Kname = "K_iso" if iso else "K_gm"
+    arrays = [_check_array(Kname, getattr(vs, Kname), (X, Y, nz))]
     arrays += [_check_array(n, getattr(vs, n), (X, Y, nz, 2, 2)) for n in ("Ai_ez", "Ai_nz", "Ai_bx", "Ai_by")]
-    arrays += [_check_array(n, getattr(vs, n), (X, Y, nz)) for n in ("K_11", "K_22", "K_33")]
+    arrays += [_check_array(n, getattr(vs, n), (X, Y, nz)) for n in ("K_11", "K_22")]
+    if iso:
+        arrays.append(_check_array("K_33", vs.K_33, (X, Y, nz)))
     energy = bool(st.enable_conserve_energy)
+    pname = "P_diss_iso" if iso else "P_diss_skew"
     if energy:
         I = _check_array("int_drhodT" if istemp else "int_drhodS",
                          vs.int_drhodT if istemp else vs.int_drhodS, (X, Y, nz, 3))
-        P = _check_array("P_diss_iso", vs.P_diss_iso, (X, Y, nz))
+        P = _check_array(pname, getattr(vs, pname), (X, Y, nz))
     else:
         I = P = None
     L = _native.lib()
     nbytes = L.vx_iso_workspace_bytes(g)
     ws = workspace(nbytes, tr.device)
-    rc = L.vx_iso_diffusion_f64(stream_handle(), g, p, ptr(tr), int(vs.tau), int(vs.taup1),
-                                *(ptr(a) for a in arrays), ptr(dtr), ptr(I), ptr(P), ptr(ws))
-    _native.check(rc, "isoneutral_diffusion")
+    fn = L.vx_iso_diffusion_f64 if iso else L.vx_iso_skew_diffusion_f64
+    rc = fn(stream_handle(), g, p, ptr(tr), int(vs.tau), int(vs.taup1),
+            *(ptr(a) for a in arrays), ptr(dtr), ptr(I), ptr(P), ptr(ws))
+    _native.check(rc, "isoneutral_diffusion" if iso else "isoneutral_skew_diffusion")
     out = {("temp" if istemp else "salt"): tr, dname: dtr}
     if energy:
-        out["P_diss_iso"] = P
+        out[pname] = P
     return KernelOutput(**out)
+
+
+def _store(state, out):
+    for k, v in out._asdict().items():
+        setattr(state.variables, k, v)
 
 
 def isoneutral_diffusion(state, tr, istemp):
     """The routine: run the kernel and store its outputs in state.variables
     (veros/core/isoneutral/diffusion.py:286-295)."""
-    out = isoneutral_diffusion_kernel(state, tr, istemp, iso=True)
-    for k, v in out._asdict().items():
-        setattr(state.variables, k, v)
+    _store(state, isoneutral_diffusion_kernel(state, tr, istemp, iso=True))
+
+
+def isoneutral_skew_diffusion(state, tr, istemp):
+    """Skew (GM) variant of the routine (veros/core/isoneutral/diffusion.py:298-307)."""
+    _store(state, isoneutral_diffusion_kernel(state, tr, istemp, iso=False))
