@@ -180,6 +180,9 @@ def main():
         isoneutral.isoneutral_diffusion_pre(state)
 
     def run_diff():
+        isoneutral.isoneutral_diffusion_temp_salt(state)
+
+    def run_diff_two_calls():
         isoneutral.isoneutral_diffusion(state, vs.temp, True)
         isoneutral.isoneutral_diffusion(state, vs.salt, False)
 
@@ -196,7 +199,8 @@ def main():
     # per-part timings (median, min) in ms
     parts = {}
     for name, fn in (("tdma_fp64", lambda: run_tdma("fp64")), ("tdma_mixed", lambda: run_tdma("mixed")),
-                     ("iso_pre", run_pre), ("iso_diffusion_TS", run_diff)):
+                     ("iso_pre", run_pre), ("iso_diffusion_TS", run_diff),
+                     ("iso_diffusion_T_then_S", run_diff_two_calls)):
         for _ in range(args.warmup):
             fn()
         parts[name] = timed(fn, max(5, args.steps))
@@ -235,6 +239,7 @@ def main():
         "tdma_mixed": (B_td, parts["tdma_mixed"][0]),
         "iso_pre": (B_pre, parts["iso_pre"][0]),
         "iso_diffusion_TS": (B_diff, parts["iso_diffusion_TS"][0]),
+        "iso_diffusion_T_then_S": (B_diff, parts["iso_diffusion_T_then_S"][0]),
     }
     gbs = {k: b / (t * 1e-3) / 1e9 for k, (b, t) in kern.items()}
     dom = max(("iso_pre", "iso_diffusion_TS", f"tdma_{prec}"), key=lambda k: kern[k][1])

@@ -128,9 +128,10 @@ int vx_iso_pre_f64(void* stream, const vx_grid* grid, const vx_iso_params* prm,
  * dtr_iso accumulates both tendencies, and, when P_diss is non-NULL
  * (enable_conserve_energy), P_diss accumulates the dissipation of dynamic
  * enthalpy computed from int_drhodX[..., tau] (veros/core/diffusion.py:9-62).
- * `workspace` must hold at least vx_iso_workspace_bytes(grid) bytes.
+ * `workspace` must hold vx_iso_workspace_bytes(grid, 1) bytes (the face
+ * fluxes); calls that share a workspace must be ordered on one stream.
  */
-int64_t vx_iso_workspace_bytes(const vx_grid* grid);
+int64_t vx_iso_workspace_bytes(const vx_grid* grid, int32_t ntracers);
 int vx_iso_diffusion_f64(void* stream, const vx_grid* grid, const vx_iso_params* prm,
                          double* tracer, int32_t tau, int32_t taup1,
                          const double* K_iso,
@@ -142,12 +143,30 @@ int vx_iso_diffusion_f64(void* stream, const vx_grid* grid, const vx_iso_params*
                          void* workspace);
 
 /*
+ * isoneutral_diffusion(temp) followed by isoneutral_diffusion(salt), as
+ * thermodynamics calls them back to back (veros/core/thermodynamics.py:431-432),
+ * sharing every read of the isoneutral tensor. Results equal two calls of
+ * vx_iso_diffusion_f64 (temperature first), including the order in which
+ * P_diss accumulates the two tracers' terms. `workspace` must hold
+ * vx_iso_workspace_bytes(grid, 2) bytes.
+ */
+int vx_iso_diffusion_ts_f64(void* stream, const vx_grid* grid, const vx_iso_params* prm,
+                            double* temp, double* salt, int32_t tau, int32_t taup1,
+                            const double* K_iso,
+                            const double* Ai_ez, const double* Ai_nz,
+                            const double* Ai_bx, const double* Ai_by,
+                            const double* K_11, const double* K_22, const double* K_33,
+                            double* dtemp_iso, double* dsalt_iso,
+                            const double* int_drhodT, const double* int_drhodS, double* P_diss,
+                            void* workspace);
+
+/*
  * isoneutral_skew_diffusion for one tracer (veros/core/isoneutral/diffusion.py:172-207,
  * 210-283, 298-307, iso=False): the same flux stencil with K_iso = 0 and
  * K_skew = K_gm, the explicit update of tracer[..., taup1] and dtr_iso, no
  * implicit part; P_diss_skew (when non-NULL, with int_drhodX) accumulates the
- * lateral dissipation and the explicit vertical term. Same workspace size
- * as vx_iso_diffusion_f64.
+ * lateral dissipation and the explicit vertical term. `workspace` as for
+ * vx_iso_diffusion_f64.
  */
 int vx_iso_skew_diffusion_f64(void* stream, const vx_grid* grid, const vx_iso_params* prm,
                               double* tracer, int32_t tau, int32_t taup1,

@@ -68,6 +68,14 @@ def test_iso_argument_errors_without_gpu():
     for name in ("kbot", "dxt", "dxu", "dyt", "dyu", "cost", "cosu", "dzt", "dzw"):
         setattr(g, name, 16)
     assert L.vx_iso_pre_f64(None, g, p, 1, 1, 5, 1, 1, 1, 1, 1, 1, 1, 1) == _native.VX_ERR_INVALID_ARGUMENT  # tau
-    assert L.vx_iso_workspace_bytes(g) == 3 * 8 * 8 * 8 * 8
+    assert L.vx_iso_workspace_bytes(g, 1) == 3 * 8 * 8 * 8 * 8
+    assert L.vx_iso_workspace_bytes(g, 2) == 6 * 8 * 8 * 8 * 8
+    assert L.vx_iso_workspace_bytes(g, 3) == -1
     rc = L.vx_iso_diffusion_f64(None, g, p, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, None, 1, 1)
     assert rc == _native.VX_ERR_INVALID_ARGUMENT  # P_diss without int_drhodX
+    rc = L.vx_iso_diffusion_f64(None, g, p, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, None, None, None)
+    assert rc == _native.VX_ERR_INVALID_ARGUMENT  # no workspace
+    rc = L.vx_iso_diffusion_ts_f64(None, g, p, 1, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1)
+    assert rc == _native.VX_ERR_INVALID_ARGUMENT  # temp and salt alias
+    rc = L.vx_iso_skew_diffusion_f64(None, g, p, 1, 0, 1, None, 1, 1, 1, 1, 1, 1, 1, None, None, 1)
+    assert rc == _native.VX_ERR_INVALID_ARGUMENT  # null K_gm

@@ -106,19 +106,43 @@ def test_config1_step_against_oracle():
         assert rel_err(_np(getattr(out, k)), getattr(ref, k)) <= TOL, k
         setattr(st.variables, k, getattr(ref, k))
     P0 = st.variables.P_diss_iso.copy()
+    st0 = W.state_to_numpy_copy(st)
+    for k in PRE_KEYS:
+        setattr(st0.variables, k, _np(getattr(out, k)))
     O.isoneutral_diffusion(st, "temp", True)
     P1 = st.variables.P_diss_iso.copy()
     O.isoneutral_diffusion(st, "salt", False)
+    ds2 = _dev_state(st0)
     isoneutral.isoneutral_diffusion(ds, ds.variables.temp, True)
     isoneutral.isoneutral_diffusion(ds, ds.variables.salt, False)
-    errs = {k: rel_err(_np(getattr(ds.variables, k)), getattr(st.variables, k))
-            for k in ("temp", "salt", "dtemp_iso", "dsalt_iso")}
+    isoneutral.isoneutral_diffusion_temp_salt(ds2)
     Pr = st.variables.P_diss_iso
     scale = max(np.abs(Pr).max(), np.abs(P1 - P0).max(), np.abs(Pr - P1).max())
-    errs["P_diss_iso"] = float(np.abs(_np(ds.variables.P_diss_iso) - Pr).max() / scale)
-    print("errors", errs)
-    for k, e in errs.items():
-        assert e <= TOL, (k, e)
+    for dev in (ds, ds2):
+        errs = {k: rel_err(_np(getattr(dev.variables, k)), getattr(st.variables, k))
+                for k in ("temp", "salt", "dtemp_iso", "dsalt_iso")}
+        errs["P_diss_iso"] = float(np.abs(_np(dev.variables.P_diss_iso) - Pr).max() / scale)
+        print("errors", errs)
+        for k, e in errs.items():
+            assert e <= TOL, (k, e)
+
+
+@pytest.mark.parametrize("energy", [True, False])
+@pytest.mark.parametrize("shape,cyclic", [((31, 23, 19), True), ((14, 17, 3), False)])
+def test_temp_salt_fused_matches_two_calls(shape, cyclic, energy):
+    """isoneutral_diffusion_temp_salt == isoneutral_diffusion(temp) then (salt)."""
+    from veros_amd import isoneutral
+    st = _random_state(7 + sum(shape), *shape, cyclic=cyclic)
+    st.settings.enable_conserve_energy = energy
+    pre = O.isoneutral_diffusion_pre(st)
+    for k, v in pre._asdict().items():
+        setattr(st.variables, k, v)
+    ds = _dev_state(st)
+    O.isoneutral_diffusion(st, "temp", True)
+    O.isoneutral_diffusion(st, "salt", False)
+    isoneutral.isoneutral_diffusion_temp_salt(ds)
+    for k in ("temp", "salt", "dtemp_iso", "dsalt_iso", "P_diss_iso"):
+        assert rel_err(_np(getattr(ds.variables, k)), getattr(st.variables, k)) <= TOL, k
 
 
 def test_deterministic_pre():

@@ -51,11 +51,11 @@ _workspaces: dict = {}
 
 
 def workspace(nbytes: int, dev: torch.device) -> torch.Tensor:
-    """A cached device buffer of at least ``nbytes`` (grown, never shrunk).
-    Calls that use it must be ordered on one stream."""
-    key = dev
-    buf = _workspaces.get(key)
+    """A cached device buffer of at least ``nbytes`` (grown, never shrunk),
+    allocated on first use so the hot calls allocate nothing afterwards.
+    Calls that share it must be ordered on one stream."""
+    buf = _workspaces.get(dev)
     if buf is None or buf.numel() < nbytes:
-        buf = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=dev)
-        _workspaces[key] = buf
+        buf = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=dev)
+        _workspaces[dev] = buf
     return buf

@@ -47,10 +47,13 @@ _SIGNATURES = {
                                                                   c_i64, c_i64, c_i32, c_i64, c_i64, c_i32]),
     "vx_iso_pre_f64": (ctypes.c_int, [c_void_p, ctypes.POINTER(VxGrid), ctypes.POINTER(VxIsoParams),
                                       c_void_p, c_void_p, c_i32, c_void_p] + [c_void_p] * 7),
-    "vx_iso_workspace_bytes": (c_i64, [ctypes.POINTER(VxGrid)]),
+    "vx_iso_workspace_bytes": (c_i64, [ctypes.POINTER(VxGrid), c_i32]),
     "vx_iso_diffusion_f64": (ctypes.c_int, [c_void_p, ctypes.POINTER(VxGrid), ctypes.POINTER(VxIsoParams),
                                             c_void_p, c_i32, c_i32, c_void_p] + [c_void_p] * 7
                              + [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "vx_iso_diffusion_ts_f64": (ctypes.c_int, [c_void_p, ctypes.POINTER(VxGrid), ctypes.POINTER(VxIsoParams),
+                                               c_void_p, c_void_p, c_i32, c_i32, c_void_p] + [c_void_p] * 7
+                                + [c_void_p] * 6),
     "vx_iso_skew_diffusion_f64": (ctypes.c_int, [c_void_p, ctypes.POINTER(VxGrid), ctypes.POINTER(VxIsoParams),
                                                  c_void_p, c_i32, c_i32, c_void_p] + [c_void_p] * 6
                                   + [c_void_p, c_void_p, c_void_p, c_void_p]),

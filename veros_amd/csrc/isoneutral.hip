@@ -15,6 +15,15 @@
 // maskU/V = min of the two adjacent maskT, maskW = min(maskT[k], maskT[k+1]).
 #include "vx_common.hpp"
 
+// Tuning knobs (compile-time): minimum waves per SIMD requested for the pre
+// kernel (caps its VGPRs), and interior columns per block of the column-tile kernel.
+#ifndef VX_PRE_MIN_WAVES
+#define VX_PRE_MIN_WAVES 4
+#endif
+#ifndef VX_TILE_COLUMNS
+#define VX_TILE_COLUMNS 2
+#endif
+
 namespace vx {
 
 constexpr double kEpsln = 1e-20;  // veros/core/isoneutral/isoneutral.py:28
@@ -180,7 +189,8 @@ __device__ __forceinline__ double m01(int kb, int k) { return (kb > 0 && kb - 1 
 // gradients are formed from them in registers. Offsets are 32-bit element
 // offsets (the host checks the arrays fit).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void iso_pre_kernel(Grid g, Prm p, Fields f, const double* __restrict__ K,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_PRE_MIN_WAVES))) void iso_pre_kernel(
+    Grid g, Prm p, Fields f, const double* __restrict__ K,
                                                       double* __restrict__ Aez, double* __restrict__ Anz,
                                                       double* __restrict__ Abx, double* __restrict__ Aby,
                                                       double* __restrict__ K11, double* __restrict__ K22,
@@ -365,260 +375,307 @@ __global__ void k33_top_zero_kernel(Grid g, double* __restrict__ K33) {
 }
 
 // ---------------------------------------------------------------------------
-// tracer fluxes: thread per cell, i in [1, X-2), j in [1, Y-2)
-// fe is written for j in [2, Y-2), fn for i in [2, X-2), ft for i, j >= 2
-// (ft = 0 at the top level). Other workspace entries are never read.
+// Fused tracer step (veros/core/isoneutral/diffusion.py:9-283 with
+// veros/core/diffusion.py:9-62), NTR tracers sharing one pass over the
+// isoneutral tensor. Block = nz levels x TC columns (consecutive j of one i
+// row), covering (i, j) in [1, X-1) x [1, Y-1).
+//   iso_flux_kernel (thread per cell): the face fluxes -> workspace
+//   tile kernel phase 1 (thread per cell): lateral dissipation on T points,
+//     the flux divergence, rhs = tr[taup1] + dt*dtr, and the K33 rows -> LDS
+//   phase 2 (lane per interior column and tracer): Thomas sweep in LDS
+//   phase 3 (thread per cell): tracer, dtr_iso and P_diss updates
 // ---------------------------------------------------------------------------
-struct FluxArgs {
-    const double* __restrict__ tr;
-    int tau;
-    // K1 = K_iso - K_skew (east/north faces), K2 = K_iso + K_skew (top faces),
-    // either term absent when its pointer is null (veros/core/isoneutral/diffusion.py:15-16,180-188)
-    const double* __restrict__ Kiso;
-    const double* __restrict__ Kskew;
-    const double *Aez, *Anz, *Abx, *Aby, *K11, *K22;
-    double *fe, *fn, *ft;
+struct TracerArgs {
+    double* tr;           // (X, Y, nz, 3)
+    double* dtr;          // (X, Y, nz)
+    const double* I;      // int_drhodX (X, Y, nz, 3), nullable
+    double *fe, *fn, *ft; // workspace: flux_east, flux_north, flux_top (X, Y, nz)
 };
 
-__device__ __forceinline__ double k1_at(const FluxArgs& a, uint32_t o) {
+struct StepArgs {
+    TracerArgs t[2];
+    int tau, taup1;
+    // K1 = K_iso - K_skew (east/north faces), K2 = K_iso + K_skew (top faces);
+    // a null pointer is a zero term (veros/core/isoneutral/diffusion.py:15-16,180-188)
+    const double* Kiso;
+    const double* Kskew;
+    const double *Aez, *Anz, *Abx, *Aby, *K11, *K22, *K33;
+    double* P;            // P_diss_iso / P_diss_skew, nullable (energy branch off)
+};
+
+__device__ __forceinline__ double k1_at(const StepArgs& a, uint32_t o) {
     const double ki = a.Kiso ? a.Kiso[o] : 0.0;
     return a.Kskew ? ki - a.Kskew[o] : ki;
 }
-__device__ __forceinline__ double k2_at(const FluxArgs& a, uint32_t o) {
+__device__ __forceinline__ double k2_at(const StepArgs& a, uint32_t o) {
     const double ki = a.Kiso ? a.Kiso[o] : 0.0;
     return a.Kskew ? ki + a.Kskew[o] : ki;
 }
 
-__global__ __launch_bounds__(256) void iso_flux_kernel(Grid g, FluxArgs a) {
+struct Strides {
+    uint32_t si, sj;
+};
+
+// flux_east at (i, j, k), 0 outside i in [1, X-3], j in [2, Y-3]   (:22-47)
+__device__ __forceinline__ double flux_east(const Grid& g, const StepArgs& a, const double* __restrict__ tr,
+                                            const Strides& st, int i, int j, int k) {
+    if (i < 1 || i > g.X - 3 || j < 2 || j > g.Y - 3) return 0.0;
+    const uint32_t c = (uint32_t)i * st.si + (uint32_t)j * st.sj + (uint32_t)k;
+    const uint32_t ce = c + st.si;
+    const bool hm = k >= 1;
+    const uint32_t up = (k + 1 < g.nz) ? 1u : 0u, dn = hm ? 1u : 0u;  // pad_z_edges clamping
+    const double diffloc = hm ? 0.25 * (k1_at(a, c) + k1_at(a, c - 1) + k1_at(a, ce) + k1_at(a, ce - 1))
+                              : 0.5 * (k1_at(a, c) + k1_at(a, ce));
+    const double2 A01 = *reinterpret_cast<const double2*>(a.Aez + 4u * c);      // ip=0: kr=0, 1
+    const double2 A23 = *reinterpret_cast<const double2*>(a.Aez + 4u * c + 2);  // ip=1: kr=0, 1
+    const double t0 = tr[3u * c], t0u = tr[3u * (c + up)], t0d = tr[3u * (c - dn)];
+    const double t1 = tr[3u * ce], t1u = tr[3u * (ce + up)], t1d = tr[3u * (ce - dn)];
+    double sumz = 0.0;
+    sumz = sumz + diffloc * A01.x * (t0 - t0d);  // kr=0, ip=0
+    sumz = sumz + diffloc * A23.x * (t1 - t1d);  // kr=0, ip=1
+    sumz = sumz + diffloc * A01.y * (t0u - t0);  // kr=1, ip=0
+    sumz = sumz + diffloc * A23.y * (t1u - t1);  // kr=1, ip=1
+    return sumz / (4.0 * g.dzt[k]) + (t1 - t0) / (g.cost[j] * g.dxu[i]) * a.K11[c];
+}
+
+// flux_north at (i, j, k), 0 outside i in [2, X-3], j in [1, Y-3]   (:49-77)
+__device__ __forceinline__ double flux_north(const Grid& g, const StepArgs& a, const double* __restrict__ tr,
+                                             const Strides& st, int i, int j, int k) {
+    if (i < 2 || i > g.X - 3 || j < 1 || j > g.Y - 3) return 0.0;
+    const uint32_t c = (uint32_t)i * st.si + (uint32_t)j * st.sj + (uint32_t)k;
+    const uint32_t cn = c + st.sj;
+    const bool hm = k >= 1;
+    const uint32_t up = (k + 1 < g.nz) ? 1u : 0u, dn = hm ? 1u : 0u;
+    const double diffloc = hm ? 0.25 * (k1_at(a, c) + k1_at(a, c - 1) + k1_at(a, cn) + k1_at(a, cn - 1))
+                              : 0.5 * (k1_at(a, c) + k1_at(a, cn));
+    const double2 A01 = *reinterpret_cast<const double2*>(a.Anz + 4u * c);
+    const double2 A23 = *reinterpret_cast<const double2*>(a.Anz + 4u * c + 2);
+    const double t0 = tr[3u * c], t0u = tr[3u * (c + up)], t0d = tr[3u * (c - dn)];
+    const double t1 = tr[3u * cn], t1u = tr[3u * (cn + up)], t1d = tr[3u * (cn - dn)];
+    double sumz = 0.0;
+    sumz = sumz + diffloc * A01.x * (t0 - t0d);
+    sumz = sumz + diffloc * A23.x * (t1 - t1d);
+    sumz = sumz + diffloc * A01.y * (t0u - t0);
+    sumz = sumz + diffloc * A23.y * (t1u - t1);
+    return g.cosu[j] * (sumz / (4.0 * g.dzt[k]) + (t1 - t0) / g.dyu[j] * a.K22[c]);
+}
+
+// flux_top at an interior (i, j, k), k < nz-1   (:79-111)
+__device__ __forceinline__ double flux_top(const Grid& g, const StepArgs& a, const double* __restrict__ tr,
+                                           const Strides& st, int i, int j, int k) {
+    const uint32_t c = (uint32_t)i * st.si + (uint32_t)j * st.sj + (uint32_t)k;
+    const uint32_t cw = c - st.si, ce = c + st.si, cs = c - st.sj, cn = c + st.sj;
+    const double diffloc = k2_at(a, c);
+    const double2 B01 = *reinterpret_cast<const double2*>(a.Abx + 4u * c);      // ip=0: kr=0, 1
+    const double2 B23 = *reinterpret_cast<const double2*>(a.Abx + 4u * c + 2);  // ip=1: kr=0, 1
+    const double2 C01 = *reinterpret_cast<const double2*>(a.Aby + 4u * c);
+    const double2 C23 = *reinterpret_cast<const double2*>(a.Aby + 4u * c + 2);
+    const double tc0 = tr[3u * c], tc1 = tr[3u * (c + 1)];
+    const double tw0 = tr[3u * cw], tw1 = tr[3u * (cw + 1)];
+    const double te0 = tr[3u * ce], te1 = tr[3u * (ce + 1)];
+    const double ts0 = tr[3u * cs], ts1 = tr[3u * (cs + 1)];
+    const double tn0 = tr[3u * cn], tn1 = tr[3u * (cn + 1)];
+    double sumx = 0.0;
+    sumx = sumx + diffloc * B01.x / g.cost[j] * (tc0 - tw0);  // ip=0, kr=0
+    sumx = sumx + diffloc * B01.y / g.cost[j] * (tc1 - tw1);  // ip=0, kr=1
+    sumx = sumx + diffloc * B23.x / g.cost[j] * (te0 - tc0);  // ip=1, kr=0
+    sumx = sumx + diffloc * B23.y / g.cost[j] * (te1 - tc1);  // ip=1, kr=1
+    double sumy = 0.0;
+    sumy = sumy + diffloc * C01.x * g.cosu[j - 1] * (tc0 - ts0);
+    sumy = sumy + diffloc * C01.y * g.cosu[j - 1] * (tc1 - ts1);
+    sumy = sumy + diffloc * C23.x * g.cosu[j] * (tn0 - tc0);
+    sumy = sumy + diffloc * C23.y * g.cosu[j] * (tn1 - tc1);
+    return sumx / (4 * g.dxt[i]) + sumy / (4 * g.dyt[j] * g.cost[j]);
+}
+
+// Fluxes of NTR tracers, thread per cell (i, j) in [1, X-2) x [1, Y-2):
+// flux_east for j >= 2, flux_north for i >= 2, flux_top for i, j >= 2 (0 at the
+// top level). The tracers share every read of the isoneutral tensor.
+template <int NTR>
+__global__ __launch_bounds__(256) void iso_flux_kernel(Grid g, StepArgs a) {
     const CellIdx ci = cell_of(1, g.X - 3, 1, g.Y - 3);
     if (!ci.valid) return;
     const int i = ci.i, j = ci.j, k = ci.k;
-    const int nz = g.nz;
-    const uint32_t sj = (uint32_t)nz, si = (uint32_t)g.Y * (uint32_t)nz;
-    const uint32_t c = (uint32_t)i * si + (uint32_t)j * sj + (uint32_t)k;
-    const uint32_t ce = c + si, cn = c + sj;
-    const double* __restrict__ tr = a.tr + a.tau;
-    const bool hm = k >= 1, hp = k + 1 < nz;
-    // pad_z_edges clamping: tr[min(k+1, nz-1)], tr[max(k-1, 0)]
-    const uint32_t up = hp ? 1u : 0u, dn = hm ? 1u : 0u;
-    const double trc = tr[3u * c];
-
-    if (j >= 2) {  // east face (:22-47)
-        const double diffloc = hm ? 0.25 * (k1_at(a, c) + k1_at(a, c - 1) + k1_at(a, ce) + k1_at(a, ce - 1))
-                                  : 0.5 * (k1_at(a, c) + k1_at(a, ce));
-        const double2 A01 = *reinterpret_cast<const double2*>(a.Aez + 4u * c);      // (ip=0, kr=0), (ip=0, kr=1)
-        const double2 A23 = *reinterpret_cast<const double2*>(a.Aez + 4u * c + 2);  // (ip=1, kr=0), (ip=1, kr=1)
-        const double t0 = trc, t0u = tr[3u * (c + up)], t0d = tr[3u * (c - dn)];
-        const double t1 = tr[3u * ce], t1u = tr[3u * (ce + up)], t1d = tr[3u * (ce - dn)];
-        double sumz = 0.0;
-        sumz = sumz + diffloc * A01.x * (t0 - t0d);   // kr=0, ip=0
-        sumz = sumz + diffloc * A23.x * (t1 - t1d);   // kr=0, ip=1
-        sumz = sumz + diffloc * A01.y * (t0u - t0);   // kr=1, ip=0
-        sumz = sumz + diffloc * A23.y * (t1u - t1);   // kr=1, ip=1
-        a.fe[c] = sumz / (4.0 * g.dzt[k]) + (t1 - t0) / (g.cost[j] * g.dxu[i]) * a.K11[c];
-    }
-    if (i >= 2) {  // north face (:49-77)
-        const double diffloc = hm ? 0.25 * (k1_at(a, c) + k1_at(a, c - 1) + k1_at(a, cn) + k1_at(a, cn - 1))
-                                  : 0.5 * (k1_at(a, c) + k1_at(a, cn));
-        const double2 A01 = *reinterpret_cast<const double2*>(a.Anz + 4u * c);
-        const double2 A23 = *reinterpret_cast<const double2*>(a.Anz + 4u * c + 2);
-        const double t0 = trc, t0u = tr[3u * (c + up)], t0d = tr[3u * (c - dn)];
-        const double t1 = tr[3u * cn], t1u = tr[3u * (cn + up)], t1d = tr[3u * (cn - dn)];
-        double sumz = 0.0;
-        sumz = sumz + diffloc * A01.x * (t0 - t0d);
-        sumz = sumz + diffloc * A23.x * (t1 - t1d);
-        sumz = sumz + diffloc * A01.y * (t0u - t0);
-        sumz = sumz + diffloc * A23.y * (t1u - t1);
-        a.fn[c] = g.cosu[j] * (sumz / (4.0 * g.dzt[k]) + (t1 - t0) / g.dyu[j] * a.K22[c]);
-    }
-    if (i >= 2 && j >= 2) {  // top face (:79-111)
-        if (!hp) {
-            a.ft[c] = 0.0;
-        } else {
-            const double diffloc = k2_at(a, c);
-            const uint32_t cw = c - si, cs = c - sj;
-            const double2 B01 = *reinterpret_cast<const double2*>(a.Abx + 4u * c);      // ip=0: kr=0,1
-            const double2 B23 = *reinterpret_cast<const double2*>(a.Abx + 4u * c + 2);  // ip=1: kr=0,1
-            const double2 C01 = *reinterpret_cast<const double2*>(a.Aby + 4u * c);
-            const double2 C23 = *reinterpret_cast<const double2*>(a.Aby + 4u * c + 2);
-            const double tc0 = trc, tc1 = tr[3u * (c + 1)];
-            const double tw0 = tr[3u * cw], tw1 = tr[3u * (cw + 1)];
-            const double te0 = tr[3u * ce], te1 = tr[3u * (ce + 1)];
-            const double ts0 = tr[3u * cs], ts1 = tr[3u * (cs + 1)];
-            const double tn0 = tr[3u * cn], tn1 = tr[3u * (cn + 1)];
-            double sumx = 0.0;
-            sumx = sumx + diffloc * B01.x / g.cost[j] * (tc0 - tw0);   // ip=0, kr=0
-            sumx = sumx + diffloc * B01.y / g.cost[j] * (tc1 - tw1);   // ip=0, kr=1
-            sumx = sumx + diffloc * B23.x / g.cost[j] * (te0 - tc0);   // ip=1, kr=0
-            sumx = sumx + diffloc * B23.y / g.cost[j] * (te1 - tc1);   // ip=1, kr=1
-            double sumy = 0.0;
-            sumy = sumy + diffloc * C01.x * g.cosu[j - 1] * (tc0 - ts0);
-            sumy = sumy + diffloc * C01.y * g.cosu[j - 1] * (tc1 - ts1);
-            sumy = sumy + diffloc * C23.x * g.cosu[j] * (tn0 - tc0);
-            sumy = sumy + diffloc * C23.y * g.cosu[j] * (tn1 - tc1);
-            a.ft[c] = sumx / (4 * g.dxt[i]) + sumy / (4 * g.dyt[j] * g.cost[j]);
-        }
+    const Strides st{(uint32_t)g.Y * (uint32_t)g.nz, (uint32_t)g.nz};
+    const uint32_t c = (uint32_t)i * st.si + (uint32_t)j * st.sj + (uint32_t)k;
+#pragma unroll
+    for (int t = 0; t < NTR; ++t) {
+        const double* __restrict__ tr = a.t[t].tr + a.tau;
+        if (j >= 2) a.t[t].fe[c] = flux_east(g, a, tr, st, i, j, k);
+        if (i >= 2) a.t[t].fn[c] = flux_north(g, a, tr, st, i, j, k);
+        if (i >= 2 && j >= 2) a.t[t].ft[c] = (k < g.nz - 1) ? flux_top(g, a, tr, st, i, j, k) : 0.0;
     }
 }
 
-// ---------------------------------------------------------------------------
-// Column-tile kernel: everything of the tracer step after the fluxes.
-// Block = nz levels x TC columns (consecutive j of one i row), covering
-// (i, j) in [1, X-1) x [1, Y-1).
-//   phase 1 (thread per cell): lateral dissipation on T points
-//     (veros/core/diffusion.py:9-36), flux divergence dtr
-//     (veros/core/isoneutral/diffusion.py:116-139), rhs = tr[taup1] + dt*dtr
-//     (:196-197) and delta = dt/dzw*K_33 (:154-157) -> LDS
-//   phase 2 (lane per interior column): Thomas on the wet suffix, in LDS
-//     (:158-169); the rhs slots become the new tracer (dry slots keep rhs)
-//   phase 3 (thread per cell): tr[taup1] = new, dtr_iso += dtr + (new-rhs)/dt
-//     (:196,204), P += dissipation_on_wgrid(...) (veros/core/diffusion.py:39-62)
-//     + the vertical-mixing term (veros/core/isoneutral/diffusion.py:254-279)
-// ---------------------------------------------------------------------------
-struct ColArgs {
-    double* tr;
-    int tau, taup1;
-    const double* K33;
-    double* dtr;
-    const double* I;   // int_drhodX (X, Y, nz, 3), nullable
-    double* P;         // P_diss, nullable
-    const double *fe, *fn, *ft;
-};
-
-// Face fluxes with their valid index ranges (0 elsewhere).
-__device__ __forceinline__ double fe_at(const Grid& g, const ColArgs& a, int i, int j, uint32_t o) {
-    return (i >= 1 && i <= g.X - 3 && j >= 2 && j <= g.Y - 3) ? a.fe[o] : 0.0;
+// Workspace fluxes with their valid index ranges (0 elsewhere).
+__device__ __forceinline__ double fe_at(const Grid& g, const TracerArgs& t, int i, int j, uint32_t o) {
+    return (i >= 1 && i <= g.X - 3 && j >= 2 && j <= g.Y - 3) ? t.fe[o] : 0.0;
 }
-__device__ __forceinline__ double fn_at(const Grid& g, const ColArgs& a, int i, int j, uint32_t o) {
-    return (i >= 2 && i <= g.X - 3 && j >= 1 && j <= g.Y - 3) ? a.fn[o] : 0.0;
+__device__ __forceinline__ double fn_at(const Grid& g, const TracerArgs& t, int i, int j, uint32_t o) {
+    return (i >= 2 && i <= g.X - 3 && j >= 1 && j <= g.Y - 3) ? t.fn[o] : 0.0;
 }
 
-template <int TC>
-__global__ __launch_bounds__(512) void iso_column_tile_kernel(Grid g, Prm p, ColArgs a, int implicit) {
+template <int TC, int NTR, bool IMPLICIT>
+__global__ __launch_bounds__(256) void iso_step_tile_kernel(Grid g, Prm p, StepArgs a) {
     extern __shared__ double lds[];
     const int nz = g.nz;
-    double* sD = lds;                 // [TC][nz] lateral dissipation on T points
-    double* sR = lds + TC * nz;       // [TC][nz] rhs -> forward sweep -> new tracer
-    double* sA = lds + 2 * TC * nz;   // [TC][nz] sub-diagonal
-    double* sB = lds + 3 * TC * nz;   // [TC][nz] diagonal
-    double* sC = lds + 4 * TC * nz;   // [TC][nz] super-diagonal -> cp
+    const int tcnz = TC * nz;
+    // LDS: rows A, B, C (shared by the tracers), then per tracer F (flux_top),
+    // D (lateral dissipation), R (rhs -> dp -> new), Q (cp)
+    double* sA = lds;
+    double* sB = lds + tcnz;
+    double* sC = lds + 2 * tcnz;
+    double* sT = lds + 3 * tcnz;  // per tracer blocks of 4 * tcnz
 
     const CellIdx ci = cell_of(1, g.X - 2, 1, g.Y - 2);
     const int i = ci.i, j = ci.j, k = ci.k;
     const int tc = (int)threadIdx.y;
+    const int lo = tc * nz + k;
     const bool valid = ci.valid;
     const bool interior = valid && i >= 2 && i <= g.X - 3 && j >= 2 && j <= g.Y - 3;
-    const uint32_t sj = (uint32_t)nz, si = (uint32_t)g.Y * (uint32_t)nz;
-    const uint32_t c = (uint32_t)i * si + (uint32_t)j * sj + (uint32_t)k;
+    const Strides st{(uint32_t)g.Y * (uint32_t)nz, (uint32_t)nz};
+    const uint32_t c = (uint32_t)i * st.si + (uint32_t)j * st.sj + (uint32_t)k;
     const int kb = valid ? g.kbot[i * g.Y + j] : 0;
     const bool energy = a.P != nullptr;
     const double dt = p.dt_tracer;
+    const double mt = m01(kb, k);
 
-    // ---- phase 1
-    double dtrk = 0.0, rhs = 0.0, ftk = 0.0, K33k = 0.0, Ic = 0.0;
-    if (valid && energy) {
-        const double* __restrict__ I = a.I + a.tau;
-        Ic = I[3u * c];
-        const double de = (I[3u * (c + si)] - Ic) * fe_at(g, a, i, j, c) +
-                          (Ic - I[3u * (c - si)]) * fe_at(g, a, i - 1, j, c - si);
-        const double dn = (I[3u * (c + sj)] - Ic) * fn_at(g, a, i, j, c) +
-                          (Ic - I[3u * (c - sj)]) * fn_at(g, a, i, j - 1, c - sj);
-        const double g05 = 0.5 * p.grav / p.rho_0;
-        sD[tc * nz + k] = g05 * de / (g.dxt[i] * g.cost[j]) + g05 * dn / (g.dyt[j] * g.cost[j]);
-    }
-    if (interior) {
-        const double mt = m01(kb, k);
-        ftk = a.ft[c];
-        const double hor = (a.fe[c] - a.fe[c - si]) / (g.cost[j] * g.dxt[i]) +
-                           (a.fn[c] - a.fn[c - sj]) / (g.cost[j] * g.dyt[j]);
-        const double vert = (k == 0) ? ftk / g.dzt[0] : (ftk - a.ft[c - 1]) / g.dzt[k];
-        dtrk = mt * hor + mt * vert;
-        rhs = a.tr[3u * c + a.taup1] + dt * dtrk;
-        sR[tc * nz + k] = rhs;
-        if (implicit) {
-            // rows of the K33 system (veros/core/isoneutral/diffusion.py:154-164); the
-            // edge row k = kbot-1 takes b_edge and a = 0 (veros/core/operators.py:72)
-            K33k = a.K33[c];
-            const double delta = (k < nz - 1) ? dt / g.dzw[k] * K33k : 0.0;
-            const double delta_m = (k >= 1) ? dt / g.dzw[k - 1] * a.K33[c - 1] : 0.0;
-            const double dz = g.dzt[k];
-            const bool edge = k == kb - 1;
-            sA[tc * nz + k] = edge ? 0.0 : -delta_m / dz;
-            sB[tc * nz + k] = edge ? 1 + delta / dz : 1 + (delta + delta_m) / dz;
-            sC[tc * nz + k] = (k < nz - 1) ? -delta / dz : 0.0;
+    // ---- phase 1a: fluxes from the workspace, lateral dissipation, K33 rows
+    double fe_c[NTR], fe_w[NTR], fn_c[NTR], fn_s[NTR], ft_c[NTR], Ic[NTR];
+#pragma unroll
+    for (int t = 0; t < NTR; ++t) {
+        const TracerArgs& ta = a.t[t];
+        fe_c[t] = fe_w[t] = fn_c[t] = fn_s[t] = ft_c[t] = Ic[t] = 0.0;
+        if (valid) {
+            fe_c[t] = fe_at(g, ta, i, j, c);
+            fe_w[t] = fe_at(g, ta, i - 1, j, c - st.si);
+            fn_c[t] = fn_at(g, ta, i, j, c);
+            fn_s[t] = fn_at(g, ta, i, j - 1, c - st.sj);
         }
+        if (interior) ft_c[t] = ta.ft[c];
+        double* sTt = sT + t * 4 * tcnz;
+        sTt[lo] = ft_c[t];
+        if (valid && energy) {
+            // compute_dissipation (veros/core/diffusion.py:13-33)
+            const double* __restrict__ I = ta.I + a.tau;
+            Ic[t] = I[3u * c];
+            const double de = (I[3u * (c + st.si)] - Ic[t]) * fe_c[t] + (Ic[t] - I[3u * (c - st.si)]) * fe_w[t];
+            const double dn = (I[3u * (c + st.sj)] - Ic[t]) * fn_c[t] + (Ic[t] - I[3u * (c - st.sj)]) * fn_s[t];
+            const double g05 = 0.5 * p.grav / p.rho_0;
+            sTt[tcnz + lo] = g05 * de / (g.dxt[i] * g.cost[j]) + g05 * dn / (g.dyt[j] * g.cost[j]);
+        }
+    }
+    double K33k = 0.0;
+    if (IMPLICIT && interior) {
+        // rows of the K33 system (veros/core/isoneutral/diffusion.py:154-164); the
+        // edge row k = kbot-1 takes b_edge and a = 0 (veros/core/operators.py:72)
+        K33k = a.K33[c];
+        const double delta = (k < nz - 1) ? dt / g.dzw[k] * K33k : 0.0;
+        const double delta_m = (k >= 1) ? dt / g.dzw[k - 1] * a.K33[c - 1] : 0.0;
+        const double dz = g.dzt[k];
+        const bool edge = k == kb - 1;
+        sA[lo] = edge ? 0.0 : -delta_m / dz;
+        sB[lo] = edge ? 1 + delta / dz : 1 + (delta + delta_m) / dz;
+        sC[lo] = (k < nz - 1) ? -delta / dz : 0.0;
     }
     __syncthreads();
 
-    // ---- phase 2: threads 0..TC-1 (first wave) sweep one column each
-    if (implicit) {
+    // ---- phase 1b: divergence and rhs (veros/core/isoneutral/diffusion.py:116-139,196-197)
+    double dtrk[NTR], rhs[NTR];
+#pragma unroll
+    for (int t = 0; t < NTR; ++t) {
+        dtrk[t] = rhs[t] = 0.0;
+        if (interior) {
+            double* sTt = sT + t * 4 * tcnz;
+            const double ftm = (k >= 1) ? sTt[lo - 1] : 0.0;
+            const double hor = (fe_c[t] - fe_w[t]) / (g.cost[j] * g.dxt[i]) +
+                               (fn_c[t] - fn_s[t]) / (g.cost[j] * g.dyt[j]);
+            const double vert = (k == 0) ? ft_c[t] / g.dzt[0] : (ft_c[t] - ftm) / g.dzt[k];
+            dtrk[t] = mt * hor + mt * vert;
+            rhs[t] = a.t[t].tr[3u * c + a.taup1] + dt * dtrk[t];
+            sTt[2 * tcnz + lo] = rhs[t];
+        }
+    }
+
+    // ---- phase 2: one lane per (interior column, tracer) sweeps in LDS
+    if constexpr (IMPLICIT) {
+        __syncthreads();
         const int lin = (int)(threadIdx.y * blockDim.x + threadIdx.x);
-        if (lin < TC) {
-            const int cj = j - tc + lin;  // j - tc is the tile's first column
+        if (lin < TC * NTR) {
+            const int col = lin % TC, t = lin / TC;
+            const int cj = j - tc + col;  // j - tc is the tile's first column
             const bool col_ok = i >= 2 && i <= g.X - 3 && cj >= 2 && cj <= g.Y - 3;
             const int kbl = col_ok ? g.kbot[i * g.Y + cj] : 0;
             if (col_ok && kbl >= 1 && kbl <= nz) {
                 const int k0 = kbl - 1;
-                double* R = sR + lin * nz;
-                const double* Ar = sA + lin * nz;
-                const double* Br = sB + lin * nz;
-                double* Cc = sC + lin * nz;
+                const double* Ar = sA + col * nz;
+                const double* Br = sB + col * nz;
+                const double* Cr = sC + col * nz;
+                double* R = sT + t * 4 * tcnz + 2 * tcnz + col * nz;
+                double* Q = sT + t * 4 * tcnz + 3 * tcnz + col * nz;
                 double cpp = 0.0, dpp = 0.0;
 #pragma unroll 8
                 for (int kk = k0; kk < nz; ++kk) {
                     const double ak = Ar[kk];
                     const double r = rcp_nr(Br[kk] - ak * cpp);
-                    cpp = Cc[kk] * r;
+                    cpp = Cr[kk] * r;
                     dpp = (R[kk] - ak * dpp) * r;
-                    Cc[kk] = cpp;
+                    Q[kk] = cpp;
                     R[kk] = dpp;
                 }
                 double xn = 0.0;
 #pragma unroll 8
                 for (int kk = nz - 1; kk >= k0; --kk) {
-                    xn = R[kk] - Cc[kk] * xn;
+                    xn = R[kk] - Q[kk] * xn;
                     R[kk] = xn;
                 }
             }
         }
-        __syncthreads();
     }
+    __syncthreads();
 
     // ---- phase 3
     if (!valid) return;
-    if (interior) {
-        const double newk = sR[tc * nz + k];
-        a.tr[3u * c + a.taup1] = newk;
-        a.dtr[c] = (a.dtr[c] + dtrk) + (newk - rhs) / dt;
-    }
-    if (energy) {
-        const double d0 = sD[tc * nz + k];
-        const bool land = kb >= 1;  // create_water_masks(kbot): ks = kbot - 1 >= 0
-        double dw;
-        if (k == nz - 1) {
-            dw = d0 * (land ? 1.0 : 0.0);
-        } else {
-            const double d1 = sD[tc * nz + k + 1];
-            const double avg = 0.5 * (d0 + d1);
-            const bool edge = land && k == kb - 1;
-            const bool water = land && k > kb - 1;
-            const double dzw_lo = g.dzw[k >= 1 ? k - 1 : 0];  // pad_z_edges(dzw)[k]
-            dw = (avg + 0.5 * (d0 * dzw_lo / g.dzw[k])) * (edge ? 1.0 : 0.0) + avg * (water ? 1.0 : 0.0);
+    double Pk = energy ? a.P[c] : 0.0;
+#pragma unroll
+    for (int t = 0; t < NTR; ++t) {
+        const double* sTt = sT + t * 4 * tcnz;
+        if (interior) {
+            const double newk = sTt[2 * tcnz + lo];
+            a.t[t].tr[3u * c + a.taup1] = newk;
+            a.t[t].dtr[c] = (a.t[t].dtr[c] + dtrk[t]) + (newk - rhs[t]) / dt;
         }
-        double Pk = a.P[c] + dw;
-        if (interior && k < nz - 1) {
-            const double mw = m01(kb, k) * m01(kb, k + 1);
-            const double fxa = (-a.I[3u * (c + 1) + a.tau] + Ic) / g.dzw[k];
-            const double gr = -p.grav / p.rho_0;
-            if (implicit) {
-                const double dtrz = sR[tc * nz + k + 1] - sR[tc * nz + k];
-                Pk = Pk + gr * fxa * (ftk * mw + K33k * dtrz / g.dzw[k] * mw);
+        if (energy) {
+            // dissipation_on_wgrid (veros/core/diffusion.py:39-62)
+            const double d0 = sTt[tcnz + lo];
+            const bool land = kb >= 1;  // create_water_masks(kbot): ks = kbot - 1 >= 0
+            double dw;
+            if (k == nz - 1) {
+                dw = d0 * (land ? 1.0 : 0.0);
             } else {
-                Pk = Pk + gr * fxa * ftk * mw;
+                const double d1 = sTt[tcnz + lo + 1];
+                const double avg = 0.5 * (d0 + d1);
+                const bool edge = land && k == kb - 1;
+                const bool water = land && k > kb - 1;
+                const double dzw_lo = g.dzw[k >= 1 ? k - 1 : 0];  // pad_z_edges(dzw)[k]
+                dw = (avg + 0.5 * (d0 * dzw_lo / g.dzw[k])) * (edge ? 1.0 : 0.0) + avg * (water ? 1.0 : 0.0);
+            }
+            Pk = Pk + dw;
+            // vertical mixing term (veros/core/isoneutral/diffusion.py:254-279)
+            if (interior && k < nz - 1) {
+                const double mw = m01(kb, k) * m01(kb, k + 1);
+                const double fxa = (-a.t[t].I[3u * (c + 1) + a.tau] + Ic[t]) / g.dzw[k];
+                const double gr = -p.grav / p.rho_0;
+                if constexpr (IMPLICIT) {
+                    const double dtrz = sTt[2 * tcnz + lo + 1] - sTt[2 * tcnz + lo];
+                    Pk = Pk + gr * fxa * (ft_c[t] * mw + K33k * dtrz / g.dzw[k] * mw);
+                } else {
+                    Pk = Pk + gr * fxa * ft_c[t] * mw;
+                }
             }
         }
-        a.P[c] = Pk;
     }
+    if (energy) a.P[c] = Pk;
 }
 
 // ---------------------------------------------------------------------------
@@ -702,53 +759,53 @@ extern "C" int vx_iso_pre_f64(void* stream, const vx_grid* grid, const vx_iso_pa
     return check_launch("vx_iso_pre_f64 (K_33 top)");
 }
 
-extern "C" int64_t vx_iso_workspace_bytes(const vx_grid* grid) {
-    if (!grid) return -1;
-    return 3 * (grid->nx + 4) * (grid->ny + 4) * (int64_t)grid->nz * (int64_t)sizeof(double);
-}
 
 namespace vx {
 namespace {
 
-constexpr int kTileColumns = 4;
+constexpr int kTileColumns = VX_TILE_COLUMNS;
 
-int launch_tile(const Grid& g, const Prm& p, const ColArgs& a, int implicit, hipStream_t s) {
+template <int NTR, bool IMPLICIT>
+int launch_step(const Grid& g, const Prm& p, const StepArgs& a, hipStream_t s, const char* who) {
     constexpr int TC = kTileColumns;
-    if (g.nz * TC > 512) return fail(VX_ERR_UNSUPPORTED, "iso column tile: nz=%d too deep", g.nz);
+    if ((int64_t)g.X * g.Y * g.nz * 4 >= ((int64_t)1 << 32))
+        return fail(VX_ERR_UNSUPPORTED, "%s: grid too large for 32-bit offsets", who);
+    if (g.nz * TC > 256) return fail(VX_ERR_UNSUPPORTED, "%s: nz=%d too deep", who, g.nz);
+    const dim3 fblk = cell_block(g.nz);
+    hipLaunchKernelGGL((iso_flux_kernel<NTR>), cell_grid(g.X - 3, g.Y - 3, fblk), fblk, 0, s, g, a);
+    int rc = check_launch(who);
+    if (rc) return rc;
     const dim3 blk(g.nz, TC);
-    const size_t lds = (size_t)5 * TC * g.nz * sizeof(double);
-    hipLaunchKernelGGL(iso_column_tile_kernel<TC>, cell_grid(g.X - 2, g.Y - 2, blk), blk, lds, s, g, p, a, implicit);
-    return check_launch("iso_column_tile_kernel");
+    const size_t lds = (size_t)(3 + 4 * NTR) * TC * g.nz * sizeof(double);
+    hipLaunchKernelGGL((iso_step_tile_kernel<TC, NTR, IMPLICIT>), cell_grid(g.X - 2, g.Y - 2, blk), blk, lds, s, g,
+                       p, a);
+    return check_launch(who);
 }
 
-int iso_diffusion_common(void* stream, const vx_grid* grid, const vx_iso_params* prm, double* tracer, int32_t tau,
-                         int32_t taup1, const double* Kiso, const double* Kskew, const double* Ai_ez, const double* Ai_nz,
-                         const double* Ai_bx, const double* Ai_by, const double* K_11, const double* K_22,
-                         const double* K_33, double* dtr_iso, const double* int_drhodX, double* P_diss,
-                         void* workspace, int implicit, const char* who) {
-    Grid g;
+int check_common(const vx_grid* grid, const vx_iso_params* prm, int32_t tau, int32_t taup1, Grid& g,
+                 const char* who) {
     int rc = make_grid(grid, g, who);
     if (rc) return rc;
     if (!prm) return fail(VX_ERR_INVALID_ARGUMENT, "%s: null params", who);
     if (tau < 0 || tau > 2 || taup1 < 0 || taup1 > 2)
         return fail(VX_ERR_INVALID_ARGUMENT, "%s: time levels must be 0..2", who);
-    if (!tracer || (!Kiso && !Kskew) || !Ai_ez || !Ai_nz || !Ai_bx || !Ai_by || !K_11 || !K_22 || !dtr_iso || !workspace ||
-        (implicit && !K_33))
-        return fail(VX_ERR_INVALID_ARGUMENT, "%s: null array pointer", who);
-    if ((P_diss == nullptr) != (int_drhodX == nullptr))
-        return fail(VX_ERR_INVALID_ARGUMENT, "%s: P_diss and int_drhodX must both be given or both be NULL", who);
-    hipStream_t s = (hipStream_t)stream;
-    const Prm p = make_prm(prm);
-    const int64_t n3 = (int64_t)g.X * g.Y * g.nz;
-    double* ws = static_cast<double*>(workspace);
-    FluxArgs fa{tracer, tau, Kiso, Kskew, Ai_ez, Ai_nz, Ai_bx, Ai_by, K_11, K_22, ws, ws + n3, ws + 2 * n3};
-    if (n3 * 4 >= ((int64_t)1 << 32)) return fail(VX_ERR_UNSUPPORTED, "%s: grid too large for 32-bit offsets", who);
-    const dim3 blk = cell_block(g.nz);
-    hipLaunchKernelGGL(iso_flux_kernel, cell_grid(g.X - 3, g.Y - 3, blk), blk, 0, s, g, fa);
-    if ((rc = check_launch(who))) return rc;
-    ColArgs ca{tracer, tau, taup1, K_33, dtr_iso, int_drhodX, P_diss, ws, ws + n3, ws + 2 * n3};
-    if ((rc = launch_tile(g, p, ca, implicit, s))) return rc;
     return VX_OK;
+}
+
+bool tensor_ok(const double* Ai_ez, const double* Ai_nz, const double* Ai_bx, const double* Ai_by,
+               const double* K_11, const double* K_22) {
+    return Ai_ez && Ai_nz && Ai_bx && Ai_by && K_11 && K_22;
+}
+
+// three flux arrays per tracer
+void wire_workspace(const Grid& g, StepArgs& a, int ntr, void* ws) {
+    const int64_t n3 = (int64_t)g.X * g.Y * g.nz;
+    double* w = static_cast<double*>(ws);
+    for (int t = 0; t < ntr; ++t) {
+        a.t[t].fe = w + (3 * t + 0) * n3;
+        a.t[t].fn = w + (3 * t + 1) * n3;
+        a.t[t].ft = w + (3 * t + 2) * n3;
+    }
 }
 }  // namespace
 }  // namespace vx
@@ -758,10 +815,60 @@ extern "C" int vx_iso_diffusion_f64(void* stream, const vx_grid* grid, const vx_
                                     const double* Ai_nz, const double* Ai_bx, const double* Ai_by,
                                     const double* K_11, const double* K_22, const double* K_33, double* dtr_iso,
                                     const double* int_drhodX, double* P_diss, void* workspace) {
-    if (!K_iso) return vx::fail(VX_ERR_INVALID_ARGUMENT, "vx_iso_diffusion_f64: null K_iso");
-    return vx::iso_diffusion_common(stream, grid, prm, tracer, tau, taup1, K_iso, nullptr, Ai_ez, Ai_nz, Ai_bx, Ai_by,
-                                    K_11, K_22, K_33, dtr_iso, int_drhodX, P_diss, workspace, 1,
-                                    "vx_iso_diffusion_f64");
+    using namespace vx;
+    const char* who = "vx_iso_diffusion_f64";
+    Grid g;
+    int rc = check_common(grid, prm, tau, taup1, g, who);
+    if (rc) return rc;
+    if (!tracer || !K_iso || !tensor_ok(Ai_ez, Ai_nz, Ai_bx, Ai_by, K_11, K_22) || !K_33 || !dtr_iso)
+        return fail(VX_ERR_INVALID_ARGUMENT, "%s: null array pointer", who);
+    if ((P_diss == nullptr) != (int_drhodX == nullptr))
+        return fail(VX_ERR_INVALID_ARGUMENT, "%s: P_diss and int_drhodX must both be given or both be NULL", who);
+    StepArgs a{};
+    if (!workspace) return fail(VX_ERR_INVALID_ARGUMENT, "%s: null workspace", who);
+    a.t[0] = {tracer, dtr_iso, int_drhodX};
+    wire_workspace(g, a, 1, workspace);
+    a.tau = tau;
+    a.taup1 = taup1;
+    a.Kiso = K_iso;
+    a.Kskew = nullptr;
+    a.Aez = Ai_ez, a.Anz = Ai_nz, a.Abx = Ai_bx, a.Aby = Ai_by, a.K11 = K_11, a.K22 = K_22, a.K33 = K_33;
+    a.P = P_diss;
+    return launch_step<1, true>(g, make_prm(prm), a, (hipStream_t)stream, who);
+}
+
+extern "C" int vx_iso_diffusion_ts_f64(void* stream, const vx_grid* grid, const vx_iso_params* prm, double* temp,
+                                       double* salt, int32_t tau, int32_t taup1, const double* K_iso,
+                                       const double* Ai_ez, const double* Ai_nz, const double* Ai_bx,
+                                       const double* Ai_by, const double* K_11, const double* K_22,
+                                       const double* K_33, double* dtemp_iso, double* dsalt_iso,
+                                       const double* int_drhodT, const double* int_drhodS, double* P_diss,
+                                       void* workspace) {
+    using namespace vx;
+    const char* who = "vx_iso_diffusion_ts_f64";
+    Grid g;
+    int rc = check_common(grid, prm, tau, taup1, g, who);
+    if (rc) return rc;
+    if (!temp || !salt || !K_iso || !tensor_ok(Ai_ez, Ai_nz, Ai_bx, Ai_by, K_11, K_22) || !K_33 || !dtemp_iso ||
+        !dsalt_iso)
+        return fail(VX_ERR_INVALID_ARGUMENT, "%s: null array pointer", who);
+    if (temp == salt) return fail(VX_ERR_INVALID_ARGUMENT, "%s: temp and salt must be distinct arrays", who);
+    const bool e = P_diss != nullptr;
+    if (e != (int_drhodT != nullptr) || e != (int_drhodS != nullptr))
+        return fail(VX_ERR_INVALID_ARGUMENT, "%s: P_diss, int_drhodT and int_drhodS must all be given or all be NULL",
+                    who);
+    StepArgs a{};
+    if (!workspace) return fail(VX_ERR_INVALID_ARGUMENT, "%s: null workspace", who);
+    a.t[0] = {temp, dtemp_iso, int_drhodT};
+    a.t[1] = {salt, dsalt_iso, int_drhodS};
+    wire_workspace(g, a, 2, workspace);
+    a.tau = tau;
+    a.taup1 = taup1;
+    a.Kiso = K_iso;
+    a.Kskew = nullptr;
+    a.Aez = Ai_ez, a.Anz = Ai_nz, a.Abx = Ai_bx, a.Aby = Ai_by, a.K11 = K_11, a.K22 = K_22, a.K33 = K_33;
+    a.P = P_diss;
+    return launch_step<2, true>(g, make_prm(prm), a, (hipStream_t)stream, who);
 }
 
 extern "C" int vx_iso_skew_diffusion_f64(void* stream, const vx_grid* grid, const vx_iso_params* prm,
@@ -770,8 +877,29 @@ extern "C" int vx_iso_skew_diffusion_f64(void* stream, const vx_grid* grid, cons
                                          const double* Ai_by, const double* K_11, const double* K_22,
                                          double* dtr_iso, const double* int_drhodX, double* P_diss_skew,
                                          void* workspace) {
-    if (!K_gm) return vx::fail(VX_ERR_INVALID_ARGUMENT, "vx_iso_skew_diffusion_f64: null K_gm");
-    return vx::iso_diffusion_common(stream, grid, prm, tracer, tau, taup1, nullptr, K_gm, Ai_ez, Ai_nz, Ai_bx,
-                                    Ai_by, K_11, K_22, nullptr, dtr_iso, int_drhodX, P_diss_skew, workspace, 0,
-                                    "vx_iso_skew_diffusion_f64");
+    using namespace vx;
+    const char* who = "vx_iso_skew_diffusion_f64";
+    Grid g;
+    int rc = check_common(grid, prm, tau, taup1, g, who);
+    if (rc) return rc;
+    if (!tracer || !K_gm || !tensor_ok(Ai_ez, Ai_nz, Ai_bx, Ai_by, K_11, K_22) || !dtr_iso)
+        return fail(VX_ERR_INVALID_ARGUMENT, "%s: null array pointer", who);
+    if ((P_diss_skew == nullptr) != (int_drhodX == nullptr))
+        return fail(VX_ERR_INVALID_ARGUMENT, "%s: P_diss_skew and int_drhodX must both be given or both be NULL", who);
+    StepArgs a{};
+    if (!workspace) return fail(VX_ERR_INVALID_ARGUMENT, "%s: null workspace", who);
+    a.t[0] = {tracer, dtr_iso, int_drhodX};
+    wire_workspace(g, a, 1, workspace);
+    a.tau = tau;
+    a.taup1 = taup1;
+    a.Kiso = nullptr;
+    a.Kskew = K_gm;
+    a.Aez = Ai_ez, a.Anz = Ai_nz, a.Abx = Ai_bx, a.Aby = Ai_by, a.K11 = K_11, a.K22 = K_22, a.K33 = nullptr;
+    a.P = P_diss_skew;
+    return launch_step<1, false>(g, make_prm(prm), a, (hipStream_t)stream, who);
+}
+
+extern "C" int64_t vx_iso_workspace_bytes(const vx_grid* grid, int32_t ntracers) {
+    if (!grid || ntracers < 1 || ntracers > 2) return -1;
+    return 3 * (int64_t)ntracers * (grid->nx + 4) * (grid->ny + 4) * (int64_t)grid->nz * (int64_t)sizeof(double);
 }

@@ -143,8 +143,7 @@ def isoneutral_diffusion_kernel(state, tr, istemp, iso=True):
     else:
         I = P = None
     L = _native.lib()
-    nbytes = L.vx_iso_workspace_bytes(g)
-    ws = workspace(nbytes, tr.device)
+    ws = workspace(L.vx_iso_workspace_bytes(g, 1), tr.device)
     fn = L.vx_iso_diffusion_f64 if iso else L.vx_iso_skew_diffusion_f64
     rc = fn(stream_handle(), g, p, ptr(tr), int(vs.tau), int(vs.taup1),
             *(ptr(a) for a in arrays), ptr(dtr), ptr(I), ptr(P), ptr(ws))
@@ -164,6 +163,36 @@ def isoneutral_diffusion(state, tr, istemp):
     """The routine: run the kernel and store its outputs in state.variables
     (veros/core/isoneutral/diffusion.py:286-295)."""
     _store(state, isoneutral_diffusion_kernel(state, tr, istemp, iso=True))
+
+
+def isoneutral_diffusion_temp_salt(state):
+    """``isoneutral_diffusion(state, temp, True)`` then
+    ``isoneutral_diffusion(state, salt, False)`` (the pair thermodynamics runs,
+    veros/core/thermodynamics.py:431-432) in one pass over the isoneutral
+    tensor; same results as the two calls."""
+    vs, st = state.variables, state.settings
+    X, Y, nz = vs.K_iso.shape
+    keep = []
+    g = grid_struct(vs, keep)
+    p = params_struct(st)
+    T = _check_array("temp", vs.temp, (X, Y, nz, 3))
+    S = _check_array("salt", vs.salt, (X, Y, nz, 3))
+    arrays = [_check_array("K_iso", vs.K_iso, (X, Y, nz))]
+    arrays += [_check_array(n, getattr(vs, n), (X, Y, nz, 2, 2)) for n in ("Ai_ez", "Ai_nz", "Ai_bx", "Ai_by")]
+    arrays += [_check_array(n, getattr(vs, n), (X, Y, nz)) for n in ("K_11", "K_22", "K_33")]
+    dT = _check_array("dtemp_iso", vs.dtemp_iso, (X, Y, nz))
+    dS = _check_array("dsalt_iso", vs.dsalt_iso, (X, Y, nz))
+    if st.enable_conserve_energy:
+        IT = _check_array("int_drhodT", vs.int_drhodT, (X, Y, nz, 3))
+        IS = _check_array("int_drhodS", vs.int_drhodS, (X, Y, nz, 3))
+        P = _check_array("P_diss_iso", vs.P_diss_iso, (X, Y, nz))
+    else:
+        IT = IS = P = None
+    L = _native.lib()
+    ws = workspace(L.vx_iso_workspace_bytes(g, 2), T.device)
+    rc = L.vx_iso_diffusion_ts_f64(stream_handle(), g, p, ptr(T), ptr(S), int(vs.tau), int(vs.taup1),
+                                   *(ptr(a) for a in arrays), ptr(dT), ptr(dS), ptr(IT), ptr(IS), ptr(P), ptr(ws))
+    _native.check(rc, "isoneutral_diffusion (temp, salt)")
 
 
 def isoneutral_skew_diffusion(state, tr, istemp):
