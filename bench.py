@@ -23,6 +23,11 @@ REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of iso_pre_kernel on 1440x720x80, from the FETCH_SIZE and
+# WRITE_SIZE rocprofv3 passes in profiles/mixing_step_1440x720x80.md (read
+# bytes = 2.0 x FETCH_SIZE, write bytes = 1.0 x WRITE_SIZE, calibrated on a
+# 1 GiB device copy)
+PRE_TRAFFIC_BYTES = {(1440, 720, 80): (14.14 + 13.21) * 1e9}
 
 
 def parse():
@@ -37,6 +42,8 @@ def parse():
     ap.add_argument("--refine", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 1 GiB device-to-device copy (a known-traffic dispatch for PMC scaling)")
     return ap.parse_args()
 
 
@@ -169,6 +176,13 @@ def main():
     print(f"[bench] building {nx}x{ny}x{nz} workload", file=sys.stderr, flush=True)
     state, td = build_workload(nx, ny, nz)
     torch.cuda.synchronize()
+    if args.calibrate_copy:
+        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     vs = state.variables
     stream = torch.cuda.current_stream()
 
@@ -242,11 +256,13 @@ def main():
         "iso_diffusion_T_then_S": (B_diff, parts["iso_diffusion_T_then_S"][0]),
     }
     gbs = {k: b / (t * 1e-3) / 1e9 for k, (b, t) in kern.items()}
-    dom = max(("iso_pre", "iso_diffusion_TS", f"tdma_{prec}"), key=lambda k: kern[k][1])
-    roof = {"bound": "hbm", "kernel": dom, "achieved": round(gbs[dom], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(gbs[dom] / HBM_PEAK_GBS, 4), "traffic": None,
-            "algorithmic_bytes_per_launch": kern[dom][0], "avg_launch_ms": kern[dom][1]}
-
+    # roofline: iso_pre_kernel, the largest single-kernel entry point (its call
+    # also launches a 40-us kernel zeroing K_33's top level)
+    dom = "iso_pre"
+    roof = {"bound": "hbm", "kernel": "iso_pre_kernel", "achieved": round(gbs[dom], 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(gbs[dom] / HBM_PEAK_GBS, 4),
+            "traffic": PRE_TRAFFIC_BYTES.get((nx, ny, nz)),
+            "algorithmic_bytes_per_launch": kern[dom][0], "avg_launch_ms": round(kern[dom][1], 4)}
     out = {
         "metric": "TDMA Mcolumns/s + isoneutral Mcells/s; mixed vs FP64 GPU at <=1e-12 rel err",
         "value": round(N / (ms * 1e-3) / 1e6, 2),

@@ -159,9 +159,16 @@ __device__ __forceinline__ double rden(double drodz) { return rcp_nr(fmin(0.0, d
 // ---------------------------------------------------------------------------
 // Cell-parallel launch geometry: block = (nz levels) x (cpb columns along j);
 // consecutive lanes are consecutive levels of one column, so every read of a
-// neighbouring column is a contiguous run. The 1-D grid holds ni x nbx
-// blocks (rounded up to a multiple of 8) and is remapped XCD-contiguously.
+// neighbouring column is a contiguous run. Blocks are ordered in strips of
+// kStripBlocks blocks along j, walked row by row along i, and the 1-D grid is
+// remapped XCD-contiguously: the i-1 / i / i+1 rows of a strip are then in
+// flight on the same XCD together and their shared reads hit its L2.
 // ---------------------------------------------------------------------------
+#ifndef VX_STRIP_BLOCKS
+#define VX_STRIP_BLOCKS 16
+#endif
+constexpr int kStripBlocks = VX_STRIP_BLOCKS;
+
 struct CellIdx {
     int i, j, k;
     bool valid;
@@ -169,15 +176,19 @@ struct CellIdx {
 
 __device__ __forceinline__ CellIdx cell_of(int i0, int ni, int j0, int nj) {
     const int cpb = blockDim.y;
-    const int nbx = (nj + cpb - 1) / cpb;
+    const int sw = kStripBlocks * cpb;                // strip width in columns
+    const int nstrips = (nj + sw - 1) / sw;
+    const int per_strip = ni * kStripBlocks;
     const int L = (int)xcd_block(gridDim.x);
+    const int sidx = L / per_strip;
+    const int rem = L - sidx * per_strip;
+    const int ii = rem / kStripBlocks;
+    const int b = rem - ii * kStripBlocks;
     CellIdx r;
-    const int ii = L / nbx;
-    const int bx = L - ii * nbx;
     r.i = i0 + ii;
-    r.j = j0 + bx * cpb + (int)threadIdx.y;
+    r.j = j0 + sidx * sw + b * cpb + (int)threadIdx.y;
     r.k = (int)threadIdx.x;
-    r.valid = ii < ni && r.j < j0 + nj;
+    r.valid = sidx < nstrips && r.j < j0 + nj;
     return r;
 }
 
@@ -728,8 +739,9 @@ unsigned blocks_xcd(int64_t n, int bs) { return (unsigned)(((n + bs - 1) / bs + 
 // groups, rounded up to a multiple of 8 for the XCD remap
 dim3 cell_block(int nz) { return dim3(nz, nz >= 256 ? 1 : 256 / nz); }
 dim3 cell_grid(int ni, int nj, dim3 blk) {
-    const int64_t nbx = (nj + blk.y - 1) / blk.y;
-    return dim3((unsigned)(((int64_t)ni * nbx + 7) / 8 * 8));
+    const int64_t sw = (int64_t)kStripBlocks * blk.y;
+    const int64_t nstrips = (nj + sw - 1) / sw;
+    return dim3((unsigned)((nstrips * ni * kStripBlocks + 7) / 8 * 8));
 }
 
 }  // namespace
