@@ -42,6 +42,9 @@ def parse():
     ap.add_argument("--refine", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
+    ap.add_argument("--batched", action="store_true",
+                    help="instead of the step: 64 independent uneven grids, sequential launches vs one batched "
+                         "plan (untuned, autotuned, hipGraph), fp64 and mixed")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy (a known-traffic dispatch for PMC scaling)")
     return ap.parse_args()
@@ -160,8 +163,63 @@ def cpu_baseline(nx, ny, nz, budget_s):
                           cpu=platform.processor() or platform.machine(), numpy=np.__version__))
 
 
+def run_batched(args):
+    """64 grids, nx, ny ~ U{48..720}, nz ~ U{15..100}, kbot ~ U{1..nz} with
+    30% land: 64 sequential solve_implicit launches vs one batched plan."""
+    import torch
+
+    from veros_amd import utilities
+    from veros_amd.batched import TdmaBatch, random_systems
+
+    torch.cuda.set_device(0)
+    print("[bench] building 64 uneven grids", file=sys.stderr, flush=True)
+    systems = random_systems(64)
+    torch.cuda.synchronize()
+    cols = sum(s["a"].shape[0] * s["a"].shape[1] for s in systems)
+    cells = sum(s["a"].numel() for s in systems)
+    nbytes = sum(tdma_bytes(s["kbot"], s["a"].shape[2])[0] for s in systems)
+    stream = torch.cuda.current_stream()
+
+    def timed(fn, reps):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+        for s0, e0 in evs:
+            s0.record(stream)
+            fn()
+            e0.record(stream)
+        torch.cuda.synchronize()
+        ts = sorted(a.elapsed_time(b) for a, b in evs)
+        return ts[len(ts) // 2]
+
+    res = {}
+    for prec in ("fp64", "mixed"):
+        def sequential():
+            for s0 in systems:
+                utilities.solve_implicit_kbot(s0["a"], s0["b"], s0["c"], s0["d"], s0["kbot"], precision=prec,
+                                              refine_steps=1, out=s0["x"])
+        plan = TdmaBatch(systems, prec)
+        r = {"sequential_64_launches_ms": timed(sequential, args.steps),
+             "plan_untuned_ms": timed(plan.run, args.steps)}
+        info = plan.tune()
+        r["plan_tuned_ms"] = timed(plan.run, args.steps)
+        r["plan_graph_ms"] = timed(plan.graph_launch, args.steps)
+        r["tuned_choices"] = info["buckets"]
+        best = min(r["plan_tuned_ms"], r["plan_graph_ms"])
+        r["speedup_vs_sequential"] = round(r["sequential_64_launches_ms"] / best, 3)
+        r["mcolumns_per_s"] = round(cols / (best * 1e-3) / 1e6, 1)
+        r["algorithmic_GBps"] = round(nbytes / (best * 1e-3) / 1e9, 1)
+        res[prec] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+        plan.close()
+    print(json.dumps({"workload": "64 independent grids, nx,ny ~ U{48..720}, nz ~ U{15..100}, 30% land",
+                      "columns": cols, "cells": cells, "algorithmic_bytes": nbytes, "results": res}))
+
+
 def main():
     args = parse()
+    if args.batched:
+        return run_batched(args)
     if args.gpus != 1:
         print(json.dumps({"error": "this build targets one GPU; --gpus above 1 is not supported "
                                    "(columns shard with no communication, see DESIGN.md)"}))

@@ -82,6 +82,47 @@ int vx_tdma_mixed(void* stream,
                   int32_t refine_steps);
 
 /* ------------------------------------------------------------------------
+ * Batched TDMA: many independent grids of different shapes and depths
+ * (each with its own kbot), solved by one persistent launch per 16-level nz
+ * bucket. The plan is built once (it allocates a small device table and keeps
+ * the systems' pointers); run / graph_launch allocate nothing and may be
+ * called every time step. No reference counterpart: veros solves one grid
+ * per call (veros/core/utilities.py:51-59); this serves many grids or many
+ * independent sub-domains per launch.
+ * ------------------------------------------------------------------------ */
+typedef struct vx_tdma_system {
+    const double *a, *b, *c, *d;        /* (nx, ny, nz), strides (sx, sy, 1) */
+    const double *b_edge, *d_edge;      /* nullable, same layout */
+    const int32_t* kbot;                /* (nx, ny), strides (kbot_sx, kbot_sy) */
+    int64_t kbot_sx, kbot_sy;
+    double* x;                          /* output, same layout as a */
+    int64_t nx, ny;
+    int32_t nz;
+    int32_t _pad;
+    int64_t sx, sy;
+} vx_tdma_system;
+
+typedef struct vx_tdma_plan vx_tdma_plan;
+
+/* mode 0: float64 Thomas (vx_tdma_f64); 1: mixed with one refinement step
+ * (vx_tdma_mixed, refine_steps = 1). Validation errors leave *out NULL. */
+int vx_tdma_plan_create(const vx_tdma_system* systems, int32_t nsys, int32_t mode, vx_tdma_plan** out);
+/* Solve every system: one launch per non-empty nz bucket, on `stream`. */
+int vx_tdma_plan_run(vx_tdma_plan* plan, void* stream);
+/* Autotune: time (block size, blocks per CU) candidates per bucket on the
+ * plan's data, keep the fastest; synchronises the stream. Choices are cached
+ * process-wide for later plans of the same bucket, mode and size. */
+int vx_tdma_plan_tune(vx_tdma_plan* plan, void* stream);
+/* First call captures vx_tdma_plan_run into a hipGraph; every call launches
+ * that graph on `stream`. */
+int vx_tdma_plan_graph_launch(vx_tdma_plan* plan, void* stream);
+/* Buckets, total columns, and per bucket (nz bucket, block size, blocks per
+ * CU or 0 for one block per 256 columns) into choices[3*q..3*q+2]. */
+int vx_tdma_plan_info(const vx_tdma_plan* plan, int32_t* nbuckets, int64_t* total_columns,
+                      int32_t* choices, int32_t max_choices);
+void vx_tdma_plan_destroy(vx_tdma_plan* plan);
+
+/* ------------------------------------------------------------------------
  * Isoneutral stencils. Arrays are full veros state arrays including the
  * 2-cell halos: X = nx + 4, Y = ny + 4, each C-contiguous.
  *   temp/salt/tracer/int_drhodX : (X, Y, nz, 3)  time level innermost

@@ -79,3 +79,20 @@ def test_iso_argument_errors_without_gpu():
     assert rc == _native.VX_ERR_INVALID_ARGUMENT  # temp and salt alias
     rc = L.vx_iso_skew_diffusion_f64(None, g, p, 1, 0, 1, None, 1, 1, 1, 1, 1, 1, 1, None, None, 1)
     assert rc == _native.VX_ERR_INVALID_ARGUMENT  # null K_gm
+
+
+def test_batch_plan_argument_errors_without_gpu():
+    import ctypes
+    L = _native.lib()
+    h = ctypes.c_void_p()
+    arr = (_native.VxTdmaSystem * 1)()
+    assert L.vx_tdma_plan_create(arr, 0, 0, ctypes.byref(h)) == _native.VX_ERR_INVALID_ARGUMENT
+    assert L.vx_tdma_plan_create(arr, 1, 5, ctypes.byref(h)) == _native.VX_ERR_INVALID_ARGUMENT
+    arr[0].nz = 200
+    arr[0].nx = arr[0].ny = 2
+    assert L.vx_tdma_plan_create(arr, 1, 0, ctypes.byref(h)) == _native.VX_ERR_UNSUPPORTED
+    arr[0].nz = 8
+    assert L.vx_tdma_plan_create(arr, 1, 0, ctypes.byref(h)) == _native.VX_ERR_INVALID_ARGUMENT  # null arrays
+    assert not h.value
+    assert L.vx_tdma_plan_run(None, None) == _native.VX_ERR_INVALID_ARGUMENT
+    L.vx_tdma_plan_destroy(None)

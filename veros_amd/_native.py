@@ -31,6 +31,16 @@ class VxGrid(ctypes.Structure):
     ]
 
 
+class VxTdmaSystem(ctypes.Structure):
+    _fields_ = [
+        ("a", c_void_p), ("b", c_void_p), ("c", c_void_p), ("d", c_void_p),
+        ("b_edge", c_void_p), ("d_edge", c_void_p),
+        ("kbot", c_void_p), ("kbot_sx", c_i64), ("kbot_sy", c_i64),
+        ("x", c_void_p),
+        ("nx", c_i64), ("ny", c_i64), ("nz", c_i32), ("_pad", c_i32), ("sx", c_i64), ("sy", c_i64),
+    ]
+
+
 class VxIsoParams(ctypes.Structure):
     _fields_ = [(name, c_f64) for name in (
         "drhodT", "drhodS", "K_iso_steep", "iso_slopec", "iso_dslope", "dt_tracer", "grav", "rho_0")]
@@ -45,6 +55,14 @@ _SIGNATURES = {
                                                                 c_i64, c_i64, c_i32, c_i64, c_i64]),
     "vx_tdma_mixed": (ctypes.c_int, [c_void_p] + [c_void_p] * 6 + [c_void_p, c_i64, c_i64, c_void_p,
                                                                   c_i64, c_i64, c_i32, c_i64, c_i64, c_i32]),
+    "vx_tdma_plan_create": (ctypes.c_int, [ctypes.POINTER(VxTdmaSystem), c_i32, c_i32,
+                                           ctypes.POINTER(c_void_p)]),
+    "vx_tdma_plan_run": (ctypes.c_int, [c_void_p, c_void_p]),
+    "vx_tdma_plan_tune": (ctypes.c_int, [c_void_p, c_void_p]),
+    "vx_tdma_plan_graph_launch": (ctypes.c_int, [c_void_p, c_void_p]),
+    "vx_tdma_plan_info": (ctypes.c_int, [c_void_p, ctypes.POINTER(c_i32), ctypes.POINTER(c_i64),
+                                         ctypes.POINTER(c_i32), c_i32]),
+    "vx_tdma_plan_destroy": (None, [c_void_p]),
     "vx_iso_pre_f64": (ctypes.c_int, [c_void_p, ctypes.POINTER(VxGrid), ctypes.POINTER(VxIsoParams),
                                       c_void_p, c_void_p, c_i32, c_void_p] + [c_void_p] * 7),
     "vx_iso_workspace_bytes": (c_i64, [ctypes.POINTER(VxGrid), c_i32]),
