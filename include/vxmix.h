@@ -81,6 +81,28 @@ int vx_tdma_mixed(void* stream,
                   int64_t nx, int64_t ny, int32_t nz, int64_t sx, int64_t sy,
                   int32_t refine_steps);
 
+/*
+ * Reverse-mode derivative of vx_tdma_f64 (the pullback of
+ * veros/core/utilities.py:51-59 solve_implicit, which the reference gets from
+ * JAX's autodiff of its tridiagonal solve). Given the forward inputs, the
+ * forward solution x and gx = dL/dx, solves the transposed system
+ * A^T lam = gx per wet column and writes
+ *   gd = lam, gb = -lam x, ga[k] = -lam[k] x[k-1], gc[k] = -lam[k] x[k+1]
+ * on the wet rows (zero on dry rows, ga on the edge row and gc on the top
+ * row). When b_edge (d_edge) was given, the edge row's derivative goes to
+ * gb_edge (gd_edge) and gb (gd) is zero there; gb_edge/gd_edge must be
+ * non-NULL exactly when b_edge/d_edge are, and are zero off the edge row.
+ * All arrays share the layout of vx_tdma_f64. Deterministic (no atomics).
+ */
+int vx_tdma_adjoint_f64(void* stream,
+                        const double* a, const double* b, const double* c,
+                        const double* b_edge, const double* d_edge,
+                        const int32_t* kbot, int64_t kbot_sx, int64_t kbot_sy,
+                        const double* x, const double* gx,
+                        double* ga, double* gb, double* gc, double* gd,
+                        double* gb_edge, double* gd_edge,
+                        int64_t nx, int64_t ny, int32_t nz, int64_t sx, int64_t sy);
+
 /* ------------------------------------------------------------------------
  * Batched TDMA: many independent grids of different shapes and depths
  * (each with its own kbot), solved by one persistent launch per 16-level nz
@@ -218,6 +240,30 @@ int vx_iso_skew_diffusion_f64(void* stream, const vx_grid* grid, const vx_iso_pa
                               double* dtr_iso,
                               const double* int_drhodX, double* P_diss_skew,
                               void* workspace);
+
+/*
+ * Reverse-mode derivative of one vx_iso_diffusion_f64 step (the JAX pullback
+ * of veros/core/isoneutral/diffusion.py:172-207 with iso=True and the energy
+ * branch off) with respect to the tracer and K_iso, holding the isoneutral
+ * tensor (Ai_*, K_11, K_22, K_33) fixed. The forward step maps
+ * (T[tau], T[taup1], K_iso) to T_new = tracer[..., taup1] after the call.
+ * `tracer` holds the forward INPUT (level tau is read). Given g_new = dL/dT_new
+ * (X, Y, nz), writes dL/dT[tau] -> g_tau, dL/dT[taup1] -> g_taup1 and
+ * dL/dK_iso -> g_K_iso, each (X, Y, nz) and fully overwritten.
+ * `workspace` must hold vx_iso_adjoint_workspace_bytes(grid) bytes. The
+ * scatter into g_tau and g_K_iso uses float64 atomics: results agree run to
+ * run to rounding, not bitwise.
+ */
+int64_t vx_iso_adjoint_workspace_bytes(const vx_grid* grid);
+int vx_iso_diffusion_adjoint_f64(void* stream, const vx_grid* grid, const vx_iso_params* prm,
+                                 const double* tracer, int32_t tau,
+                                 const double* K_iso,
+                                 const double* Ai_ez, const double* Ai_nz,
+                                 const double* Ai_bx, const double* Ai_by,
+                                 const double* K_11, const double* K_22, const double* K_33,
+                                 const double* g_new,
+                                 double* g_tau, double* g_taup1, double* g_K_iso,
+                                 void* workspace);
 
 #ifdef __cplusplus
 }

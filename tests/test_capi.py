@@ -57,6 +57,22 @@ def test_tdma_argument_errors_without_gpu():
     assert rc == _native.VX_ERR_INVALID_ARGUMENT
 
 
+def test_tdma_adjoint_argument_errors_without_gpu():
+    L = _native.lib()
+
+    def call(be=None, de=None, gbe=None, gde=None, gd=1, nz=4, nx=2):
+        return L.vx_tdma_adjoint_f64(None, 1, 1, 1, be, de, 1, 1, 1, 1, 1, 1, 1, 1, gd, gbe, gde,
+                                     nx, 2, nz, 8, 4)
+
+    assert call(be=1) == _native.VX_ERR_INVALID_ARGUMENT            # b_edge without gb_edge
+    assert call(gde=1) == _native.VX_ERR_INVALID_ARGUMENT           # gd_edge without d_edge
+    assert call(gd=None) == _native.VX_ERR_INVALID_ARGUMENT
+    assert call(nz=0) == _native.VX_ERR_INVALID_ARGUMENT
+    assert call(nz=_native.max_nz() + 1) == _native.VX_ERR_UNSUPPORTED
+    assert call(nx=0) == _native.VX_OK
+    assert "vx_tdma_adjoint_f64" in L.vx_last_error().decode()
+
+
 def test_iso_argument_errors_without_gpu():
     L = _native.lib()
     g = _native.VxGrid()
@@ -79,6 +95,12 @@ def test_iso_argument_errors_without_gpu():
     assert rc == _native.VX_ERR_INVALID_ARGUMENT  # temp and salt alias
     rc = L.vx_iso_skew_diffusion_f64(None, g, p, 1, 0, 1, None, 1, 1, 1, 1, 1, 1, 1, None, None, 1)
     assert rc == _native.VX_ERR_INVALID_ARGUMENT  # null K_gm
+    assert L.vx_iso_adjoint_workspace_bytes(g) == 4 * 8 * 8 * 8 * 8
+    assert L.vx_iso_adjoint_workspace_bytes(None) == -1
+    rc = L.vx_iso_diffusion_adjoint_f64(None, g, p, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, None)
+    assert rc == _native.VX_ERR_INVALID_ARGUMENT  # no workspace
+    rc = L.vx_iso_diffusion_adjoint_f64(None, g, p, 1, 3, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1)
+    assert rc == _native.VX_ERR_INVALID_ARGUMENT  # tau out of range
 
 
 def test_batch_plan_argument_errors_without_gpu():

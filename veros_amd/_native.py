@@ -55,6 +55,8 @@ _SIGNATURES = {
                                                                 c_i64, c_i64, c_i32, c_i64, c_i64]),
     "vx_tdma_mixed": (ctypes.c_int, [c_void_p] + [c_void_p] * 6 + [c_void_p, c_i64, c_i64, c_void_p,
                                                                   c_i64, c_i64, c_i32, c_i64, c_i64, c_i32]),
+    "vx_tdma_adjoint_f64": (ctypes.c_int, [c_void_p] + [c_void_p] * 5 + [c_void_p, c_i64, c_i64]
+                            + [c_void_p] * 8 + [c_i64, c_i64, c_i32, c_i64, c_i64]),
     "vx_tdma_plan_create": (ctypes.c_int, [ctypes.POINTER(VxTdmaSystem), c_i32, c_i32,
                                            ctypes.POINTER(c_void_p)]),
     "vx_tdma_plan_run": (ctypes.c_int, [c_void_p, c_void_p]),
@@ -75,6 +77,10 @@ _SIGNATURES = {
     "vx_iso_skew_diffusion_f64": (ctypes.c_int, [c_void_p, ctypes.POINTER(VxGrid), ctypes.POINTER(VxIsoParams),
                                                  c_void_p, c_i32, c_i32, c_void_p] + [c_void_p] * 6
                                   + [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "vx_iso_adjoint_workspace_bytes": (c_i64, [ctypes.POINTER(VxGrid)]),
+    "vx_iso_diffusion_adjoint_f64": (ctypes.c_int, [c_void_p, ctypes.POINTER(VxGrid),
+                                                    ctypes.POINTER(VxIsoParams), c_void_p, c_i32]
+                                     + [c_void_p] * 8 + [c_void_p] * 4 + [c_void_p]),
 }
 
 _lock = threading.Lock()

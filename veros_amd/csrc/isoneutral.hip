@@ -690,6 +690,224 @@ __global__ __launch_bounds__(256) void iso_step_tile_kernel(Grid g, Prm p, StepA
 }
 
 // ---------------------------------------------------------------------------
+// Adjoint of one isoneutral tracer step (iso=True, energy branch off) with the
+// isoneutral tensor (Ai_*, K_11, K_22, K_33) held fixed:
+//   T_new = M^-1 (T[taup1] + dt * D(T[tau], K_iso))   on interior wet cells,
+//   T_new = T[taup1] + dt * D                          on interior dry cells,
+//   T_new = T[taup1]                                   elsewhere,
+// with D the flux divergence (linear in T[tau], and in K_iso through K1 = K2 =
+// K_iso). Given mu = dL/dT_new:
+//   adj_column : lam = M^-T mu (transposed K33 Thomas, rows re-derived from
+//                K_33), g_taup1 = lam, nu = dt * lam on interior cells
+//   adj_faces  : adjoints of flux_east / flux_north / flux_top from nu
+//   adj_scatter: each face scatters (atomics) into dL/dT[tau] and dL/dK_iso
+// ---------------------------------------------------------------------------
+struct AdjArgs {
+    const double* tr;       // forward input tracer (X, Y, nz, 3), level tau read
+    int tau;
+    const double* Kiso;
+    const double *Aez, *Anz, *Abx, *Aby, *K11, *K22, *K33;
+    const double* mu;       // dL/dT_new (X, Y, nz)
+    double *g_tau, *g_taup1, *g_K;
+    double *nu, *mfe, *mfn, *mft;  // workspace (X, Y, nz) each
+};
+
+// g_taup1 = mu, nu = 0 on interior cells below, g_tau = g_K = 0 everywhere
+__global__ __launch_bounds__(256) void iso_adj_init_kernel(int64_t n, AdjArgs a) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    a.g_taup1[t] = a.mu[t];
+    a.g_tau[t] = 0.0;
+    a.g_K[t] = 0.0;
+    a.nu[t] = 0.0;
+    a.mfe[t] = 0.0;
+    a.mfn[t] = 0.0;
+    a.mft[t] = 0.0;
+}
+
+template <int NZB>
+__global__ __launch_bounds__(256) void iso_adj_column_kernel(Grid g, Prm p, AdjArgs a) {
+    const int nz = g.nz;
+    const int ny4 = g.Y - 4;
+    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= (int64_t)(g.X - 4) * ny4) return;
+    const int i = 2 + (int)(col / ny4);
+    const int j = 2 + (int)(col - (int64_t)(i - 2) * ny4);
+    const int kb = kb_of(g, i, j);
+    const int k0 = (kb >= 1 && kb <= nz) ? kb - 1 : nz;
+    const int64_t o = idx3(g, i, j, 0);
+    const double dt = p.dt_tracer;
+    // transposed rows: sub'[k] = c[k-1] = -delta[k-1]/dzt[k-1], sup'[k] = a[k+1] = -delta[k]/dzt[k+1]
+    double cp[NZB], lm[NZB];
+    double cpp = 0.0, dpp = 0.0, delta_m = 0.0;
+#pragma unroll
+    for (int k = 0; k < NZB; ++k) {
+        if (k < nz && k >= k0) {
+            const double delta = (k < nz - 1) ? dt / g.dzw[k] * a.K33[o + k] : 0.0;
+            const bool edge = k == k0;
+            const double dz = g.dzt[k];
+            const double diag = edge ? 1 + delta / dz : 1 + (delta + delta_m) / dz;
+            const double sub = edge ? 0.0 : -delta_m / g.dzt[k - 1];
+            const double sup = (k < nz - 1) ? -delta / g.dzt[k + 1] : 0.0;
+            const double r = 1.0 / (diag - sub * cpp);
+            cpp = sup * r;
+            dpp = (a.mu[o + k] - sub * dpp) * r;
+            delta_m = delta;
+        }
+        cp[k] = cpp;
+        lm[k] = dpp;
+    }
+    double ln = 0.0;
+#pragma unroll
+    for (int k = NZB - 1; k >= 0; --k) {
+        if (k < nz && k >= k0) {
+            ln = lm[k] - cp[k] * ln;
+            lm[k] = ln;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NZB; ++k) {
+        if (k < nz) {
+            const double lam = (k >= k0) ? lm[k] : a.mu[o + k];
+            if (k >= k0) a.g_taup1[o + k] = lam;
+            a.nu[o + k] = dt * lam;
+        }
+    }
+}
+
+// adjoints of the face fluxes: dtr = m*((fe - fe_w)/(cost dxt) + (fn - fn_s)/(cost dyt)) + m*(ft - ft_m)/dzt
+__global__ __launch_bounds__(256) void iso_adj_faces_kernel(Grid g, AdjArgs a) {
+    const CellIdx ci = cell_of(1, g.X - 3, 1, g.Y - 3);
+    if (!ci.valid) return;
+    const int i = ci.i, j = ci.j, k = ci.k;
+    const int nz = g.nz;
+    const uint32_t si = (uint32_t)g.Y * (uint32_t)nz, sj = (uint32_t)nz;
+    const uint32_t c = (uint32_t)i * si + (uint32_t)j * sj + (uint32_t)k;
+    auto inner = [&](int ii, int jj) { return ii >= 2 && ii <= g.X - 3 && jj >= 2 && jj <= g.Y - 3; };
+    auto wnu = [&](int ii, int jj, int kk, uint32_t o) {
+        return inner(ii, jj) ? a.nu[o] * m01(kb_of(g, ii, jj), kk) : 0.0;
+    };
+    if (j >= 2) {  // flux_east at (i, j, k): divergence of cells i and i+1
+        a.mfe[c] = wnu(i, j, k, c) / (g.cost[j] * g.dxt[i]) - wnu(i + 1, j, k, c + si) / (g.cost[j] * g.dxt[i + 1]);
+    }
+    if (i >= 2) {  // flux_north at (i, j, k): cells j and j+1
+        a.mfn[c] = wnu(i, j, k, c) / (g.cost[j] * g.dyt[j]) - wnu(i, j + 1, k, c + sj) / (g.cost[j + 1] * g.dyt[j + 1]);
+    }
+    if (i >= 2 && j >= 2 && k < nz - 1) {  // flux_top at (i, j, k): levels k and k+1
+        a.mft[c] = wnu(i, j, k, c) / g.dzt[k] - wnu(i, j, k + 1, c + 1) / g.dzt[k + 1];
+    }
+}
+
+__device__ __forceinline__ void adj_add(double* p, double v) {
+    if (v != 0.0) unsafeAtomicAdd(p, v);
+}
+
+__global__ __launch_bounds__(256) void iso_adj_scatter_kernel(Grid g, AdjArgs a) {
+    const CellIdx ci = cell_of(1, g.X - 3, 1, g.Y - 3);
+    if (!ci.valid) return;
+    const int i = ci.i, j = ci.j, k = ci.k;
+    const int nz = g.nz;
+    const uint32_t si = (uint32_t)g.Y * (uint32_t)nz, sj = (uint32_t)nz;
+    const uint32_t c = (uint32_t)i * si + (uint32_t)j * sj + (uint32_t)k;
+    const double* __restrict__ tr = a.tr + a.tau;
+    const bool hm = k >= 1;
+    const uint32_t up = (k + 1 < nz) ? 1u : 0u, dn = hm ? 1u : 0u;
+    const double* K = a.Kiso;
+
+    if (j >= 2) {  // flux_east (veros/core/isoneutral/diffusion.py:22-47)
+        const double mu = a.mfe[c];
+        if (mu != 0.0) {
+            const uint32_t ce = c + si;
+            const double diffloc = hm ? 0.25 * (K[c] + K[c - 1] + K[ce] + K[ce - 1]) : 0.5 * (K[c] + K[ce]);
+            const double* A = a.Aez + 4u * c;  // (ip, kr) at 2 ip + kr
+            const double q4 = 1.0 / (4.0 * g.dzt[k]);
+            const double t0 = tr[3u * c], t0u = tr[3u * (c + up)], t0d = tr[3u * (c - dn)];
+            const double t1 = tr[3u * ce], t1u = tr[3u * (ce + up)], t1d = tr[3u * (ce - dn)];
+            // diffloc terms
+            const double w00 = mu * diffloc * A[0] * q4, w01 = mu * diffloc * A[1] * q4;
+            const double w10 = mu * diffloc * A[2] * q4, w11 = mu * diffloc * A[3] * q4;
+            adj_add(a.g_tau + c, w00 - w01);
+            adj_add(a.g_tau + c - dn, -w00);
+            adj_add(a.g_tau + c + up, w01);
+            adj_add(a.g_tau + ce, w10 - w11);
+            adj_add(a.g_tau + ce - dn, -w10);
+            adj_add(a.g_tau + ce + up, w11);
+            // K_11 term
+            const double wk = mu * a.K11[c] / (g.cost[j] * g.dxu[i]);
+            adj_add(a.g_tau + ce, wk);
+            adj_add(a.g_tau + c, -wk);
+            // K_iso through diffloc
+            const double S = (A[0] * (t0 - t0d) + A[2] * (t1 - t1d) + A[1] * (t0u - t0) + A[3] * (t1u - t1)) * q4;
+            const double wK = mu * S * (hm ? 0.25 : 0.5);
+            adj_add(a.g_K + c, wK);
+            adj_add(a.g_K + ce, wK);
+            if (hm) {
+                adj_add(a.g_K + c - 1, wK);
+                adj_add(a.g_K + ce - 1, wK);
+            }
+        }
+    }
+    if (i >= 2) {  // flux_north (:49-77)
+        const double mu = a.mfn[c] * g.cosu[j];
+        if (mu != 0.0) {
+            const uint32_t cn = c + sj;
+            const double diffloc = hm ? 0.25 * (K[c] + K[c - 1] + K[cn] + K[cn - 1]) : 0.5 * (K[c] + K[cn]);
+            const double* A = a.Anz + 4u * c;
+            const double q4 = 1.0 / (4.0 * g.dzt[k]);
+            const double t0 = tr[3u * c], t0u = tr[3u * (c + up)], t0d = tr[3u * (c - dn)];
+            const double t1 = tr[3u * cn], t1u = tr[3u * (cn + up)], t1d = tr[3u * (cn - dn)];
+            const double w00 = mu * diffloc * A[0] * q4, w01 = mu * diffloc * A[1] * q4;
+            const double w10 = mu * diffloc * A[2] * q4, w11 = mu * diffloc * A[3] * q4;
+            adj_add(a.g_tau + c, w00 - w01);
+            adj_add(a.g_tau + c - dn, -w00);
+            adj_add(a.g_tau + c + up, w01);
+            adj_add(a.g_tau + cn, w10 - w11);
+            adj_add(a.g_tau + cn - dn, -w10);
+            adj_add(a.g_tau + cn + up, w11);
+            const double wk = mu * a.K22[c] / g.dyu[j];
+            adj_add(a.g_tau + cn, wk);
+            adj_add(a.g_tau + c, -wk);
+            const double S = (A[0] * (t0 - t0d) + A[2] * (t1 - t1d) + A[1] * (t0u - t0) + A[3] * (t1u - t1)) * q4;
+            const double wK = mu * S * (hm ? 0.25 : 0.5);
+            adj_add(a.g_K + c, wK);
+            adj_add(a.g_K + cn, wK);
+            if (hm) {
+                adj_add(a.g_K + c - 1, wK);
+                adj_add(a.g_K + cn - 1, wK);
+            }
+        }
+    }
+    if (i >= 2 && j >= 2 && k < nz - 1) {  // flux_top (:79-111)
+        const double mu = a.mft[c];
+        if (mu != 0.0) {
+            const double K2 = K[c];
+            const double* B = a.Abx + 4u * c;
+            const double* C = a.Aby + 4u * c;
+            const uint32_t cw = c - si, ce = c + si, cs = c - sj, cn = c + sj;
+            const double fx = 1.0 / (4 * g.dxt[i]) / g.cost[j];
+            const double fy = 1.0 / (4 * g.dyt[j] * g.cost[j]);
+            double S = 0.0;
+#pragma unroll
+            for (int kr = 0; kr < 2; ++kr) {
+                // x: (ip=0) tr[i] - tr[i-1], (ip=1) tr[i+1] - tr[i], at level k + kr
+                const double bx0 = B[kr] * fx, bx1 = B[2 + kr] * fx;
+                const double cy0 = C[kr] * g.cosu[j - 1] * fy, cy1 = C[2 + kr] * g.cosu[j] * fy;
+                const uint32_t ck = c + kr;
+                S += bx0 * (tr[3u * ck] - tr[3u * (cw + kr)]) + bx1 * (tr[3u * (ce + kr)] - tr[3u * ck]) +
+                     cy0 * (tr[3u * ck] - tr[3u * (cs + kr)]) + cy1 * (tr[3u * (cn + kr)] - tr[3u * ck]);
+                const double m = mu * K2;
+                adj_add(a.g_tau + ck, m * (bx0 - bx1 + cy0 - cy1));
+                adj_add(a.g_tau + cw + kr, -m * bx0);
+                adj_add(a.g_tau + ce + kr, m * bx1);
+                adj_add(a.g_tau + cs + kr, -m * cy0);
+                adj_add(a.g_tau + cn + kr, m * cy1);
+            }
+            adj_add(a.g_K + c, mu * S);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 namespace {
@@ -914,4 +1132,58 @@ extern "C" int vx_iso_skew_diffusion_f64(void* stream, const vx_grid* grid, cons
 extern "C" int64_t vx_iso_workspace_bytes(const vx_grid* grid, int32_t ntracers) {
     if (!grid || ntracers < 1 || ntracers > 2) return -1;
     return 3 * (int64_t)ntracers * (grid->nx + 4) * (grid->ny + 4) * (int64_t)grid->nz * (int64_t)sizeof(double);
+}
+
+extern "C" int64_t vx_iso_adjoint_workspace_bytes(const vx_grid* grid) {
+    if (!grid) return -1;
+    return 4 * (grid->nx + 4) * (grid->ny + 4) * (int64_t)grid->nz * (int64_t)sizeof(double);
+}
+
+extern "C" int vx_iso_diffusion_adjoint_f64(void* stream, const vx_grid* grid, const vx_iso_params* prm,
+                                            const double* tracer, int32_t tau, const double* K_iso,
+                                            const double* Ai_ez, const double* Ai_nz, const double* Ai_bx,
+                                            const double* Ai_by, const double* K_11, const double* K_22,
+                                            const double* K_33, const double* g_new, double* g_tau,
+                                            double* g_taup1, double* g_K_iso, void* workspace) {
+    using namespace vx;
+    const char* who = "vx_iso_diffusion_adjoint_f64";
+    Grid g;
+    int rc = check_common(grid, prm, tau, 0, g, who);
+    if (rc) return rc;
+    if (!tracer || !K_iso || !tensor_ok(Ai_ez, Ai_nz, Ai_bx, Ai_by, K_11, K_22) || !K_33 || !g_new || !g_tau ||
+        !g_taup1 || !g_K_iso || !workspace)
+        return fail(VX_ERR_INVALID_ARGUMENT, "%s: null array pointer", who);
+    if ((int64_t)g.X * g.Y * g.nz * 4 >= ((int64_t)1 << 32))
+        return fail(VX_ERR_UNSUPPORTED, "%s: grid too large for 32-bit offsets", who);
+    const int64_t n3 = (int64_t)g.X * g.Y * g.nz;
+    double* w = static_cast<double*>(workspace);
+    AdjArgs a{tracer, tau, K_iso, Ai_ez, Ai_nz, Ai_bx, Ai_by, K_11, K_22, K_33, g_new, g_tau, g_taup1, g_K_iso,
+              w, w + n3, w + 2 * n3, w + 3 * n3};
+    hipStream_t s = (hipStream_t)stream;
+    const Prm p = make_prm(prm);
+    hipLaunchKernelGGL(iso_adj_init_kernel, dim3(blocks_for(n3, 256)), dim3(256), 0, s, n3, a);
+    if ((rc = check_launch(who))) return rc;
+    const int64_t ncol = (int64_t)(g.X - 4) * (g.Y - 4);
+    const int nzb = ((g.nz + 15) / 16) * 16;
+    const dim3 cgrid(blocks_for(ncol, 256)), cblk(256);
+#define VX_ADJ_CASE(N) \
+    case N: hipLaunchKernelGGL(iso_adj_column_kernel<N>, cgrid, cblk, 0, s, g, p, a); break;
+    switch (nzb) {
+        VX_ADJ_CASE(16)
+        VX_ADJ_CASE(32)
+        VX_ADJ_CASE(48)
+        VX_ADJ_CASE(64)
+        VX_ADJ_CASE(80)
+        VX_ADJ_CASE(96)
+        VX_ADJ_CASE(112)
+        VX_ADJ_CASE(128)
+        default: return fail(VX_ERR_UNSUPPORTED, "%s: nz=%d", who, g.nz);
+    }
+#undef VX_ADJ_CASE
+    if ((rc = check_launch(who))) return rc;
+    const dim3 blk = cell_block(g.nz);
+    hipLaunchKernelGGL(iso_adj_faces_kernel, cell_grid(g.X - 3, g.Y - 3, blk), blk, 0, s, g, a);
+    if ((rc = check_launch(who))) return rc;
+    hipLaunchKernelGGL(iso_adj_scatter_kernel, cell_grid(g.X - 3, g.Y - 3, blk), blk, 0, s, g, a);
+    return check_launch(who);
 }

@@ -29,6 +29,82 @@ __global__ __launch_bounds__(kTdmaBlock) void tdma_mixed_kernel(TdmaArgs p) {
 }
 
 // ---------------------------------------------------------------------------
+// Adjoint of the float64 solve: given x = A^-1 d and g = dL/dx, the
+// transposed system A^T lam = g (sub-diagonal c[k-1], super-diagonal a[k+1]
+// on the wet rows) is solved by Thomas, re-deriving the elimination from a, b,
+// c instead of storing forward factors; then dL/dd = lam, dL/db = -lam*x,
+// dL/da[k] = -lam[k]*x[k-1] (k above the edge), dL/dc[k] = -lam[k]*x[k+1]
+// (k below the top). Entries the forward solve ignores (a at the edge, c at
+// the top, dry cells, b/d at the edge when b_edge/d_edge replace them) get 0.
+// ---------------------------------------------------------------------------
+struct TdmaAdjArgs {
+    TdmaArgs f;       // forward inputs; f.x is the forward solution
+    const double* g;  // dL/dx
+    double *ga, *gb, *gc, *gd;
+    double *gbe, *gde;  // dL/db_edge, dL/dd_edge (nullable; used when b_edge / d_edge were given)
+};
+
+template <int NZB>
+__global__ __launch_bounds__(kTdmaBlock) void tdma_adjoint_kernel(TdmaAdjArgs q) {
+    const TdmaArgs& p = q.f;
+    ColumnRef cr;
+    if (!column_of(p, cr)) return;
+    const int nz = p.nz;
+    const int k0 = cr.k0;
+    const int64_t o = cr.base;
+    const bool has_be = p.b_edge != nullptr;
+    const bool has_de = p.d_edge != nullptr;
+    double cp[NZB], lm[NZB];
+    double cpp = 0.0, dpp = 0.0;
+#pragma unroll
+    for (int k = 0; k < NZB; ++k) {
+        if (k < nz && k >= k0) {
+            const double sub = (k > k0) ? p.c[o + k - 1] : 0.0;
+            const double diag = (k == k0 && has_be) ? p.b_edge[o + k] : p.b[o + k];
+            const double sup = (k < nz - 1) ? p.a[o + k + 1] : 0.0;
+            const double r = 1.0 / (diag - sub * cpp);
+            cpp = sup * r;
+            dpp = (q.g[o + k] - sub * dpp) * r;
+        }
+        cp[k] = cpp;
+        lm[k] = dpp;
+    }
+    double ln = 0.0;
+#pragma unroll
+    for (int k = NZB - 1; k >= 0; --k) {
+        if (k < nz && k >= k0) {
+            ln = lm[k] - cp[k] * ln;
+            lm[k] = ln;
+        }
+    }
+    double xm = 0.0;  // x[k-1]
+#pragma unroll
+    for (int k = 0; k < NZB; ++k) {
+        if (k < nz) {
+            double ga = 0.0, gb = 0.0, gc = 0.0, gd = 0.0, gbe = 0.0, gde = 0.0;
+            if (k >= k0) {
+                const double lam = lm[k];
+                const double xk = p.x[o + k];
+                const bool edge = k == k0;
+                if (edge && has_de) gde = lam;
+                else gd = lam;
+                if (edge && has_be) gbe = -lam * xk;
+                else gb = -lam * xk;
+                if (!edge) ga = -lam * xm;
+                if (k < nz - 1) gc = -lam * p.x[o + k + 1];
+                xm = xk;
+            }
+            q.ga[o + k] = ga;
+            q.gb[o + k] = gb;
+            q.gc[o + k] = gc;
+            q.gd[o + k] = gd;
+            if (q.gbe) q.gbe[o + k] = gbe;
+            if (q.gde) q.gde[o + k] = gde;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 namespace {
@@ -145,4 +221,40 @@ extern "C" int vx_tdma_mixed(void* stream, const double* a, const double* b, con
         case 1: return launch_bucket<MixK1>(p, (hipStream_t)stream, "vx_tdma_mixed");
         default: return launch_bucket<MixK2>(p, (hipStream_t)stream, "vx_tdma_mixed");
     }
+}
+
+extern "C" int vx_tdma_adjoint_f64(void* stream, const double* a, const double* b, const double* c,
+                                   const double* b_edge, const double* d_edge, const int32_t* kbot,
+                                   int64_t kbot_sx, int64_t kbot_sy, const double* x, const double* gx,
+                                   double* ga, double* gb, double* gc, double* gd, double* gb_edge,
+                                   double* gd_edge, int64_t nx, int64_t ny, int32_t nz, int64_t sx, int64_t sy) {
+    using namespace vx;
+    const char* who = "vx_tdma_adjoint_f64";
+    TdmaArgs p = make_args(a, b, c, gx, b_edge, d_edge, kbot, kbot_sx, kbot_sy, const_cast<double*>(x), nx, ny,
+                           nz, sx, sy);
+    int rc = validate_tdma(p, nx, ny, nz, who);
+    if (rc != VX_OK || p.ncol == 0) return rc;
+    if (!gx || !ga || !gb || !gc || !gd) return fail(VX_ERR_INVALID_ARGUMENT, "%s: null gradient array", who);
+    if ((b_edge != nullptr) != (gb_edge != nullptr) || (d_edge != nullptr) != (gd_edge != nullptr))
+        return fail(VX_ERR_INVALID_ARGUMENT, "%s: gb_edge/gd_edge must be given exactly when b_edge/d_edge are",
+                    who);
+    TdmaAdjArgs q{p, gx, ga, gb, gc, gd, gb_edge, gd_edge};
+    const int nzb = ((nz + 15) / 16) * 16;
+    const dim3 grid((unsigned)((p.ncol + kTdmaBlock - 1) / kTdmaBlock)), block(kTdmaBlock);
+    hipStream_t s = (hipStream_t)stream;
+#define VX_ADJ_CASE(N) \
+    case N: hipLaunchKernelGGL(tdma_adjoint_kernel<N>, grid, block, 0, s, q); break;
+    switch (nzb) {
+        VX_ADJ_CASE(16)
+        VX_ADJ_CASE(32)
+        VX_ADJ_CASE(48)
+        VX_ADJ_CASE(64)
+        VX_ADJ_CASE(80)
+        VX_ADJ_CASE(96)
+        VX_ADJ_CASE(112)
+        VX_ADJ_CASE(128)
+        default: return fail(VX_ERR_UNSUPPORTED, "%s: nz=%d", who, nz);
+    }
+#undef VX_ADJ_CASE
+    return check_launch(who);
 }

@@ -318,6 +318,22 @@ def iso_config2(seed=1002, nx=1440, ny=720, nz=80, **kw):
                      dz_top=10.0, dz_bottom=250.0, kmin=min(5, nz), land_frac=0.35, **kw)
 
 
+def mixing_config4(seed=1004, nx=180, ny=90, nz=40):
+    """The differentiated step's inputs on a 180x90x40 grid: a config-1 style
+    state (2 degree channel, dz 10->250 m, 25% land), config-0 style
+    vertical-diffusion coefficients on its interior columns (same kbot),
+    and loss weights w ~ N(0, 1) on T_new. Returns (state, TdmaProblem, w)."""
+    rng = np.random.default_rng(seed)
+    st = iso_state(rng, nx, ny, nz, dx_deg=2.0, dy_deg=1.0, y_origin=-79.0,
+                   dz_top=10.0, dz_bottom=250.0, land_frac=0.25)
+    st.settings.enable_conserve_energy = False
+    dz = geometric_dz(nz, 10.0, 250.0)
+    p = diffusion_tdma(rng, nx, ny, nz, dz=dz, land_frac=0.0, with_b_edge=True)
+    p.kbot = np.ascontiguousarray(st.variables.kbot[2:-2, 2:-2]).astype(np.int32)
+    w = rng.standard_normal(st.variables.K_iso.shape)
+    return st, p, w
+
+
 def config2_settings(nx=1440, ny=720, nz=80):
     """Linear EOS rho = 1025*(1 - 2e-4*T + 7.6e-4*(S-35)) and ACC mixing constants."""
     return MixSettings(nx=nx, ny=ny, nz=nz, drhodT=-1025.0 * 2e-4, drhodS=1025.0 * 7.6e-4)
