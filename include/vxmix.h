@@ -126,8 +126,9 @@ typedef struct vx_tdma_system {
 
 typedef struct vx_tdma_plan vx_tdma_plan;
 
-/* mode 0: float64 Thomas (vx_tdma_f64); 1: mixed with one refinement step
- * (vx_tdma_mixed, refine_steps = 1). Validation errors leave *out NULL. */
+/* mode 0: float64 Thomas (vx_tdma_f64). mode 1 (mixed with one refinement
+ * step) is reserved and currently returns VX_ERR_UNSUPPORTED.
+ * Validation errors leave *out NULL. */
 int vx_tdma_plan_create(const vx_tdma_system* systems, int32_t nsys, int32_t mode, vx_tdma_plan** out);
 /* Solve every system: one launch per non-empty nz bucket, on `stream`. */
 int vx_tdma_plan_run(vx_tdma_plan* plan, void* stream);

@@ -40,7 +40,11 @@ def _oracle(s):
                                  b_edge=f(s.get("b_edge")), d_edge=f(s.get("d_edge")))
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+MIXED_DISABLED = ("the mixed batched kernel took a device memory fault (nz bucket 80) whose cause is not "
+                  "found; mode 1 is disabled in vx_tdma_plan_create")
+
+
+@pytest.mark.parametrize("precision", ["fp64", pytest.param("mixed", marks=pytest.mark.skip(reason=MIXED_DISABLED))])
 def test_plan_matches_single_and_oracle(precision):
     from veros_amd.batched import TdmaBatch
     sys = _systems()
@@ -85,3 +89,5 @@ def test_plan_rejects_bad_systems():
         TdmaBatch([sys[0], bad])
     with pytest.raises(ValueError):
         TdmaBatch(sys, "bf16")
+    with pytest.raises(NotImplementedError):
+        TdmaBatch(sys, "mixed")
