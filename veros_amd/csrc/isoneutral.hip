@@ -42,6 +42,8 @@ __device__ __forceinline__ int64_t cidx(const Grid& g, int i, int j) { return (i
 __device__ __forceinline__ int64_t idx3(const Grid& g, int i, int j, int k) { return cidx(g, i, j) * g.nz + k; }
 __device__ __forceinline__ int kb_of(const Grid& g, int i, int j) { return g.kbot[cidx(g, i, j)]; }
 __device__ __forceinline__ double mT(int kb, int k) { return (kb > 0 && kb - 1 <= k) ? 1.0 : 0.0; }
+// Level k of a column with bottom index kb is a wet T cell (maskT = 1).
+__device__ __forceinline__ bool wet(int kb, int k) { return kb > 0 && kb - 1 <= k; }
 
 // Tracer value at time level `lev` of an (X, Y, nz, 3) array.
 __device__ __forceinline__ double tr3(const double* __restrict__ f, const Grid& g, int i, int j, int k, int lev) {
@@ -218,6 +220,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_PRE_MIN_
     const double* __restrict__ S = f.S + f.tau;
     const int kb_c = g.kbot[i * g.Y + j], kb_e = g.kbot[(i + 1) * g.Y + j], kb_w = g.kbot[(i - 1) * g.Y + j];
     const int kb_n = g.kbot[i * g.Y + j + 1], kb_s = g.kbot[i * g.Y + j - 1];
+
+    // A dry cell has every output exactly zero: its own maskT factor is in
+    // maskU (Ai_ez, K_11), maskV (Ai_nz, K_22) and maskW (Ai_bx, Ai_by, K_33),
+    // which multiply each entry and each summed term. Write the zeros over the
+    // same index ranges and skip the slopes.
+    if (!wet(kb_c, k)) {
+        const double2 z2 = make_double2(0.0, 0.0);
+        if (j >= 2) {
+            if (hm) {
+                *reinterpret_cast<double2*>(Aez + 4u * c) = z2;
+                *reinterpret_cast<double2*>(Aez + 4u * c + 2) = z2;
+            } else {
+                Aez[4u * c + 1] = 0.0;
+                Aez[4u * c + 3] = 0.0;
+            }
+            K11[c] = 0.0;
+        }
+        if (i >= 2) {
+            if (hm) {
+                *reinterpret_cast<double2*>(Anz + 4u * c) = z2;
+                *reinterpret_cast<double2*>(Anz + 4u * c + 2) = z2;
+            } else {
+                Anz[4u * c + 1] = 0.0;
+                Anz[4u * c + 3] = 0.0;
+            }
+            K22[c] = 0.0;
+        }
+        if (i >= 2 && j >= 2 && hp) {
+            *reinterpret_cast<double2*>(Abx + 4u * c) = z2;
+            *reinterpret_cast<double2*>(Abx + 4u * c + 2) = z2;
+            *reinterpret_cast<double2*>(Aby + 4u * c) = z2;
+            *reinterpret_cast<double2*>(Aby + 4u * c + 2) = z2;
+            K33[c] = 0.0;
+        }
+        return;
+    }
 
     // tracer values: c(enter), e(ast), w(est), n(orth), s(outh); p = k+1, m = k-1
     auto L2 = [&](uint32_t o, bool ok, double& t, double& sv) {
@@ -502,6 +540,10 @@ __device__ __forceinline__ double flux_top(const Grid& g, const StepArgs& a, con
 // Fluxes of NTR tracers, thread per cell (i, j) in [1, X-2) x [1, Y-2):
 // flux_east for j >= 2, flux_north for i >= 2, flux_top for i, j >= 2 (0 at the
 // top level). The tracers share every read of the isoneutral tensor.
+// Every use of a face flux is multiplied by the maskT of a cell next to the
+// face (the divergence, the lateral dissipation and the vertical mixing term
+// of that cell), so a face between two dry cells is neither computed nor
+// written; the tile kernel reads only faces that touch a wet cell.
 template <int NTR>
 __global__ __launch_bounds__(256) void iso_flux_kernel(Grid g, StepArgs a) {
     const CellIdx ci = cell_of(1, g.X - 3, 1, g.Y - 3);
@@ -509,12 +551,17 @@ __global__ __launch_bounds__(256) void iso_flux_kernel(Grid g, StepArgs a) {
     const int i = ci.i, j = ci.j, k = ci.k;
     const Strides st{(uint32_t)g.Y * (uint32_t)g.nz, (uint32_t)g.nz};
     const uint32_t c = (uint32_t)i * st.si + (uint32_t)j * st.sj + (uint32_t)k;
+    const int kb = g.kbot[i * g.Y + j];
+    const bool wc = wet(kb, k);
+    const bool need_e = j >= 2 && (wc || wet(g.kbot[(i + 1) * g.Y + j], k));
+    const bool need_n = i >= 2 && (wc || wet(g.kbot[i * g.Y + j + 1], k));
+    const bool need_t = i >= 2 && j >= 2 && (wc || (k + 1 < g.nz && wet(kb, k + 1)));
 #pragma unroll
     for (int t = 0; t < NTR; ++t) {
         const double* __restrict__ tr = a.t[t].tr + a.tau;
-        if (j >= 2) a.t[t].fe[c] = flux_east(g, a, tr, st, i, j, k);
-        if (i >= 2) a.t[t].fn[c] = flux_north(g, a, tr, st, i, j, k);
-        if (i >= 2 && j >= 2) a.t[t].ft[c] = (k < g.nz - 1) ? flux_top(g, a, tr, st, i, j, k) : 0.0;
+        if (need_e) a.t[t].fe[c] = flux_east(g, a, tr, st, i, j, k);
+        if (need_n) a.t[t].fn[c] = flux_north(g, a, tr, st, i, j, k);
+        if (need_t) a.t[t].ft[c] = (k < g.nz - 1) ? flux_top(g, a, tr, st, i, j, k) : 0.0;
     }
 }
 
@@ -550,6 +597,12 @@ __global__ __launch_bounds__(256) void iso_step_tile_kernel(Grid g, Prm p, StepA
     const bool energy = a.P != nullptr;
     const double dt = p.dt_tracer;
     const double mt = m01(kb, k);
+    // Dry cells (maskT = 0) keep tr, dtr and P unchanged: their flux
+    // divergence, dissipation and vertical mixing term are multiplied by
+    // maskT. Their lanes only hand the bottom face flux of the lowest wet cell
+    // (flux_top at k = kbot-2) to the sweep's rhs.
+    const bool wc = valid && wet(kb, k);
+    const bool wu = valid && k + 1 < nz && wet(kb, k + 1);
 
     // ---- phase 1a: fluxes from the workspace, lateral dissipation, K33 rows
     double fe_c[NTR], fe_w[NTR], fn_c[NTR], fn_s[NTR], ft_c[NTR], Ic[NTR];
@@ -557,27 +610,31 @@ __global__ __launch_bounds__(256) void iso_step_tile_kernel(Grid g, Prm p, StepA
     for (int t = 0; t < NTR; ++t) {
         const TracerArgs& ta = a.t[t];
         fe_c[t] = fe_w[t] = fn_c[t] = fn_s[t] = ft_c[t] = Ic[t] = 0.0;
-        if (valid) {
+        if (wc) {
             fe_c[t] = fe_at(g, ta, i, j, c);
             fe_w[t] = fe_at(g, ta, i - 1, j, c - st.si);
             fn_c[t] = fn_at(g, ta, i, j, c);
             fn_s[t] = fn_at(g, ta, i, j - 1, c - st.sj);
         }
-        if (interior) ft_c[t] = ta.ft[c];
+        if (interior && (wc || wu)) ft_c[t] = ta.ft[c];
         double* sTt = sT + t * 4 * tcnz;
         sTt[lo] = ft_c[t];
         if (valid && energy) {
-            // compute_dissipation (veros/core/diffusion.py:13-33)
-            const double* __restrict__ I = ta.I + a.tau;
-            Ic[t] = I[3u * c];
-            const double de = (I[3u * (c + st.si)] - Ic[t]) * fe_c[t] + (Ic[t] - I[3u * (c - st.si)]) * fe_w[t];
-            const double dn = (I[3u * (c + st.sj)] - Ic[t]) * fn_c[t] + (Ic[t] - I[3u * (c - st.sj)]) * fn_s[t];
-            const double g05 = 0.5 * p.grav / p.rho_0;
-            sTt[tcnz + lo] = g05 * de / (g.dxt[i] * g.cost[j]) + g05 * dn / (g.dyt[j] * g.cost[j]);
+            double diss = 0.0;
+            if (wc) {
+                // compute_dissipation (veros/core/diffusion.py:13-33)
+                const double* __restrict__ I = ta.I + a.tau;
+                Ic[t] = I[3u * c];
+                const double de = (I[3u * (c + st.si)] - Ic[t]) * fe_c[t] + (Ic[t] - I[3u * (c - st.si)]) * fe_w[t];
+                const double dn = (I[3u * (c + st.sj)] - Ic[t]) * fn_c[t] + (Ic[t] - I[3u * (c - st.sj)]) * fn_s[t];
+                const double g05 = 0.5 * p.grav / p.rho_0;
+                diss = g05 * de / (g.dxt[i] * g.cost[j]) + g05 * dn / (g.dyt[j] * g.cost[j]);
+            }
+            sTt[tcnz + lo] = diss;
         }
     }
     double K33k = 0.0;
-    if (IMPLICIT && interior) {
+    if (IMPLICIT && interior && wc) {
         // rows of the K33 system (veros/core/isoneutral/diffusion.py:154-164); the
         // edge row k = kbot-1 takes b_edge and a = 0 (veros/core/operators.py:72)
         K33k = a.K33[c];
@@ -596,7 +653,7 @@ __global__ __launch_bounds__(256) void iso_step_tile_kernel(Grid g, Prm p, StepA
 #pragma unroll
     for (int t = 0; t < NTR; ++t) {
         dtrk[t] = rhs[t] = 0.0;
-        if (interior) {
+        if (interior && wc) {
             double* sTt = sT + t * 4 * tcnz;
             const double ftm = (k >= 1) ? sTt[lo - 1] : 0.0;
             const double hor = (fe_c[t] - fe_w[t]) / (g.cost[j] * g.dxt[i]) +
@@ -646,7 +703,7 @@ __global__ __launch_bounds__(256) void iso_step_tile_kernel(Grid g, Prm p, StepA
     __syncthreads();
 
     // ---- phase 3
-    if (!valid) return;
+    if (!wc) return;
     double Pk = energy ? a.P[c] : 0.0;
 #pragma unroll
     for (int t = 0; t < NTR; ++t) {
