@@ -196,6 +196,49 @@ __device__ __forceinline__ CellIdx cell_of(int i0, int ni, int j0, int nj) {
 
 __device__ __forceinline__ double m01(int kb, int k) { return (kb > 0 && kb - 1 <= k) ? 1.0 : 0.0; }
 
+// Stencil columns of a cell block, staged in LDS. A block holds the columns
+// (i, jb .. jb+cpb-1) (cpb = blockDim.y); its stencil reaches the columns
+//   q = 0 .. cpb+1        : (i,   jb-1 .. jb+cpb)   south, own, north
+//   q = cpb+2 .. 2cpb+1   : (i-1, jb .. jb+cpb-1)   west
+//   q = 2cpb+2 .. 3cpb+1  : (i+1, jb .. jb+cpb-1)   east
+// Staging level `lev` of an (X, Y, nz, 3) field once per block replaces the
+// 13 strided loads per cell and field with LDS reads (each global line is
+// fetched once per block instead of once per stencil point). j is clamped
+// into the array: columns past the grid edge are staged but never read.
+__device__ __forceinline__ int stencil_columns(int cpb) { return 3 * cpb + 2; }
+
+__device__ __forceinline__ void stage_level(const Grid& g, const double* __restrict__ f, int lev, int i, int jb,
+                                            double* __restrict__ dst) {
+    // block = (nz levels) x (cpb columns): thread (k, ty) stages level k of
+    // the columns ty, ty + cpb, ...
+    const int nz = g.nz, cpb = (int)blockDim.y, k = (int)threadIdx.x;
+    const int ncol = stencil_columns(cpb);
+    for (int q = (int)threadIdx.y; q < ncol; q += cpb) {
+        int ii = i, jj;
+        if (q < cpb + 2) {
+            jj = jb - 1 + q;
+        } else if (q < 2 * cpb + 2) {
+            ii = i - 1;
+            jj = jb + q - cpb - 2;
+        } else {
+            ii = i + 1;
+            jj = jb + q - 2 * cpb - 2;
+        }
+        jj = jj < 0 ? 0 : (jj >= g.Y ? g.Y - 1 : jj);
+        dst[q * nz + k] = f[3u * (((uint32_t)ii * (uint32_t)g.Y + (uint32_t)jj) * (uint32_t)nz + (uint32_t)k) + (uint32_t)lev];
+    }
+}
+
+// Column pointers of one thread into a staged field.
+struct Cols {
+    const double *c, *e, *w, *n, *s;
+};
+__device__ __forceinline__ Cols cols_of(const double* staged, int nz) {
+    const int ty = (int)threadIdx.y, cpb = (int)blockDim.y;
+    return {staged + (1 + ty) * nz, staged + (2 * cpb + 2 + ty) * nz, staged + (cpb + 2 + ty) * nz,
+            staged + (2 + ty) * nz, staged + ty * nz};
+}
+
 // ---------------------------------------------------------------------------
 // isoneutral_diffusion_pre: thread per cell (i, j, k), i in [1, X-2), j in [1, Y-2).
 // The 13 tracer points of the stencil are loaded once per field; all face
@@ -465,20 +508,22 @@ struct Strides {
     uint32_t si, sj;
 };
 
+// The face fluxes read the tracer (level tau) from the block's staged
+// stencil columns `tc` and the isoneutral tensor from global memory.
 // flux_east at (i, j, k), 0 outside i in [1, X-3], j in [2, Y-3]   (:22-47)
-__device__ __forceinline__ double flux_east(const Grid& g, const StepArgs& a, const double* __restrict__ tr,
-                                            const Strides& st, int i, int j, int k) {
+__device__ __forceinline__ double flux_east(const Grid& g, const StepArgs& a, const Cols& tc, const Strides& st,
+                                            int i, int j, int k) {
     if (i < 1 || i > g.X - 3 || j < 2 || j > g.Y - 3) return 0.0;
     const uint32_t c = (uint32_t)i * st.si + (uint32_t)j * st.sj + (uint32_t)k;
     const uint32_t ce = c + st.si;
     const bool hm = k >= 1;
-    const uint32_t up = (k + 1 < g.nz) ? 1u : 0u, dn = hm ? 1u : 0u;  // pad_z_edges clamping
+    const int up = (k + 1 < g.nz) ? 1 : 0, dn = hm ? 1 : 0;  // pad_z_edges clamping
     const double diffloc = hm ? 0.25 * (k1_at(a, c) + k1_at(a, c - 1) + k1_at(a, ce) + k1_at(a, ce - 1))
                               : 0.5 * (k1_at(a, c) + k1_at(a, ce));
     const double2 A01 = *reinterpret_cast<const double2*>(a.Aez + 4u * c);      // ip=0: kr=0, 1
     const double2 A23 = *reinterpret_cast<const double2*>(a.Aez + 4u * c + 2);  // ip=1: kr=0, 1
-    const double t0 = tr[3u * c], t0u = tr[3u * (c + up)], t0d = tr[3u * (c - dn)];
-    const double t1 = tr[3u * ce], t1u = tr[3u * (ce + up)], t1d = tr[3u * (ce - dn)];
+    const double t0 = tc.c[k], t0u = tc.c[k + up], t0d = tc.c[k - dn];
+    const double t1 = tc.e[k], t1u = tc.e[k + up], t1d = tc.e[k - dn];
     double sumz = 0.0;
     sumz = sumz + diffloc * A01.x * (t0 - t0d);  // kr=0, ip=0
     sumz = sumz + diffloc * A23.x * (t1 - t1d);  // kr=0, ip=1
@@ -488,19 +533,19 @@ __device__ __forceinline__ double flux_east(const Grid& g, const StepArgs& a, co
 }
 
 // flux_north at (i, j, k), 0 outside i in [2, X-3], j in [1, Y-3]   (:49-77)
-__device__ __forceinline__ double flux_north(const Grid& g, const StepArgs& a, const double* __restrict__ tr,
-                                             const Strides& st, int i, int j, int k) {
+__device__ __forceinline__ double flux_north(const Grid& g, const StepArgs& a, const Cols& tc, const Strides& st,
+                                             int i, int j, int k) {
     if (i < 2 || i > g.X - 3 || j < 1 || j > g.Y - 3) return 0.0;
     const uint32_t c = (uint32_t)i * st.si + (uint32_t)j * st.sj + (uint32_t)k;
     const uint32_t cn = c + st.sj;
     const bool hm = k >= 1;
-    const uint32_t up = (k + 1 < g.nz) ? 1u : 0u, dn = hm ? 1u : 0u;
+    const int up = (k + 1 < g.nz) ? 1 : 0, dn = hm ? 1 : 0;
     const double diffloc = hm ? 0.25 * (k1_at(a, c) + k1_at(a, c - 1) + k1_at(a, cn) + k1_at(a, cn - 1))
                               : 0.5 * (k1_at(a, c) + k1_at(a, cn));
     const double2 A01 = *reinterpret_cast<const double2*>(a.Anz + 4u * c);
     const double2 A23 = *reinterpret_cast<const double2*>(a.Anz + 4u * c + 2);
-    const double t0 = tr[3u * c], t0u = tr[3u * (c + up)], t0d = tr[3u * (c - dn)];
-    const double t1 = tr[3u * cn], t1u = tr[3u * (cn + up)], t1d = tr[3u * (cn - dn)];
+    const double t0 = tc.c[k], t0u = tc.c[k + up], t0d = tc.c[k - dn];
+    const double t1 = tc.n[k], t1u = tc.n[k + up], t1d = tc.n[k - dn];
     double sumz = 0.0;
     sumz = sumz + diffloc * A01.x * (t0 - t0d);
     sumz = sumz + diffloc * A23.x * (t1 - t1d);
@@ -510,20 +555,19 @@ __device__ __forceinline__ double flux_north(const Grid& g, const StepArgs& a, c
 }
 
 // flux_top at an interior (i, j, k), k < nz-1   (:79-111)
-__device__ __forceinline__ double flux_top(const Grid& g, const StepArgs& a, const double* __restrict__ tr,
-                                           const Strides& st, int i, int j, int k) {
+__device__ __forceinline__ double flux_top(const Grid& g, const StepArgs& a, const Cols& tc, const Strides& st,
+                                           int i, int j, int k) {
     const uint32_t c = (uint32_t)i * st.si + (uint32_t)j * st.sj + (uint32_t)k;
-    const uint32_t cw = c - st.si, ce = c + st.si, cs = c - st.sj, cn = c + st.sj;
     const double diffloc = k2_at(a, c);
     const double2 B01 = *reinterpret_cast<const double2*>(a.Abx + 4u * c);      // ip=0: kr=0, 1
     const double2 B23 = *reinterpret_cast<const double2*>(a.Abx + 4u * c + 2);  // ip=1: kr=0, 1
     const double2 C01 = *reinterpret_cast<const double2*>(a.Aby + 4u * c);
     const double2 C23 = *reinterpret_cast<const double2*>(a.Aby + 4u * c + 2);
-    const double tc0 = tr[3u * c], tc1 = tr[3u * (c + 1)];
-    const double tw0 = tr[3u * cw], tw1 = tr[3u * (cw + 1)];
-    const double te0 = tr[3u * ce], te1 = tr[3u * (ce + 1)];
-    const double ts0 = tr[3u * cs], ts1 = tr[3u * (cs + 1)];
-    const double tn0 = tr[3u * cn], tn1 = tr[3u * (cn + 1)];
+    const double tc0 = tc.c[k], tc1 = tc.c[k + 1];
+    const double tw0 = tc.w[k], tw1 = tc.w[k + 1];
+    const double te0 = tc.e[k], te1 = tc.e[k + 1];
+    const double ts0 = tc.s[k], ts1 = tc.s[k + 1];
+    const double tn0 = tc.n[k], tn1 = tc.n[k + 1];
     double sumx = 0.0;
     sumx = sumx + diffloc * B01.x / g.cost[j] * (tc0 - tw0);  // ip=0, kr=0
     sumx = sumx + diffloc * B01.y / g.cost[j] * (tc1 - tw1);  // ip=0, kr=1
@@ -546,22 +590,30 @@ __device__ __forceinline__ double flux_top(const Grid& g, const StepArgs& a, con
 // written; the tile kernel reads only faces that touch a wet cell.
 template <int NTR>
 __global__ __launch_bounds__(256) void iso_flux_kernel(Grid g, StepArgs a) {
+    extern __shared__ double lds[];
     const CellIdx ci = cell_of(1, g.X - 3, 1, g.Y - 3);
+    const int jb = ci.j - (int)threadIdx.y;  // the block's first column
+    if (jb > g.Y - 3) return;                 // padding block (uniform)
+    const int nz = g.nz;
+    const int per = stencil_columns((int)blockDim.y) * nz;
+#pragma unroll
+    for (int t = 0; t < NTR; ++t) stage_level(g, a.t[t].tr, a.tau, ci.i, jb, lds + t * per);
+    __syncthreads();
     if (!ci.valid) return;
     const int i = ci.i, j = ci.j, k = ci.k;
-    const Strides st{(uint32_t)g.Y * (uint32_t)g.nz, (uint32_t)g.nz};
+    const Strides st{(uint32_t)g.Y * (uint32_t)nz, (uint32_t)nz};
     const uint32_t c = (uint32_t)i * st.si + (uint32_t)j * st.sj + (uint32_t)k;
     const int kb = g.kbot[i * g.Y + j];
     const bool wc = wet(kb, k);
     const bool need_e = j >= 2 && (wc || wet(g.kbot[(i + 1) * g.Y + j], k));
     const bool need_n = i >= 2 && (wc || wet(g.kbot[i * g.Y + j + 1], k));
-    const bool need_t = i >= 2 && j >= 2 && (wc || (k + 1 < g.nz && wet(kb, k + 1)));
+    const bool need_t = i >= 2 && j >= 2 && (wc || (k + 1 < nz && wet(kb, k + 1)));
 #pragma unroll
     for (int t = 0; t < NTR; ++t) {
-        const double* __restrict__ tr = a.t[t].tr + a.tau;
-        if (need_e) a.t[t].fe[c] = flux_east(g, a, tr, st, i, j, k);
-        if (need_n) a.t[t].fn[c] = flux_north(g, a, tr, st, i, j, k);
-        if (need_t) a.t[t].ft[c] = (k < g.nz - 1) ? flux_top(g, a, tr, st, i, j, k) : 0.0;
+        const Cols tc = cols_of(lds + t * per, nz);
+        if (need_e) a.t[t].fe[c] = flux_east(g, a, tc, st, i, j, k);
+        if (need_n) a.t[t].fn[c] = flux_north(g, a, tc, st, i, j, k);
+        if (need_t) a.t[t].ft[c] = (k < nz - 1) ? flux_top(g, a, tc, st, i, j, k) : 0.0;
     }
 }
 
@@ -604,42 +656,65 @@ __global__ __launch_bounds__(256) void iso_step_tile_kernel(Grid g, Prm p, StepA
     const bool wc = valid && wet(kb, k);
     const bool wu = valid && k + 1 < nz && wet(kb, k + 1);
 
-    // ---- phase 1a: fluxes from the workspace, lateral dissipation, K33 rows
-    double fe_c[NTR], fe_w[NTR], fn_c[NTR], fn_s[NTR], ft_c[NTR], Ic[NTR];
+    // ---- every global read of the step first: the block then waits on memory
+    // once instead of once per phase
+    double fe_c[NTR], fe_w[NTR], fn_c[NTR], fn_s[NTR], ft_c[NTR];
+    double Ic[NTR], Ie[NTR], Iw[NTR], In[NTR], Is[NTR], Iu[NTR], trp[NTR], dtr0[NTR];
 #pragma unroll
     for (int t = 0; t < NTR; ++t) {
         const TracerArgs& ta = a.t[t];
-        fe_c[t] = fe_w[t] = fn_c[t] = fn_s[t] = ft_c[t] = Ic[t] = 0.0;
+        fe_c[t] = fe_w[t] = fn_c[t] = fn_s[t] = ft_c[t] = 0.0;
+        Ic[t] = Ie[t] = Iw[t] = In[t] = Is[t] = Iu[t] = trp[t] = dtr0[t] = 0.0;
         if (wc) {
             fe_c[t] = fe_at(g, ta, i, j, c);
             fe_w[t] = fe_at(g, ta, i - 1, j, c - st.si);
             fn_c[t] = fn_at(g, ta, i, j, c);
             fn_s[t] = fn_at(g, ta, i, j - 1, c - st.sj);
+            if (interior) {
+                trp[t] = ta.tr[3u * c + a.taup1];
+                dtr0[t] = ta.dtr[c];
+            }
+            if (energy) {
+                const double* __restrict__ I = ta.I + a.tau;
+                Ic[t] = I[3u * c];
+                Ie[t] = I[3u * (c + st.si)];
+                Iw[t] = I[3u * (c - st.si)];
+                In[t] = I[3u * (c + st.sj)];
+                Is[t] = I[3u * (c - st.sj)];
+                if (interior && k < nz - 1) Iu[t] = I[3u * (c + 1)];
+            }
         }
         if (interior && (wc || wu)) ft_c[t] = ta.ft[c];
+    }
+    double K33k = 0.0, K33m = 0.0;
+    if (IMPLICIT && interior && wc) {
+        K33k = a.K33[c];
+        if (k >= 1) K33m = a.K33[c - 1];
+    }
+    const double P0 = (energy && wc) ? a.P[c] : 0.0;
+
+    // ---- phase 1a: flux_top and lateral dissipation -> LDS, K33 rows -> LDS
+#pragma unroll
+    for (int t = 0; t < NTR; ++t) {
         double* sTt = sT + t * 4 * tcnz;
         sTt[lo] = ft_c[t];
         if (valid && energy) {
             double diss = 0.0;
             if (wc) {
                 // compute_dissipation (veros/core/diffusion.py:13-33)
-                const double* __restrict__ I = ta.I + a.tau;
-                Ic[t] = I[3u * c];
-                const double de = (I[3u * (c + st.si)] - Ic[t]) * fe_c[t] + (Ic[t] - I[3u * (c - st.si)]) * fe_w[t];
-                const double dn = (I[3u * (c + st.sj)] - Ic[t]) * fn_c[t] + (Ic[t] - I[3u * (c - st.sj)]) * fn_s[t];
+                const double de = (Ie[t] - Ic[t]) * fe_c[t] + (Ic[t] - Iw[t]) * fe_w[t];
+                const double dn = (In[t] - Ic[t]) * fn_c[t] + (Ic[t] - Is[t]) * fn_s[t];
                 const double g05 = 0.5 * p.grav / p.rho_0;
                 diss = g05 * de / (g.dxt[i] * g.cost[j]) + g05 * dn / (g.dyt[j] * g.cost[j]);
             }
             sTt[tcnz + lo] = diss;
         }
     }
-    double K33k = 0.0;
     if (IMPLICIT && interior && wc) {
         // rows of the K33 system (veros/core/isoneutral/diffusion.py:154-164); the
         // edge row k = kbot-1 takes b_edge and a = 0 (veros/core/operators.py:72)
-        K33k = a.K33[c];
         const double delta = (k < nz - 1) ? dt / g.dzw[k] * K33k : 0.0;
-        const double delta_m = (k >= 1) ? dt / g.dzw[k - 1] * a.K33[c - 1] : 0.0;
+        const double delta_m = (k >= 1) ? dt / g.dzw[k - 1] * K33m : 0.0;
         const double dz = g.dzt[k];
         const bool edge = k == kb - 1;
         sA[lo] = edge ? 0.0 : -delta_m / dz;
@@ -660,7 +735,7 @@ __global__ __launch_bounds__(256) void iso_step_tile_kernel(Grid g, Prm p, StepA
                                (fn_c[t] - fn_s[t]) / (g.cost[j] * g.dyt[j]);
             const double vert = (k == 0) ? ft_c[t] / g.dzt[0] : (ft_c[t] - ftm) / g.dzt[k];
             dtrk[t] = mt * hor + mt * vert;
-            rhs[t] = a.t[t].tr[3u * c + a.taup1] + dt * dtrk[t];
+            rhs[t] = trp[t] + dt * dtrk[t];
             sTt[2 * tcnz + lo] = rhs[t];
         }
     }
@@ -704,14 +779,14 @@ __global__ __launch_bounds__(256) void iso_step_tile_kernel(Grid g, Prm p, StepA
 
     // ---- phase 3
     if (!wc) return;
-    double Pk = energy ? a.P[c] : 0.0;
+    double Pk = P0;
 #pragma unroll
     for (int t = 0; t < NTR; ++t) {
         const double* sTt = sT + t * 4 * tcnz;
         if (interior) {
             const double newk = sTt[2 * tcnz + lo];
             a.t[t].tr[3u * c + a.taup1] = newk;
-            a.t[t].dtr[c] = (a.t[t].dtr[c] + dtrk[t]) + (newk - rhs[t]) / dt;
+            a.t[t].dtr[c] = (dtr0[t] + dtrk[t]) + (newk - rhs[t]) / dt;
         }
         if (energy) {
             // dissipation_on_wgrid (veros/core/diffusion.py:39-62)
@@ -732,7 +807,7 @@ __global__ __launch_bounds__(256) void iso_step_tile_kernel(Grid g, Prm p, StepA
             // vertical mixing term (veros/core/isoneutral/diffusion.py:254-279)
             if (interior && k < nz - 1) {
                 const double mw = m01(kb, k) * m01(kb, k + 1);
-                const double fxa = (-a.t[t].I[3u * (c + 1) + a.tau] + Ic[t]) / g.dzw[k];
+                const double fxa = (-Iu[t] + Ic[t]) / g.dzw[k];
                 const double gr = -p.grav / p.rho_0;
                 if constexpr (IMPLICIT) {
                     const double dtrz = sTt[2 * tcnz + lo + 1] - sTt[2 * tcnz + lo];
@@ -1059,7 +1134,8 @@ int launch_step(const Grid& g, const Prm& p, const StepArgs& a, hipStream_t s, c
         return fail(VX_ERR_UNSUPPORTED, "%s: grid too large for 32-bit offsets", who);
     if (g.nz * TC > 256) return fail(VX_ERR_UNSUPPORTED, "%s: nz=%d too deep", who, g.nz);
     const dim3 fblk = cell_block(g.nz);
-    hipLaunchKernelGGL((iso_flux_kernel<NTR>), cell_grid(g.X - 3, g.Y - 3, fblk), fblk, 0, s, g, a);
+    const size_t flux_lds = NTR * (size_t)(3 * fblk.y + 2) * g.nz * sizeof(double);
+    hipLaunchKernelGGL((iso_flux_kernel<NTR>), cell_grid(g.X - 3, g.Y - 3, fblk), fblk, flux_lds, s, g, a);
     int rc = check_launch(who);
     if (rc) return rc;
     const dim3 blk(g.nz, TC);
