@@ -321,11 +321,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_PRE_MIN_
     L2(cs, true, Ts, Ss);
     L2(cs + 1, hp, Tsp, Ssp);
 
+    // Every other global read up front: reads placed inside the divergent
+    // face branches below would each cost a separate memory round trip.
+    const double dzw_k = g.dzw[k], dzw_m = hm ? g.dzw[k - 1] : 0.0, dzt_k = g.dzt[k];
+    const double dxu_i = g.dxu[i], dxu_w = g.dxu[i - 1], dxt_i = g.dxt[i];
+    const double cost_j = g.cost[j], dyu_j = g.dyu[j], dyu_s = g.dyu[j - 1], dyt_j = g.dyt[j];
+    const double cosu_j = g.cosu[j], cosu_s = g.cosu[j - 1];
+    const double Kc = K[c], Kcm = hm ? K[c - 1] : 0.0, Ke = K[ce], Kem = hm ? K[ce - 1] : 0.0;
+    const double Kn = K[cn], Knm = hm ? K[cn - 1] : 0.0;
+
     // reciprocal metric factors
-    const double rz_c = hp ? rcp_nr(g.dzw[k]) : 0.0;
-    const double rz_m = hm ? rcp_nr(g.dzw[k - 1]) : 0.0;
-    const double rx_c = rcp_nr(g.dxu[i] * g.cost[j]);
-    const double ry_c = rcp_nr(g.dyu[j]);
+    const double rz_c = hp ? rcp_nr(dzw_k) : 0.0;
+    const double rz_m = hm ? rcp_nr(dzw_m) : 0.0;
+    const double rx_c = rcp_nr(dxu_i * cost_j);
+    const double ry_c = rcp_nr(dyu_j);
 
     // masked face gradients (veros/core/isoneutral/isoneutral.py:46-95)
     auto G = [](double m, double a1, double a0, double b1, double b0, double rd) -> Grad {
@@ -346,7 +355,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_PRE_MIN_
         const Grad gx = G(mU, Te, Tc, Se, Sc, rx_c);
         const Grad gz_e = hp ? G(me * mep, Tep, Te, Sep, Se, rz_c) : Grad{0.0, 0.0};
         const Grad gz_em = hm ? G(mem * me, Te, Tem, Se, Sem, rz_m) : Grad{0.0, 0.0};
-        const double diffloc = hm ? 0.25 * (K[c] + K[c - 1] + K[ce] + K[ce - 1]) : 0.5 * (K[c] + K[ce]);
+        const double diffloc = hm ? 0.25 * (Kc + Kcm + Ke + Kem) : 0.5 * (Kc + Ke);
         double sumz = 0.0;
         double out[4];
 #pragma unroll
@@ -359,7 +368,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_PRE_MIN_
                 const Grad& gz = ip ? (kr ? gz_e : gz_em) : (kr ? gz_c : gz_m);
                 const double sl = -dot(w, gx) * rden(dot(w, gz));
                 const double tp = taper(p, ec, tds, sl);
-                sumz += g.dzw[k - 1 + kr] * mU * fmax(p.K_iso_steep, diffloc * tp);
+                sumz += (kr ? dzw_k : dzw_m) * mU * fmax(p.K_iso_steep, diffloc * tp);
                 out[ip * 2 + kr] = tp * sl * mU;
             }
         }
@@ -373,7 +382,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_PRE_MIN_
         } else {
             Aez[4u * c + 3] = out[3];
         }
-        K11[c] = sumz / (4.0 * g.dzt[k]);
+        K11[c] = sumz / (4.0 * dzt_k);
     }
 
     // --- north face (i, j+1/2, k), i in [2, X-2)   (:134-168)
@@ -382,7 +391,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_PRE_MIN_
         const Grad gy = G(mV, Tn, Tc, Sn, Sc, ry_c);
         const Grad gz_n = hp ? G(mn * mnp, Tnp, Tn, Snp, Sn, rz_c) : Grad{0.0, 0.0};
         const Grad gz_nm = hm ? G(mnm * mn, Tn, Tnm, Sn, Snm, rz_m) : Grad{0.0, 0.0};
-        const double diffloc = hm ? 0.25 * (K[c] + K[c - 1] + K[cn] + K[cn - 1]) : 0.5 * (K[c] + K[cn]);
+        const double diffloc = hm ? 0.25 * (Kc + Kcm + Kn + Knm) : 0.5 * (Kc + Kn);
         double sumz = 0.0;
         double out[4];
 #pragma unroll
@@ -395,7 +404,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_PRE_MIN_
                 const Grad& gz = jp ? (kr ? gz_n : gz_nm) : (kr ? gz_c : gz_m);
                 const double sl = -dot(w, gy) * rden(dot(w, gz));
                 const double tp = taper(p, ec, tds, sl);
-                sumz += g.dzw[k - 1 + kr] * mV * fmax(p.K_iso_steep, diffloc * tp);
+                sumz += (kr ? dzw_k : dzw_m) * mV * fmax(p.K_iso_steep, diffloc * tp);
                 out[jp * 2 + kr] = tp * sl * mV;
             }
         }
@@ -406,16 +415,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_PRE_MIN_
             Anz[4u * c + 1] = out[1];
             Anz[4u * c + 3] = out[3];
         }
-        K22[c] = sumz / (4.0 * g.dzt[k]);
+        K22[c] = sumz / (4.0 * dzt_k);
     }
 
     // --- top face (i, j, k+1/2), i, j in [2, X/Y-2), k < nz-1   (:170-225)
     if (i >= 2 && j >= 2 && hp) {
         const int kb_w2 = kb_w, kb_s2 = kb_s;
         const double mW = mc * mcp;
-        const double Kc = K[c];
-        const double rx_w = rcp_nr(g.dxu[i - 1] * g.cost[j]);
-        const double ry_s = rcp_nr(g.dyu[j - 1]);
+        const double rx_w = rcp_nr(dxu_w * cost_j);
+        const double ry_s = rcp_nr(dyu_s);
         const double mw = m01(kb_w2, k), mwp = m01(kb_w2, k + 1);
         const double ms = m01(kb_s2, k), msp = m01(kb_s2, k + 1);
         // horizontal gradients at (i-1+ip, k+kr) and (j-1+jp, k+kr)
@@ -438,7 +446,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_PRE_MIN_
                 const Grad& gx = ip ? (kr ? gx_cp : gx_c) : (kr ? gx_wp : gx_w);
                 const double sl = -dot(w, gx) * rd;
                 const double tp = taper(p, ec, tds, sl);
-                sumx += g.dxu[i - 1 + ip] * Kc * tp * (sl * sl) * mW;
+                sumx += (ip ? dxu_i : dxu_w) * Kc * tp * (sl * sl) * mW;
                 ox[ip * 2 + kr] = tp * sl * mW;
             }
 #pragma unroll
@@ -446,7 +454,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_PRE_MIN_
                 const Grad& gy = jp ? (kr ? gy_cp : gy_c) : (kr ? gy_sp : gy_s);
                 const double sl = -dot(w, gy) * rd;
                 const double tp = taper(p, ec, tds, sl);
-                const double facty = g.cosu[j - 1 + jp] * g.dyu[j - 1 + jp];
+                const double facty = jp ? cosu_j * dyu_j : cosu_s * dyu_s;
                 sumy += facty * Kc * tp * (sl * sl) * mW;
                 oy[jp * 2 + kr] = tp * sl * mW;
             }
@@ -455,7 +463,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_PRE_MIN_
         *reinterpret_cast<double2*>(Abx + 4u * c + 2) = make_double2(ox[2], ox[3]);
         *reinterpret_cast<double2*>(Aby + 4u * c) = make_double2(oy[0], oy[1]);
         *reinterpret_cast<double2*>(Aby + 4u * c + 2) = make_double2(oy[2], oy[3]);
-        K33[c] = sumx / (4 * g.dxt[i]) + sumy / (4 * g.dyt[j] * g.cost[j]);
+        K33[c] = sumx / (4 * dxt_i) + sumy / (4 * dyt_j * cost_j);
     }
 }
 
@@ -495,90 +503,84 @@ struct StepArgs {
     double* P;            // P_diss_iso / P_diss_skew, nullable (energy branch off)
 };
 
-__device__ __forceinline__ double k1_at(const StepArgs& a, uint32_t o) {
-    const double ki = a.Kiso ? a.Kiso[o] : 0.0;
-    return a.Kskew ? ki - a.Kskew[o] : ki;
-}
-__device__ __forceinline__ double k2_at(const StepArgs& a, uint32_t o) {
-    const double ki = a.Kiso ? a.Kiso[o] : 0.0;
-    return a.Kskew ? ki + a.Kskew[o] : ki;
-}
-
 struct Strides {
     uint32_t si, sj;
 };
 
+// Inputs of the face fluxes of one cell (i, j, k), read from global memory
+// before any flux is formed. `K` is the one diffusivity of the step (K_iso for
+// the isoneutral step, K_gm for the skew step) at c, c-1, the east column
+// (e, e-1) and the north column (n, n-1).
+struct FaceIn {
+    double K_c, K_cm, K_e, K_em, K_n, K_nm;
+    double2 ez01, ez23, nz01, nz23, bx01, bx23, by01, by23;
+    double K11, K22;
+    double dzt, cost, dxu, cosu, cosu_s, dyu, dxt, dyt;
+};
+
+// K1 = K_iso - K_skew (east/north faces) and K2 = K_iso + K_skew (top faces)
+// (veros/core/isoneutral/diffusion.py:15-16,180-188); one of the two is zero.
+template <bool ISO>
+__device__ __forceinline__ double k1_of(double K) { return ISO ? K : 0.0 - K; }
+template <bool ISO>
+__device__ __forceinline__ double k2_of(double K) { return ISO ? K : 0.0 + K; }
+
 // The face fluxes read the tracer (level tau) from the block's staged
-// stencil columns `tc` and the isoneutral tensor from global memory.
-// flux_east at (i, j, k), 0 outside i in [1, X-3], j in [2, Y-3]   (:22-47)
-__device__ __forceinline__ double flux_east(const Grid& g, const StepArgs& a, const Cols& tc, const Strides& st,
-                                            int i, int j, int k) {
-    if (i < 1 || i > g.X - 3 || j < 2 || j > g.Y - 3) return 0.0;
-    const uint32_t c = (uint32_t)i * st.si + (uint32_t)j * st.sj + (uint32_t)k;
-    const uint32_t ce = c + st.si;
+// stencil columns `tc`.
+// flux_east at (i, j, k), i in [1, X-3], j in [2, Y-3]   (:22-47)
+template <bool ISO>
+__device__ __forceinline__ double flux_east(const FaceIn& in, const Cols& tc, int k, int nz) {
     const bool hm = k >= 1;
-    const int up = (k + 1 < g.nz) ? 1 : 0, dn = hm ? 1 : 0;  // pad_z_edges clamping
-    const double diffloc = hm ? 0.25 * (k1_at(a, c) + k1_at(a, c - 1) + k1_at(a, ce) + k1_at(a, ce - 1))
-                              : 0.5 * (k1_at(a, c) + k1_at(a, ce));
-    const double2 A01 = *reinterpret_cast<const double2*>(a.Aez + 4u * c);      // ip=0: kr=0, 1
-    const double2 A23 = *reinterpret_cast<const double2*>(a.Aez + 4u * c + 2);  // ip=1: kr=0, 1
+    const int up = (k + 1 < nz) ? 1 : 0, dn = hm ? 1 : 0;  // pad_z_edges clamping
+    const double diffloc = hm ? 0.25 * (k1_of<ISO>(in.K_c) + k1_of<ISO>(in.K_cm) + k1_of<ISO>(in.K_e) + k1_of<ISO>(in.K_em))
+                              : 0.5 * (k1_of<ISO>(in.K_c) + k1_of<ISO>(in.K_e));
     const double t0 = tc.c[k], t0u = tc.c[k + up], t0d = tc.c[k - dn];
     const double t1 = tc.e[k], t1u = tc.e[k + up], t1d = tc.e[k - dn];
     double sumz = 0.0;
-    sumz = sumz + diffloc * A01.x * (t0 - t0d);  // kr=0, ip=0
-    sumz = sumz + diffloc * A23.x * (t1 - t1d);  // kr=0, ip=1
-    sumz = sumz + diffloc * A01.y * (t0u - t0);  // kr=1, ip=0
-    sumz = sumz + diffloc * A23.y * (t1u - t1);  // kr=1, ip=1
-    return sumz / (4.0 * g.dzt[k]) + (t1 - t0) / (g.cost[j] * g.dxu[i]) * a.K11[c];
+    sumz = sumz + diffloc * in.ez01.x * (t0 - t0d);  // kr=0, ip=0
+    sumz = sumz + diffloc * in.ez23.x * (t1 - t1d);  // kr=0, ip=1
+    sumz = sumz + diffloc * in.ez01.y * (t0u - t0);  // kr=1, ip=0
+    sumz = sumz + diffloc * in.ez23.y * (t1u - t1);  // kr=1, ip=1
+    return sumz / (4.0 * in.dzt) + (t1 - t0) / (in.cost * in.dxu) * in.K11;
 }
 
-// flux_north at (i, j, k), 0 outside i in [2, X-3], j in [1, Y-3]   (:49-77)
-__device__ __forceinline__ double flux_north(const Grid& g, const StepArgs& a, const Cols& tc, const Strides& st,
-                                             int i, int j, int k) {
-    if (i < 2 || i > g.X - 3 || j < 1 || j > g.Y - 3) return 0.0;
-    const uint32_t c = (uint32_t)i * st.si + (uint32_t)j * st.sj + (uint32_t)k;
-    const uint32_t cn = c + st.sj;
+// flux_north at (i, j, k), i in [2, X-3], j in [1, Y-3]   (:49-77)
+template <bool ISO>
+__device__ __forceinline__ double flux_north(const FaceIn& in, const Cols& tc, int k, int nz) {
     const bool hm = k >= 1;
-    const int up = (k + 1 < g.nz) ? 1 : 0, dn = hm ? 1 : 0;
-    const double diffloc = hm ? 0.25 * (k1_at(a, c) + k1_at(a, c - 1) + k1_at(a, cn) + k1_at(a, cn - 1))
-                              : 0.5 * (k1_at(a, c) + k1_at(a, cn));
-    const double2 A01 = *reinterpret_cast<const double2*>(a.Anz + 4u * c);
-    const double2 A23 = *reinterpret_cast<const double2*>(a.Anz + 4u * c + 2);
+    const int up = (k + 1 < nz) ? 1 : 0, dn = hm ? 1 : 0;
+    const double diffloc = hm ? 0.25 * (k1_of<ISO>(in.K_c) + k1_of<ISO>(in.K_cm) + k1_of<ISO>(in.K_n) + k1_of<ISO>(in.K_nm))
+                              : 0.5 * (k1_of<ISO>(in.K_c) + k1_of<ISO>(in.K_n));
     const double t0 = tc.c[k], t0u = tc.c[k + up], t0d = tc.c[k - dn];
     const double t1 = tc.n[k], t1u = tc.n[k + up], t1d = tc.n[k - dn];
     double sumz = 0.0;
-    sumz = sumz + diffloc * A01.x * (t0 - t0d);
-    sumz = sumz + diffloc * A23.x * (t1 - t1d);
-    sumz = sumz + diffloc * A01.y * (t0u - t0);
-    sumz = sumz + diffloc * A23.y * (t1u - t1);
-    return g.cosu[j] * (sumz / (4.0 * g.dzt[k]) + (t1 - t0) / g.dyu[j] * a.K22[c]);
+    sumz = sumz + diffloc * in.nz01.x * (t0 - t0d);
+    sumz = sumz + diffloc * in.nz23.x * (t1 - t1d);
+    sumz = sumz + diffloc * in.nz01.y * (t0u - t0);
+    sumz = sumz + diffloc * in.nz23.y * (t1u - t1);
+    return in.cosu * (sumz / (4.0 * in.dzt) + (t1 - t0) / in.dyu * in.K22);
 }
 
 // flux_top at an interior (i, j, k), k < nz-1   (:79-111)
-__device__ __forceinline__ double flux_top(const Grid& g, const StepArgs& a, const Cols& tc, const Strides& st,
-                                           int i, int j, int k) {
-    const uint32_t c = (uint32_t)i * st.si + (uint32_t)j * st.sj + (uint32_t)k;
-    const double diffloc = k2_at(a, c);
-    const double2 B01 = *reinterpret_cast<const double2*>(a.Abx + 4u * c);      // ip=0: kr=0, 1
-    const double2 B23 = *reinterpret_cast<const double2*>(a.Abx + 4u * c + 2);  // ip=1: kr=0, 1
-    const double2 C01 = *reinterpret_cast<const double2*>(a.Aby + 4u * c);
-    const double2 C23 = *reinterpret_cast<const double2*>(a.Aby + 4u * c + 2);
+template <bool ISO>
+__device__ __forceinline__ double flux_top(const FaceIn& in, const Cols& tc, int k) {
+    const double diffloc = k2_of<ISO>(in.K_c);
     const double tc0 = tc.c[k], tc1 = tc.c[k + 1];
     const double tw0 = tc.w[k], tw1 = tc.w[k + 1];
     const double te0 = tc.e[k], te1 = tc.e[k + 1];
     const double ts0 = tc.s[k], ts1 = tc.s[k + 1];
     const double tn0 = tc.n[k], tn1 = tc.n[k + 1];
     double sumx = 0.0;
-    sumx = sumx + diffloc * B01.x / g.cost[j] * (tc0 - tw0);  // ip=0, kr=0
-    sumx = sumx + diffloc * B01.y / g.cost[j] * (tc1 - tw1);  // ip=0, kr=1
-    sumx = sumx + diffloc * B23.x / g.cost[j] * (te0 - tc0);  // ip=1, kr=0
-    sumx = sumx + diffloc * B23.y / g.cost[j] * (te1 - tc1);  // ip=1, kr=1
+    sumx = sumx + diffloc * in.bx01.x / in.cost * (tc0 - tw0);  // ip=0, kr=0
+    sumx = sumx + diffloc * in.bx01.y / in.cost * (tc1 - tw1);  // ip=0, kr=1
+    sumx = sumx + diffloc * in.bx23.x / in.cost * (te0 - tc0);  // ip=1, kr=0
+    sumx = sumx + diffloc * in.bx23.y / in.cost * (te1 - tc1);  // ip=1, kr=1
     double sumy = 0.0;
-    sumy = sumy + diffloc * C01.x * g.cosu[j - 1] * (tc0 - ts0);
-    sumy = sumy + diffloc * C01.y * g.cosu[j - 1] * (tc1 - ts1);
-    sumy = sumy + diffloc * C23.x * g.cosu[j] * (tn0 - tc0);
-    sumy = sumy + diffloc * C23.y * g.cosu[j] * (tn1 - tc1);
-    return sumx / (4 * g.dxt[i]) + sumy / (4 * g.dyt[j] * g.cost[j]);
+    sumy = sumy + diffloc * in.by01.x * in.cosu_s * (tc0 - ts0);
+    sumy = sumy + diffloc * in.by01.y * in.cosu_s * (tc1 - ts1);
+    sumy = sumy + diffloc * in.by23.x * in.cosu * (tn0 - tc0);
+    sumy = sumy + diffloc * in.by23.y * in.cosu * (tn1 - tc1);
+    return sumx / (4 * in.dxt) + sumy / (4 * in.dyt * in.cost);
 }
 
 // Fluxes of NTR tracers, thread per cell (i, j) in [1, X-2) x [1, Y-2):
@@ -588,32 +590,74 @@ __device__ __forceinline__ double flux_top(const Grid& g, const StepArgs& a, con
 // face (the divergence, the lateral dissipation and the vertical mixing term
 // of that cell), so a face between two dry cells is neither computed nor
 // written; the tile kernel reads only faces that touch a wet cell.
-template <int NTR>
+template <int NTR, bool ISO>
 __global__ __launch_bounds__(256) void iso_flux_kernel(Grid g, StepArgs a) {
     extern __shared__ double lds[];
     const CellIdx ci = cell_of(1, g.X - 3, 1, g.Y - 3);
     const int jb = ci.j - (int)threadIdx.y;  // the block's first column
     if (jb > g.Y - 3) return;                 // padding block (uniform)
     const int nz = g.nz;
+    const int i = ci.i, j = ci.j, k = ci.k;
+    int kb = 0, kb_e = 0, kb_n = 0;
+    if (ci.valid) {
+        kb = g.kbot[i * g.Y + j];
+        kb_e = g.kbot[(i + 1) * g.Y + j];
+        kb_n = g.kbot[i * g.Y + j + 1];
+    }
     const int per = stencil_columns((int)blockDim.y) * nz;
 #pragma unroll
-    for (int t = 0; t < NTR; ++t) stage_level(g, a.t[t].tr, a.tau, ci.i, jb, lds + t * per);
+    for (int t = 0; t < NTR; ++t) stage_level(g, a.t[t].tr, a.tau, i, jb, lds + t * per);
     __syncthreads();
     if (!ci.valid) return;
-    const int i = ci.i, j = ci.j, k = ci.k;
-    const Strides st{(uint32_t)g.Y * (uint32_t)nz, (uint32_t)nz};
-    const uint32_t c = (uint32_t)i * st.si + (uint32_t)j * st.sj + (uint32_t)k;
-    const int kb = g.kbot[i * g.Y + j];
     const bool wc = wet(kb, k);
-    const bool need_e = j >= 2 && (wc || wet(g.kbot[(i + 1) * g.Y + j], k));
-    const bool need_n = i >= 2 && (wc || wet(g.kbot[i * g.Y + j + 1], k));
+    const bool need_e = j >= 2 && (wc || wet(kb_e, k));
+    const bool need_n = i >= 2 && (wc || wet(kb_n, k));
     const bool need_t = i >= 2 && j >= 2 && (wc || (k + 1 < nz && wet(kb, k + 1)));
+    if (!(need_e || need_n || need_t)) return;
+
+    // every global read before the first flux
+    const uint32_t si = (uint32_t)g.Y * (uint32_t)nz, sj = (uint32_t)nz;
+    const uint32_t c = (uint32_t)i * si + (uint32_t)j * sj + (uint32_t)k;
+    const bool hm = k >= 1;
+    const double* __restrict__ Kf = ISO ? a.Kiso : a.Kskew;
+    FaceIn in{};
+    in.K_c = Kf[c];
+    in.K_cm = hm ? Kf[c - 1] : 0.0;
+    in.dzt = g.dzt[k];
+    in.cost = g.cost[j];
+    in.dxu = g.dxu[i];
+    in.cosu = g.cosu[j];
+    in.cosu_s = g.cosu[j - 1];
+    in.dyu = g.dyu[j];
+    in.dxt = g.dxt[i];
+    in.dyt = g.dyt[j];
+    if (need_e) {
+        in.K_e = Kf[c + si];
+        in.K_em = hm ? Kf[c + si - 1] : 0.0;
+        in.ez01 = *reinterpret_cast<const double2*>(a.Aez + 4u * c);      // ip=0: kr=0, 1
+        in.ez23 = *reinterpret_cast<const double2*>(a.Aez + 4u * c + 2);  // ip=1: kr=0, 1
+        in.K11 = a.K11[c];
+    }
+    if (need_n) {
+        in.K_n = Kf[c + sj];
+        in.K_nm = hm ? Kf[c + sj - 1] : 0.0;
+        in.nz01 = *reinterpret_cast<const double2*>(a.Anz + 4u * c);
+        in.nz23 = *reinterpret_cast<const double2*>(a.Anz + 4u * c + 2);
+        in.K22 = a.K22[c];
+    }
+    const bool top = need_t && k < nz - 1;
+    if (top) {
+        in.bx01 = *reinterpret_cast<const double2*>(a.Abx + 4u * c);
+        in.bx23 = *reinterpret_cast<const double2*>(a.Abx + 4u * c + 2);
+        in.by01 = *reinterpret_cast<const double2*>(a.Aby + 4u * c);
+        in.by23 = *reinterpret_cast<const double2*>(a.Aby + 4u * c + 2);
+    }
 #pragma unroll
     for (int t = 0; t < NTR; ++t) {
         const Cols tc = cols_of(lds + t * per, nz);
-        if (need_e) a.t[t].fe[c] = flux_east(g, a, tc, st, i, j, k);
-        if (need_n) a.t[t].fn[c] = flux_north(g, a, tc, st, i, j, k);
-        if (need_t) a.t[t].ft[c] = (k < nz - 1) ? flux_top(g, a, tc, st, i, j, k) : 0.0;
+        if (need_e) a.t[t].fe[c] = flux_east<ISO>(in, tc, k, nz);
+        if (need_n) a.t[t].fn[c] = flux_north<ISO>(in, tc, k, nz);
+        if (need_t) a.t[t].ft[c] = top ? flux_top<ISO>(in, tc, k) : 0.0;
     }
 }
 
@@ -692,6 +736,14 @@ __global__ __launch_bounds__(256) void iso_step_tile_kernel(Grid g, Prm p, StepA
         if (k >= 1) K33m = a.K33[c - 1];
     }
     const double P0 = (energy && wc) ? a.P[c] : 0.0;
+    double dxt_i = 0.0, cost_j = 1.0, dyt_j = 0.0;
+    if (valid) {
+        dxt_i = g.dxt[i];
+        cost_j = g.cost[j];
+        dyt_j = g.dyt[j];
+    }
+    const double dzt_k = g.dzt[k], dzw_k = g.dzw[k];
+    const double dzw_lo = g.dzw[k >= 1 ? k - 1 : 0];  // pad_z_edges(dzw)[k]
 
     // ---- phase 1a: flux_top and lateral dissipation -> LDS, K33 rows -> LDS
 #pragma unroll
@@ -705,7 +757,7 @@ __global__ __launch_bounds__(256) void iso_step_tile_kernel(Grid g, Prm p, StepA
                 const double de = (Ie[t] - Ic[t]) * fe_c[t] + (Ic[t] - Iw[t]) * fe_w[t];
                 const double dn = (In[t] - Ic[t]) * fn_c[t] + (Ic[t] - Is[t]) * fn_s[t];
                 const double g05 = 0.5 * p.grav / p.rho_0;
-                diss = g05 * de / (g.dxt[i] * g.cost[j]) + g05 * dn / (g.dyt[j] * g.cost[j]);
+                diss = g05 * de / (dxt_i * cost_j) + g05 * dn / (dyt_j * cost_j);
             }
             sTt[tcnz + lo] = diss;
         }
@@ -713,9 +765,9 @@ __global__ __launch_bounds__(256) void iso_step_tile_kernel(Grid g, Prm p, StepA
     if (IMPLICIT && interior && wc) {
         // rows of the K33 system (veros/core/isoneutral/diffusion.py:154-164); the
         // edge row k = kbot-1 takes b_edge and a = 0 (veros/core/operators.py:72)
-        const double delta = (k < nz - 1) ? dt / g.dzw[k] * K33k : 0.0;
-        const double delta_m = (k >= 1) ? dt / g.dzw[k - 1] * K33m : 0.0;
-        const double dz = g.dzt[k];
+        const double delta = (k < nz - 1) ? dt / dzw_k * K33k : 0.0;
+        const double delta_m = (k >= 1) ? dt / dzw_lo * K33m : 0.0;
+        const double dz = dzt_k;
         const bool edge = k == kb - 1;
         sA[lo] = edge ? 0.0 : -delta_m / dz;
         sB[lo] = edge ? 1 + delta / dz : 1 + (delta + delta_m) / dz;
@@ -731,9 +783,9 @@ __global__ __launch_bounds__(256) void iso_step_tile_kernel(Grid g, Prm p, StepA
         if (interior && wc) {
             double* sTt = sT + t * 4 * tcnz;
             const double ftm = (k >= 1) ? sTt[lo - 1] : 0.0;
-            const double hor = (fe_c[t] - fe_w[t]) / (g.cost[j] * g.dxt[i]) +
-                               (fn_c[t] - fn_s[t]) / (g.cost[j] * g.dyt[j]);
-            const double vert = (k == 0) ? ft_c[t] / g.dzt[0] : (ft_c[t] - ftm) / g.dzt[k];
+            const double hor = (fe_c[t] - fe_w[t]) / (cost_j * dxt_i) +
+                               (fn_c[t] - fn_s[t]) / (cost_j * dyt_j);
+            const double vert = (k == 0) ? ft_c[t] / dzt_k : (ft_c[t] - ftm) / dzt_k;
             dtrk[t] = mt * hor + mt * vert;
             rhs[t] = trp[t] + dt * dtrk[t];
             sTt[2 * tcnz + lo] = rhs[t];
@@ -800,18 +852,17 @@ __global__ __launch_bounds__(256) void iso_step_tile_kernel(Grid g, Prm p, StepA
                 const double avg = 0.5 * (d0 + d1);
                 const bool edge = land && k == kb - 1;
                 const bool water = land && k > kb - 1;
-                const double dzw_lo = g.dzw[k >= 1 ? k - 1 : 0];  // pad_z_edges(dzw)[k]
-                dw = (avg + 0.5 * (d0 * dzw_lo / g.dzw[k])) * (edge ? 1.0 : 0.0) + avg * (water ? 1.0 : 0.0);
+                dw = (avg + 0.5 * (d0 * dzw_lo / dzw_k)) * (edge ? 1.0 : 0.0) + avg * (water ? 1.0 : 0.0);
             }
             Pk = Pk + dw;
             // vertical mixing term (veros/core/isoneutral/diffusion.py:254-279)
             if (interior && k < nz - 1) {
                 const double mw = m01(kb, k) * m01(kb, k + 1);
-                const double fxa = (-Iu[t] + Ic[t]) / g.dzw[k];
+                const double fxa = (-Iu[t] + Ic[t]) / dzw_k;
                 const double gr = -p.grav / p.rho_0;
                 if constexpr (IMPLICIT) {
                     const double dtrz = sTt[2 * tcnz + lo + 1] - sTt[2 * tcnz + lo];
-                    Pk = Pk + gr * fxa * (ft_c[t] * mw + K33k * dtrz / g.dzw[k] * mw);
+                    Pk = Pk + gr * fxa * (ft_c[t] * mw + K33k * dtrz / dzw_k * mw);
                 } else {
                     Pk = Pk + gr * fxa * ft_c[t] * mw;
                 }
@@ -1135,7 +1186,7 @@ int launch_step(const Grid& g, const Prm& p, const StepArgs& a, hipStream_t s, c
     if (g.nz * TC > 256) return fail(VX_ERR_UNSUPPORTED, "%s: nz=%d too deep", who, g.nz);
     const dim3 fblk = cell_block(g.nz);
     const size_t flux_lds = NTR * (size_t)(3 * fblk.y + 2) * g.nz * sizeof(double);
-    hipLaunchKernelGGL((iso_flux_kernel<NTR>), cell_grid(g.X - 3, g.Y - 3, fblk), fblk, flux_lds, s, g, a);
+    hipLaunchKernelGGL((iso_flux_kernel<NTR, IMPLICIT>), cell_grid(g.X - 3, g.Y - 3, fblk), fblk, flux_lds, s, g, a);
     int rc = check_launch(who);
     if (rc) return rc;
     const dim3 blk(g.nz, TC);
