@@ -18,7 +18,7 @@
 // Tuning knobs (compile-time): minimum waves per SIMD requested for the pre
 // kernel (caps its VGPRs), and interior columns per block of the column-tile kernel.
 #ifndef VX_PRE_MIN_WAVES
-#define VX_PRE_MIN_WAVES 4
+#define VX_PRE_MIN_WAVES 3
 #endif
 #ifndef VX_TILE_COLUMNS
 #define VX_TILE_COLUMNS 2
@@ -143,9 +143,10 @@ __device__ __forceinline__ double exp_reduced(const ExpCoef& e, double x) {
 // taken without evaluating the exponential. `two_over_dslope` = 2/dslope.
 __device__ __forceinline__ double taper(const Prm& p, const ExpCoef& ec, double two_over_dslope, double s) {
     const double x = (fabs(s) - p.iso_slopec) * two_over_dslope;  // -2u
-    if (x > 38.12) return 0.0;
-    if (x < -38.12) return 1.0;
-    return rcp_nr(1.0 + exp_reduced(ec, x));
+    // branch-free: the exponential of the clamped argument, then the exact
+    // end values by selection (a NaN slope stays NaN)
+    const double t = rcp_nr(1.0 + exp_reduced(ec, fmin(fmax(x, -38.12), 38.12)));
+    return x > 38.12 ? 0.0 : (x < -38.12 ? 1.0 : (x == x ? t : x));
 }
 
 // Horizontal and vertical density gradients of one triad, weighted by the
@@ -349,6 +350,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_PRE_MIN_
     const double tds = 2.0 / p.iso_dslope;
     const ExpCoef ec = load_exp_coef();
 
+    // Two triads belong to two faces of this cell: east (ip=0, kr=1) is the
+    // top-face x triad (ip=1, kr=0) and north (jp=0, kr=1) the top-face y triad
+    // (jp=1, kr=0); same density point, same gradients, so the same slope
+    // and taper, formed once here.
+    const Dens w_c = dens(p, mc);
+    const double rd_c = rden(dot(w_c, gz_c));
+    double sl_x = 0.0, tp_x = 0.0, sl_y = 0.0, tp_y = 0.0;
+    if (j >= 2) {
+        sl_x = -dot(w_c, G(mc * me, Te, Tc, Se, Sc, rx_c)) * rd_c;
+        tp_x = taper(p, ec, tds, sl_x);
+    }
+    if (i >= 2) {
+        sl_y = -dot(w_c, G(mc * mn, Tn, Tc, Sn, Sc, ry_c)) * rd_c;
+        tp_y = taper(p, ec, tds, sl_y);
+    }
+
     // --- east face (i+1/2, j, k), j in [2, Y-2)   (:97-132)
     if (j >= 2) {
         const double mU = mc * me;
@@ -364,10 +381,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_PRE_MIN_
             for (int ip = 0; ip < 2; ++ip) {
                 out[ip * 2 + kr] = 0.0;
                 if (kr == 0 && !hm) continue;  // levels k >= 1 - kr
-                const Dens w = dens(p, ip ? me : mc);
-                const Grad& gz = ip ? (kr ? gz_e : gz_em) : (kr ? gz_c : gz_m);
-                const double sl = -dot(w, gx) * rden(dot(w, gz));
-                const double tp = taper(p, ec, tds, sl);
+                double sl = sl_x, tp = tp_x;
+                if (ip != 0 || kr != 1) {
+                    const Dens w = dens(p, ip ? me : mc);
+                    const Grad& gz = ip ? (kr ? gz_e : gz_em) : (kr ? gz_c : gz_m);
+                    sl = -dot(w, gx) * rden(dot(w, gz));
+                    tp = taper(p, ec, tds, sl);
+                }
                 sumz += (kr ? dzw_k : dzw_m) * mU * fmax(p.K_iso_steep, diffloc * tp);
                 out[ip * 2 + kr] = tp * sl * mU;
             }
@@ -400,10 +420,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_PRE_MIN_
             for (int jp = 0; jp < 2; ++jp) {
                 out[jp * 2 + kr] = 0.0;
                 if (kr == 0 && !hm) continue;
-                const Dens w = dens(p, jp ? mn : mc);
-                const Grad& gz = jp ? (kr ? gz_n : gz_nm) : (kr ? gz_c : gz_m);
-                const double sl = -dot(w, gy) * rden(dot(w, gz));
-                const double tp = taper(p, ec, tds, sl);
+                double sl = sl_y, tp = tp_y;
+                if (jp != 0 || kr != 1) {
+                    const Dens w = dens(p, jp ? mn : mc);
+                    const Grad& gz = jp ? (kr ? gz_n : gz_nm) : (kr ? gz_c : gz_m);
+                    sl = -dot(w, gy) * rden(dot(w, gz));
+                    tp = taper(p, ec, tds, sl);
+                }
                 sumz += (kr ? dzw_k : dzw_m) * mV * fmax(p.K_iso_steep, diffloc * tp);
                 out[jp * 2 + kr] = tp * sl * mV;
             }
@@ -440,20 +463,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_PRE_MIN_
 #pragma unroll
         for (int kr = 0; kr < 2; ++kr) {
             const Dens w = dens(p, kr ? mcp : mc);
-            const double rd = rden(dot(w, gz_c));
+            const double rd = kr ? rden(dot(w, gz_c)) : rd_c;
 #pragma unroll
             for (int ip = 0; ip < 2; ++ip) {
-                const Grad& gx = ip ? (kr ? gx_cp : gx_c) : (kr ? gx_wp : gx_w);
-                const double sl = -dot(w, gx) * rd;
-                const double tp = taper(p, ec, tds, sl);
+                double sl = sl_x, tp = tp_x;
+                if (ip != 1 || kr != 0) {
+                    const Grad& gx = ip ? (kr ? gx_cp : gx_c) : (kr ? gx_wp : gx_w);
+                    sl = -dot(w, gx) * rd;
+                    tp = taper(p, ec, tds, sl);
+                }
                 sumx += (ip ? dxu_i : dxu_w) * Kc * tp * (sl * sl) * mW;
                 ox[ip * 2 + kr] = tp * sl * mW;
             }
 #pragma unroll
             for (int jp = 0; jp < 2; ++jp) {
-                const Grad& gy = jp ? (kr ? gy_cp : gy_c) : (kr ? gy_sp : gy_s);
-                const double sl = -dot(w, gy) * rd;
-                const double tp = taper(p, ec, tds, sl);
+                double sl = sl_y, tp = tp_y;
+                if (jp != 1 || kr != 0) {
+                    const Grad& gy = jp ? (kr ? gy_cp : gy_c) : (kr ? gy_sp : gy_s);
+                    sl = -dot(w, gy) * rd;
+                    tp = taper(p, ec, tds, sl);
+                }
                 const double facty = jp ? cosu_j * dyu_j : cosu_s * dyu_s;
                 sumy += facty * Kc * tp * (sl * sl) * mW;
                 oy[jp * 2 + kr] = tp * sl * mW;
