@@ -56,6 +56,22 @@ __device__ __forceinline__ void store_levels(double* __restrict__ col, int nz, c
     }
 }
 
+// Store value(k) for every level k < nz without materialising the column.
+template <bool VEC, int NZB, class F>
+__device__ __forceinline__ void store_levels_fn(double* __restrict__ col, int nz, F value) {
+    if constexpr (VEC) {
+#pragma unroll
+        for (int k = 0; k < NZB; k += 2) {
+            if (k < nz) *reinterpret_cast<double2*>(col + k) = make_double2(value(k), value(k + 1));
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NZB; ++k) {
+            if (k < nz) col[k] = value(k);
+        }
+    }
+}
+
 // Column geometry shared by the kernels: base offset and first wet level
 // (nz when the column has no wet level: kbot outside [1, nz]).
 struct ColumnRef {
@@ -157,8 +173,9 @@ __device__ __forceinline__ void solve_mixed_column(const TdmaArgs& p, const Colu
     const bool has_be = p.b_edge != nullptr;
     const bool has_de = p.d_edge != nullptr;
 
-    // float32 factors: cp, 1/den, a/den; x0 = float32 solution
-    float cpf[NZB], rif[NZB], arf[NZB], x0[NZB];
+    // float32 factors: cp and 1/den (a/den is re-formed from the re-streamed
+    // a); x0 = float32 solution
+    float cpf[NZB], rif[NZB], x0[NZB];
     // second float word of the solution (REFINE == 2 keeps x = x0 + z1)
     float z1[REFINE == 2 ? NZB : 1];
     {
@@ -185,7 +202,6 @@ __device__ __forceinline__ void solve_mixed_column(const TdmaArgs& p, const Colu
                         const float ri = __builtin_amdgcn_rcpf(bf - af * cpp);
                         cpp = cf * ri;
                         ypp = (df - af * ypp) * ri;
-                        arf[k] = af * ri;
                         rif[k] = ri;
                     }
                     cpf[k] = cpp;
@@ -204,10 +220,7 @@ __device__ __forceinline__ void solve_mixed_column(const TdmaArgs& p, const Colu
         }
     }
     if constexpr (REFINE == 0) {
-        double out[NZB];
-#pragma unroll
-        for (int k = 0; k < NZB; ++k) out[k] = (double)x0[k];
-        store_levels<VEC, NZB>(p.x + cr.base, nz, out);
+        store_levels_fn<VEC, NZB>(p.x + cr.base, nz, [&](int k) { return (double)x0[k]; });
         return;
     } else {
         if constexpr (REFINE == 2) {
@@ -247,7 +260,7 @@ __device__ __forceinline__ void solve_mixed_column(const TdmaArgs& p, const Colu
                             const double ck = (k == nz - 1) ? 0.0 : vc[t];
                             const double dk = (edge && has_de) ? de : vd[t];
                             const double r = dk - ak * xprev - bk * xk - ck * xu;
-                            zpp = (float)r * rif[k] - arf[k] * zpp;
+                            zpp = (float)r * rif[k] - ((float)ak * rif[k]) * zpp;
                             xprev = xk;
                         }
                         zz[k] = zpp;
@@ -264,10 +277,7 @@ __device__ __forceinline__ void solve_mixed_column(const TdmaArgs& p, const Colu
                 }
             }
             if (step == REFINE - 1) {
-                double out[NZB];
-#pragma unroll
-                for (int k = 0; k < NZB; ++k) out[k] = xval(k) + (double)zz[k];
-                store_levels<VEC, NZB>(p.x + cr.base, nz, out);
+                store_levels_fn<VEC, NZB>(p.x + cr.base, nz, [&](int k) { return xval(k) + (double)zz[k]; });
             } else if constexpr (REFINE == 2) {
                 // x = x0 + z1 + zz, re-split into two float words (x0 + z1 is exact in float64)
 #pragma unroll
