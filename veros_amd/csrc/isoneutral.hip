@@ -21,7 +21,7 @@
 #define VX_PRE_MIN_WAVES 3
 #endif
 #ifndef VX_TILE_COLUMNS
-#define VX_TILE_COLUMNS 2
+#define VX_TILE_COLUMNS 3
 #endif
 
 namespace vx {
@@ -246,16 +246,91 @@ __device__ __forceinline__ Cols cols_of(const double* staged, int nz) {
 // gradients are formed from them in registers. Offsets are 32-bit element
 // offsets (the host checks the arrays fit).
 // ---------------------------------------------------------------------------
+// The dry cells of isoneutral_diffusion_pre. A dry cell has every output
+// exactly zero: its own maskT factor is in maskU (Ai_ez, K_11), maskV (Ai_nz,
+// K_22) and maskW (Ai_bx, Ai_by, K_33), which multiply each entry and each
+// summed term. The dry cells of a column are its levels 0 .. kbot-2, one
+// contiguous run per array, so a wave per column zero-fills those runs with
+// lane-contiguous stores (full cache lines) over the index ranges the
+// reference writes; iso_pre_kernel handles the wet cells.
+__device__ __forceinline__ void zero_run(double* __restrict__ p, int n, int lane) {
+    for (int m = lane; m < n; m += 64) p[m] = 0.0;
+}
+// Ai entries 4 c + 2 ip + kr of the cells 0 .. nd-1 of a column; at level 0
+// only the kr = 1 entries are written (the kr = 0 ones keep their values).
+__device__ __forceinline__ void zero_ai_run(double* __restrict__ p, int nd, int lane, bool keep_level0_kr0) {
+    const int n2 = 2 * nd;  // double2 pieces
+    for (int m = lane; m < n2; m += 64) {
+        if (keep_level0_kr0 && m < 2) p[2 * m + 1] = 0.0;
+        else *reinterpret_cast<double2*>(p + 2 * m) = make_double2(0.0, 0.0);
+    }
+}
+
+__global__ __launch_bounds__(256) void iso_pre_dry_kernel(Grid g, double* __restrict__ Aez, double* __restrict__ Anz,
+                                                          double* __restrict__ Abx, double* __restrict__ Aby,
+                                                          double* __restrict__ K11, double* __restrict__ K22,
+                                                          double* __restrict__ K33) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int64_t ni = g.X - 3, nj = g.Y - 3;
+    const int64_t col = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (col >= ni * nj) return;
+    const int i = 1 + (int)(col / nj), j = 1 + (int)(col - (col / nj) * nj);
+    const int nz = g.nz;
+    const int kb = g.kbot[i * g.Y + j];
+    const int nd = (kb >= 1 && kb <= nz) ? kb - 1 : nz;  // dry levels 0 .. nd-1
+    if (nd == 0) return;
+    const uint32_t c0 = ((uint32_t)i * (uint32_t)g.Y + (uint32_t)j) * (uint32_t)nz;
+    if (j >= 2) {
+        zero_ai_run(Aez + 4u * c0, nd, lane, true);
+        zero_run(K11 + c0, nd, lane);
+    }
+    if (i >= 2) {
+        zero_ai_run(Anz + 4u * c0, nd, lane, true);
+        zero_run(K22 + c0, nd, lane);
+    }
+    if (i >= 2 && j >= 2) {
+        const int nt = nd < nz - 1 ? nd : nz - 1;  // top faces: levels below nz-1
+        zero_ai_run(Abx + 4u * c0, nt, lane, false);
+        zero_ai_run(Aby + 4u * c0, nt, lane, false);
+        zero_run(K33 + c0, nt, lane);
+    }
+}
+
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_PRE_MIN_WAVES))) void iso_pre_kernel(
     Grid g, Prm p, Fields f, const double* __restrict__ K,
                                                       double* __restrict__ Aez, double* __restrict__ Anz,
                                                       double* __restrict__ Abx, double* __restrict__ Aby,
                                                       double* __restrict__ K11, double* __restrict__ K22,
                                                       double* __restrict__ K33) {
-    const CellIdx ci = cell_of(1, g.X - 3, 1, g.Y - 3);
-    if (!ci.valid) return;
-    const int i = ci.i, j = ci.j, k = ci.k;
-    const int nz = g.nz;
+    // Wet cells only (the dry ones are written by iso_pre_dry_kernel).
+    // Block-local compaction: a block owns the columns (i, jb .. jb+cpb-1);
+    // its wet cells (the wet suffixes of those columns) are handed to its
+    // first threads, so the slope work fills whole waves.
+    extern __shared__ int pre_counts[];  // prefix sums of the columns' wet counts
+    const CellIdx ci0 = cell_of(1, g.X - 3, 1, g.Y - 3);
+    const int nz = g.nz, cpb = (int)blockDim.y;
+    const int jb = ci0.j - (int)threadIdx.y;
+    if (jb > g.Y - 3) return;  // padding block (uniform)
+    const int i = ci0.i;
+    const int tid = (int)(threadIdx.y * blockDim.x + threadIdx.x);
+    int* sW = pre_counts;
+    if (tid < cpb) {
+        const bool col_ok = jb + tid <= g.Y - 3;
+        const int kbq = col_ok ? g.kbot[i * g.Y + jb + tid] : 0;
+        sW[tid + 1] = (kbq >= 1 && kbq <= nz) ? nz - kbq + 1 : 0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        sW[0] = 0;
+        for (int q = 0; q < cpb; ++q) sW[q + 1] += sW[q];
+    }
+    __syncthreads();
+    const int nwet = sW[cpb];
+    if (tid >= nwet) return;  // dry cells: iso_pre_dry_kernel
+    int q = 0;
+    while (sW[q + 1] <= tid) ++q;
+    const int k = nz - (sW[q + 1] - sW[q]) + (tid - sW[q]);  // wet suffix of column q
+    const int j = jb + q;
     const uint32_t sj = (uint32_t)nz, si = (uint32_t)g.Y * (uint32_t)nz;
     const uint32_t c = (uint32_t)i * si + (uint32_t)j * sj + (uint32_t)k;
     const uint32_t ce = c + si, cw = c - si, cn = c + sj, cs = c - sj;
@@ -264,42 +339,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_PRE_MIN_
     const double* __restrict__ S = f.S + f.tau;
     const int kb_c = g.kbot[i * g.Y + j], kb_e = g.kbot[(i + 1) * g.Y + j], kb_w = g.kbot[(i - 1) * g.Y + j];
     const int kb_n = g.kbot[i * g.Y + j + 1], kb_s = g.kbot[i * g.Y + j - 1];
-
-    // A dry cell has every output exactly zero: its own maskT factor is in
-    // maskU (Ai_ez, K_11), maskV (Ai_nz, K_22) and maskW (Ai_bx, Ai_by, K_33),
-    // which multiply each entry and each summed term. Write the zeros over the
-    // same index ranges and skip the slopes.
-    if (!wet(kb_c, k)) {
-        const double2 z2 = make_double2(0.0, 0.0);
-        if (j >= 2) {
-            if (hm) {
-                *reinterpret_cast<double2*>(Aez + 4u * c) = z2;
-                *reinterpret_cast<double2*>(Aez + 4u * c + 2) = z2;
-            } else {
-                Aez[4u * c + 1] = 0.0;
-                Aez[4u * c + 3] = 0.0;
-            }
-            K11[c] = 0.0;
-        }
-        if (i >= 2) {
-            if (hm) {
-                *reinterpret_cast<double2*>(Anz + 4u * c) = z2;
-                *reinterpret_cast<double2*>(Anz + 4u * c + 2) = z2;
-            } else {
-                Anz[4u * c + 1] = 0.0;
-                Anz[4u * c + 3] = 0.0;
-            }
-            K22[c] = 0.0;
-        }
-        if (i >= 2 && j >= 2 && hp) {
-            *reinterpret_cast<double2*>(Abx + 4u * c) = z2;
-            *reinterpret_cast<double2*>(Abx + 4u * c + 2) = z2;
-            *reinterpret_cast<double2*>(Aby + 4u * c) = z2;
-            *reinterpret_cast<double2*>(Aby + 4u * c + 2) = z2;
-            K33[c] = 0.0;
-        }
-        return;
-    }
 
     // tracer values: c(enter), e(ast), w(est), n(orth), s(outh); p = k+1, m = k-1
     auto L2 = [&](uint32_t o, bool ok, double& t, double& sv) {
@@ -699,7 +738,7 @@ __device__ __forceinline__ double fn_at(const Grid& g, const TracerArgs& t, int 
 }
 
 template <int TC, int NTR, bool IMPLICIT>
-__global__ __launch_bounds__(256) void iso_step_tile_kernel(Grid g, Prm p, StepArgs a) {
+__global__ __launch_bounds__(TC * kMaxNz) void iso_step_tile_kernel(Grid g, Prm p, StepArgs a) {
     extern __shared__ double lds[];
     const int nz = g.nz;
     const int tcnz = TC * nz;
@@ -1194,7 +1233,12 @@ extern "C" int vx_iso_pre_f64(void* stream, const vx_grid* grid, const vx_iso_pa
     if ((int64_t)g.X * g.Y * g.nz * 4 >= ((int64_t)1 << 32))
         return fail(VX_ERR_UNSUPPORTED, "vx_iso_pre_f64: grid too large for 32-bit offsets");
     const dim3 blk = cell_block(g.nz);
-    hipLaunchKernelGGL(iso_pre_kernel, cell_grid(g.X - 3, g.Y - 3, blk), blk, 0, s, g, p, f, K_iso, Ai_ez, Ai_nz,
+    const int64_t ncol = (int64_t)(g.X - 3) * (g.Y - 3);  // one wave per column, 4 per block
+    hipLaunchKernelGGL(iso_pre_dry_kernel, dim3(blocks_for(ncol, 4)), dim3(256), 0, s, g, Ai_ez, Ai_nz, Ai_bx, Ai_by,
+                       K_11, K_22, K_33);
+    if ((rc = check_launch("vx_iso_pre_f64 (dry cells)"))) return rc;
+    const size_t pre_lds = (size_t)(blk.y + 1) * sizeof(int);
+    hipLaunchKernelGGL(iso_pre_kernel, cell_grid(g.X - 3, g.Y - 3, blk), blk, pre_lds, s, g, p, f, K_iso, Ai_ez, Ai_nz,
                        Ai_bx, Ai_by, K_11, K_22, K_33);
     if ((rc = check_launch("vx_iso_pre_f64"))) return rc;
     hipLaunchKernelGGL(k33_top_zero_kernel, dim3(blocks_for((int64_t)g.X * g.Y, 256)), dim3(256), 0, s, g, K_33);
@@ -1212,7 +1256,7 @@ int launch_step(const Grid& g, const Prm& p, const StepArgs& a, hipStream_t s, c
     constexpr int TC = kTileColumns;
     if ((int64_t)g.X * g.Y * g.nz * 4 >= ((int64_t)1 << 32))
         return fail(VX_ERR_UNSUPPORTED, "%s: grid too large for 32-bit offsets", who);
-    if (g.nz * TC > 256) return fail(VX_ERR_UNSUPPORTED, "%s: nz=%d too deep", who, g.nz);
+    if (g.nz > kMaxNz) return fail(VX_ERR_UNSUPPORTED, "%s: nz=%d too deep", who, g.nz);
     const dim3 fblk = cell_block(g.nz);
     const size_t flux_lds = NTR * (size_t)(3 * fblk.y + 2) * g.nz * sizeof(double);
     hipLaunchKernelGGL((iso_flux_kernel<NTR, IMPLICIT>), cell_grid(g.X - 3, g.Y - 3, fblk), fblk, flux_lds, s, g, a);
