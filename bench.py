@@ -23,11 +23,16 @@ REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO))
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of iso_pre_kernel on 1440x720x80, from the FETCH_SIZE and
-# WRITE_SIZE rocprofv3 passes in profiles/mixing_step_1440x720x80.md (read
-# bytes = 2.0 x FETCH_SIZE, write bytes = 1.0 x WRITE_SIZE, calibrated on a
-# 1 GiB device copy)
-PRE_TRAFFIC_BYTES = {(1440, 720, 80): (14.14 + 13.21) * 1e9}
+# HBM bytes per call of vx_iso_diffusion_ts_f64 (its two kernels) on
+# 1440x720x80, from the FETCH_SIZE and WRITE_SIZE rocprofv3 passes recorded in
+# profiles/ (read bytes = 2.0 x FETCH_SIZE, write bytes = 1.0 x WRITE_SIZE,
+# calibrated on a 1 GiB device copy)
+TS_TRAFFIC_BYTES = {(1440, 720, 80): 21_040_000_000}
+# algorithmic bytes per interior cell: the fused temperature + salinity step
+# with the energy branch reads K_iso, Ai_* and K_11/22/33 once (8 + 128 + 24),
+# P_diss read and written (16), and per tracer tr[tau], tr[taup1] read and
+# written, dtr_iso read and written, int_drhodX (8 + 16 + 16 + 8)
+TS_BYTES_PER_CELL = 8 + 128 + 24 + 16 + 2 * (8 + 16 + 16 + 8)
 
 
 def parse():
@@ -306,7 +311,8 @@ def main():
     B_td, W_td = tdma_bytes(td["kbot"], nz)
     C = nx * ny
     B_pre = 176 * N + 4 * (nx + 4) * (ny + 4)
-    B_diff = 2 * (224 * N + 4 * (nx + 4) * (ny + 4))
+    B_diff_two = 2 * (224 * N + 4 * (nx + 4) * (ny + 4))
+    B_diff = TS_BYTES_PER_CELL * N + 4 * (nx + 4) * (ny + 4)
     rates = {
         "tdma_mcolumns_per_s": {p: C / (parts[f"tdma_{p}"][0] * 1e-3) / 1e6 for p in ("fp64", "mixed")},
         "iso_mcells_per_s": N / ((parts["iso_pre"][0] + parts["iso_diffusion_TS"][0]) * 1e-3) / 1e6,
@@ -316,16 +322,20 @@ def main():
         "tdma_mixed": (B_td, parts["tdma_mixed"][0]),
         "iso_pre": (B_pre, parts["iso_pre"][0]),
         "iso_diffusion_TS": (B_diff, parts["iso_diffusion_TS"][0]),
-        "iso_diffusion_T_then_S": (B_diff, parts["iso_diffusion_T_then_S"][0]),
+        "iso_diffusion_T_then_S": (B_diff_two, parts["iso_diffusion_T_then_S"][0]),
     }
     gbs = {k: b / (t * 1e-3) / 1e9 for k, (b, t) in kern.items()}
-    # roofline: iso_pre_kernel, the largest single-kernel entry point (its call
-    # also launches a 40-us kernel zeroing K_33's top level)
-    dom = "iso_pre"
-    roof = {"bound": "hbm", "kernel": "iso_pre_kernel", "achieved": round(gbs[dom], 1), "peak": HBM_PEAK_GBS,
+    # roofline: the fused temperature + salinity diffusion, the largest launch
+    # of the step (one C-ABI call, two kernels: the face fluxes, then the
+    # divergence + K33 solve + updates)
+    dom = "iso_diffusion_TS"
+    roof = {"bound": "hbm",
+            "kernel": "vx_iso_diffusion_ts_f64 = iso_flux_kernel<2,true> + iso_step_tile_kernel<3,2,true>",
+            "achieved": round(gbs[dom], 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(gbs[dom] / HBM_PEAK_GBS, 4),
-            "traffic": PRE_TRAFFIC_BYTES.get((nx, ny, nz)),
-            "algorithmic_bytes_per_launch": kern[dom][0], "avg_launch_ms": round(kern[dom][1], 4)}
+            "traffic": TS_TRAFFIC_BYTES.get((nx, ny, nz)),
+            "algorithmic_bytes_per_launch": kern[dom][0], "algorithmic_bytes_per_cell": TS_BYTES_PER_CELL,
+            "avg_launch_ms": round(kern[dom][1], 4)}
     out = {
         "metric": "TDMA Mcolumns/s + isoneutral Mcells/s; mixed vs FP64 GPU at <=1e-12 rel err",
         "value": round(N / (ms * 1e-3) / 1e6, 2),
