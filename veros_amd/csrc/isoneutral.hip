@@ -13,6 +13,9 @@
 // contiguous run. Masks are derived from kbot exactly as calc_topo_kernel
 // builds them (veros/core/numerics.py:195-221): maskT = kbot>0 & k>=kbot-1,
 // maskU/V = min of the two adjacent maskT, maskW = min(maskT[k], maskT[k+1]).
+#include <map>
+#include <mutex>
+
 #include "vx_common.hpp"
 
 // Tuning knobs (compile-time): minimum waves per SIMD requested for the pre
@@ -1201,6 +1204,31 @@ Prm make_prm(const vx_iso_params* q) {
     return p;
 }
 
+// A helper stream per device (default priority, non-blocking) with a fork and
+// a join event, created on first use and kept for the process.
+struct AuxStream {
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+std::mutex& aux_mutex() {
+    static std::mutex m;
+    return m;
+}
+// Caller holds aux_mutex(). Null when the stream or events cannot be made.
+AuxStream* aux_stream() {
+    static std::map<int, AuxStream> per_device;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    auto it = per_device.find(dev);
+    if (it != per_device.end()) return &it->second;
+    AuxStream a;
+    if (hipStreamCreateWithFlags(&a.s, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&a.fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&a.join, hipEventDisableTiming) != hipSuccess)
+        return nullptr;
+    return &(per_device[dev] = a);
+}
+
 unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 unsigned blocks_xcd(int64_t n, int bs) { return (unsigned)(((n + bs - 1) / bs + 7) / 8 * 8); }
 
@@ -1234,7 +1262,15 @@ extern "C" int vx_iso_pre_f64(void* stream, const vx_grid* grid, const vx_iso_pa
         return fail(VX_ERR_UNSUPPORTED, "vx_iso_pre_f64: grid too large for 32-bit offsets");
     const dim3 blk = cell_block(g.nz);
     const int64_t ncol = (int64_t)(g.X - 3) * (g.Y - 3);  // one wave per column, 4 per block
-    hipLaunchKernelGGL(iso_pre_dry_kernel, dim3(blocks_for(ncol, 4)), dim3(256), 0, s, g, Ai_ez, Ai_nz, Ai_bx, Ai_by,
+    // The dry-cell zero fill (store-bound) and the wet-cell slopes (latency-
+    // bound) write disjoint cells: the zero fill runs on a helper stream forked
+    // from and joined back into `s`, so the call stays ordered on `s`.
+    std::lock_guard<std::mutex> lk(aux_mutex());
+    AuxStream* ax = aux_stream();
+    hipStream_t sd = s;
+    if (ax && hipEventRecord(ax->fork, s) == hipSuccess && hipStreamWaitEvent(ax->s, ax->fork, 0) == hipSuccess)
+        sd = ax->s;
+    hipLaunchKernelGGL(iso_pre_dry_kernel, dim3(blocks_for(ncol, 4)), dim3(256), 0, sd, g, Ai_ez, Ai_nz, Ai_bx, Ai_by,
                        K_11, K_22, K_33);
     if ((rc = check_launch("vx_iso_pre_f64 (dry cells)"))) return rc;
     const size_t pre_lds = (size_t)(blk.y + 1) * sizeof(int);
@@ -1242,7 +1278,12 @@ extern "C" int vx_iso_pre_f64(void* stream, const vx_grid* grid, const vx_iso_pa
                        Ai_bx, Ai_by, K_11, K_22, K_33);
     if ((rc = check_launch("vx_iso_pre_f64"))) return rc;
     hipLaunchKernelGGL(k33_top_zero_kernel, dim3(blocks_for((int64_t)g.X * g.Y, 256)), dim3(256), 0, s, g, K_33);
-    return check_launch("vx_iso_pre_f64 (K_33 top)");
+    if ((rc = check_launch("vx_iso_pre_f64 (K_33 top)"))) return rc;
+    if (sd != s) {
+        if (hipEventRecord(ax->join, sd) != hipSuccess || hipStreamWaitEvent(s, ax->join, 0) != hipSuccess)
+            return fail(VX_ERR_LAUNCH, "vx_iso_pre_f64: joining the helper stream failed");
+    }
+    return VX_OK;
 }
 
 
