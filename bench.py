@@ -3,7 +3,9 @@
 One step = the kbot-masked TDMA solve (solve_implicit on 1440x720x80
 vertical-diffusion systems) + isoneutral_diffusion_pre + isoneutral_diffusion
 for temp and for salt on the 1440x720x80 quarter-degree grid, inputs resident
-in HBM. Timed with HIP events on the stream the kernels run on.
+in HBM; the isoneutral part runs as one call (vx_iso_pre_diffusion_ts_f64),
+with the separate pre and temperature + salinity calls timed beside it.
+Timed with HIP events on the stream the kernels run on.
 
 Prints ONE JSON line (rank 0). ``value`` is the step throughput in Mcells/s
 (interior cells / step time) of the configured precision form; TDMA
@@ -23,16 +25,20 @@ REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO))
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per call of vx_iso_diffusion_ts_f64 (its two kernels) on
-# 1440x720x80, from the FETCH_SIZE and WRITE_SIZE rocprofv3 passes recorded in
-# profiles/ (read bytes = 2.0 x FETCH_SIZE, write bytes = 1.0 x WRITE_SIZE,
-# calibrated on a 1 GiB device copy)
+# HBM bytes per call of vx_iso_diffusion_ts_f64 (its two kernels) and of
+# vx_iso_pre_diffusion_ts_f64 (its four) on 1440x720x80, from the FETCH_SIZE
+# and WRITE_SIZE rocprofv3 passes recorded in profiles/ (read bytes = 2.0 x
+# FETCH_SIZE, write bytes = 1.0 x WRITE_SIZE, calibrated on a 1 GiB device copy)
 TS_TRAFFIC_BYTES = {(1440, 720, 80): 21_040_000_000}
+PRE_TS_TRAFFIC_BYTES = {}
 # algorithmic bytes per interior cell: the fused temperature + salinity step
 # with the energy branch reads K_iso, Ai_* and K_11/22/33 once (8 + 128 + 24),
 # P_diss read and written (16), and per tracer tr[tau], tr[taup1] read and
 # written, dtr_iso read and written, int_drhodX (8 + 16 + 16 + 8)
 TS_BYTES_PER_CELL = 8 + 128 + 24 + 16 + 2 * (8 + 16 + 16 + 8)
+# pre + the temperature + salinity step in one call: K_iso read (8), Ai_* and
+# K_11/22/33 written instead of read (128 + 24), the rest as above
+PRE_TS_BYTES_PER_CELL = 8 + 128 + 24 + 16 + 2 * (8 + 16 + 16 + 8)
 
 
 def parse():
@@ -264,6 +270,9 @@ def main():
     def run_diff():
         isoneutral.isoneutral_diffusion_temp_salt(state)
 
+    def run_pre_diff():
+        isoneutral.isoneutral_pre_and_diffusion_temp_salt(state)
+
     def run_diff_two_calls():
         isoneutral.isoneutral_diffusion(state, vs.temp, True)
         isoneutral.isoneutral_diffusion(state, vs.salt, False)
@@ -282,7 +291,7 @@ def main():
     parts = {}
     for name, fn in (("tdma_fp64", lambda: run_tdma("fp64")), ("tdma_mixed", lambda: run_tdma("mixed")),
                      ("iso_pre", run_pre), ("iso_diffusion_TS", run_diff),
-                     ("iso_diffusion_T_then_S", run_diff_two_calls)):
+                     ("iso_diffusion_T_then_S", run_diff_two_calls), ("iso_pre_and_diffusion_TS", run_pre_diff)):
         for _ in range(args.warmup):
             fn()
         parts[name] = timed(fn, max(5, args.steps))
@@ -293,8 +302,7 @@ def main():
 
     def step():
         run_tdma(prec)
-        run_pre()
-        run_diff()
+        run_pre_diff()
 
     for _ in range(args.warmup):
         step()
@@ -313,9 +321,11 @@ def main():
     B_pre = 176 * N + 4 * (nx + 4) * (ny + 4)
     B_diff_two = 2 * (224 * N + 4 * (nx + 4) * (ny + 4))
     B_diff = TS_BYTES_PER_CELL * N + 4 * (nx + 4) * (ny + 4)
+    B_pre_diff = PRE_TS_BYTES_PER_CELL * N + 4 * (nx + 4) * (ny + 4)
     rates = {
         "tdma_mcolumns_per_s": {p: C / (parts[f"tdma_{p}"][0] * 1e-3) / 1e6 for p in ("fp64", "mixed")},
-        "iso_mcells_per_s": N / ((parts["iso_pre"][0] + parts["iso_diffusion_TS"][0]) * 1e-3) / 1e6,
+        "iso_mcells_per_s": N / (parts["iso_pre_and_diffusion_TS"][0] * 1e-3) / 1e6,
+        "iso_mcells_per_s_pre_then_TS": N / ((parts["iso_pre"][0] + parts["iso_diffusion_TS"][0]) * 1e-3) / 1e6,
     }
     kern = {
         "tdma_fp64": (B_td, parts["tdma_fp64"][0]),
@@ -323,18 +333,21 @@ def main():
         "iso_pre": (B_pre, parts["iso_pre"][0]),
         "iso_diffusion_TS": (B_diff, parts["iso_diffusion_TS"][0]),
         "iso_diffusion_T_then_S": (B_diff_two, parts["iso_diffusion_T_then_S"][0]),
+        "iso_pre_and_diffusion_TS": (B_pre_diff, parts["iso_pre_and_diffusion_TS"][0]),
     }
     gbs = {k: b / (t * 1e-3) / 1e9 for k, (b, t) in kern.items()}
-    # roofline: the fused temperature + salinity diffusion, the largest launch
-    # of the step (one C-ABI call, two kernels: the face fluxes, then the
-    # divergence + K33 solve + updates)
-    dom = "iso_diffusion_TS"
+    # roofline: pre + the temperature + salinity diffusion in one C-ABI call,
+    # the largest launch of the step (four kernels: the dry-column zero fill on
+    # a helper stream beside the wet-cell slopes + face fluxes, the top-level
+    # K_33 zero, then the divergence + K33 solve + updates)
+    dom = "iso_pre_and_diffusion_TS"
     roof = {"bound": "hbm",
-            "kernel": "vx_iso_diffusion_ts_f64 = iso_flux_kernel<2,true> + iso_step_tile_kernel<3,2,true>",
+            "kernel": "vx_iso_pre_diffusion_ts_f64 = iso_pre_dry_kernel || iso_pre_kernel<true> + "
+                      "k33_top_zero_kernel + iso_step_tile_kernel<3,2,true,true>",
             "achieved": round(gbs[dom], 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(gbs[dom] / HBM_PEAK_GBS, 4),
-            "traffic": TS_TRAFFIC_BYTES.get((nx, ny, nz)),
-            "algorithmic_bytes_per_launch": kern[dom][0], "algorithmic_bytes_per_cell": TS_BYTES_PER_CELL,
+            "traffic": PRE_TS_TRAFFIC_BYTES.get((nx, ny, nz)),
+            "algorithmic_bytes_per_launch": kern[dom][0], "algorithmic_bytes_per_cell": PRE_TS_BYTES_PER_CELL,
             "avg_launch_ms": round(kern[dom][1], 4)}
     out = {
         "metric": "TDMA Mcolumns/s + isoneutral Mcells/s; mixed vs FP64 GPU at <=1e-12 rel err",
@@ -349,6 +362,7 @@ def main():
                    "precision": prec, "refine_steps": args.refine, "tdma_wet_cells": W_td},
         "tdma_mcolumns_per_s": {k: round(v, 1) for k, v in rates["tdma_mcolumns_per_s"].items()},
         "iso_mcells_per_s": round(rates["iso_mcells_per_s"], 1),
+        "iso_mcells_per_s_pre_then_TS": round(rates["iso_mcells_per_s_pre_then_TS"], 1),
         "mixed_over_fp64_tdma": round(parts["tdma_fp64"][0] / parts["tdma_mixed"][0], 3),
         "kernel_ms_median": {k: round(v[0], 4) for k, v in parts.items()},
         "kernel_algorithmic_GBps": {k: round(v, 1) for k, v in gbs.items()},

@@ -225,6 +225,24 @@ int vx_iso_diffusion_ts_f64(void* stream, const vx_grid* grid, const vx_iso_para
                             void* workspace);
 
 /*
+ * The sequence veros/core/thermodynamics.py:430-432 runs each step,
+ *   isoneutral_diffusion_pre, isoneutral_diffusion(temp), isoneutral_diffusion(salt),
+ * in one call: the outputs of vx_iso_pre_f64 followed by
+ * vx_iso_diffusion_ts_f64 on the same arrays. The face fluxes of temperature
+ * and salinity are formed from the isoneutral tensor while it is computed,
+ * instead of reading it back. `workspace` as for vx_iso_diffusion_ts_f64
+ * (vx_iso_workspace_bytes(grid, 2) bytes).
+ */
+int vx_iso_pre_diffusion_ts_f64(void* stream, const vx_grid* grid, const vx_iso_params* prm,
+                                double* temp, double* salt, int32_t tau, int32_t taup1,
+                                const double* K_iso,
+                                double* Ai_ez, double* Ai_nz, double* Ai_bx, double* Ai_by,
+                                double* K_11, double* K_22, double* K_33,
+                                double* dtemp_iso, double* dsalt_iso,
+                                const double* int_drhodT, const double* int_drhodS, double* P_diss,
+                                void* workspace);
+
+/*
  * isoneutral_skew_diffusion for one tracer (veros/core/isoneutral/diffusion.py:172-207,
  * 210-283, 298-307, iso=False): the same flux stencil with K_iso = 0 and
  * K_skew = K_gm, the explicit update of tracer[..., taup1] and dtr_iso, no

@@ -93,6 +93,10 @@ def test_iso_argument_errors_without_gpu():
     assert rc == _native.VX_ERR_INVALID_ARGUMENT  # no workspace
     rc = L.vx_iso_diffusion_ts_f64(None, g, p, 1, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1)
     assert rc == _native.VX_ERR_INVALID_ARGUMENT  # temp and salt alias
+    rc = L.vx_iso_pre_diffusion_ts_f64(None, g, p, 1, 2, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, None, None, None, None)
+    assert rc == _native.VX_ERR_INVALID_ARGUMENT  # no workspace
+    rc = L.vx_iso_pre_diffusion_ts_f64(None, g, p, 1, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, None, None, None, 1)
+    assert rc == _native.VX_ERR_INVALID_ARGUMENT  # temp and salt alias
     rc = L.vx_iso_skew_diffusion_f64(None, g, p, 1, 0, 1, None, 1, 1, 1, 1, 1, 1, 1, None, None, 1)
     assert rc == _native.VX_ERR_INVALID_ARGUMENT  # null K_gm
     assert L.vx_iso_adjoint_workspace_bytes(g) == 4 * 8 * 8 * 8 * 8

@@ -168,3 +168,42 @@ def test_shape_and_dtype_errors():
     ds.settings.eq_of_state_type = 3
     with pytest.raises(NotImplementedError):
         isoneutral.isoneutral_diffusion_pre(ds)
+
+
+@pytest.mark.parametrize("energy", [True, False])
+@pytest.mark.parametrize("shape,cyclic", [((31, 23, 19), True), ((14, 17, 3), False), ((9, 8, 2), True),
+                                          ((18, 11, 96), True)])
+def test_pre_and_temp_salt_fused_matches_three_calls(shape, cyclic, energy):
+    """isoneutral_pre_and_diffusion_temp_salt == isoneutral_diffusion_pre, then
+    isoneutral_diffusion(temp), then (salt): every output, value for value, and
+    the oracle's sequence to 1e-12."""
+    from veros_amd import isoneutral
+    st = _random_state(11 + sum(shape), *shape, cyclic=cyclic)
+    st.settings.enable_conserve_energy = energy
+    ds_a, ds_b = _dev_state(st), _dev_state(st)
+    isoneutral.isoneutral_diffusion_pre(ds_a)
+    isoneutral.isoneutral_diffusion_temp_salt(ds_a)
+    isoneutral.isoneutral_pre_and_diffusion_temp_salt(ds_b)
+    keys = PRE_KEYS + ("temp", "salt", "dtemp_iso", "dsalt_iso", "P_diss_iso")
+    for k in keys:
+        assert torch.equal(getattr(ds_a.variables, k), getattr(ds_b.variables, k)), k
+    pre = O.isoneutral_diffusion_pre(st)
+    for k, v in pre._asdict().items():
+        setattr(st.variables, k, v)
+    O.isoneutral_diffusion(st, "temp", True)
+    O.isoneutral_diffusion(st, "salt", False)
+    for k in keys:
+        assert rel_err(_np(getattr(ds_b.variables, k)), getattr(st.variables, k)) <= TOL, k
+
+
+def test_pre_and_temp_salt_fused_config1():
+    """The 360x160x60 channel through the fused call against pre + two calls."""
+    from veros_amd import isoneutral
+    st = W.iso_config1()
+    ds_a, ds_b = _dev_state(st), _dev_state(st)
+    isoneutral.isoneutral_diffusion_pre(ds_a)
+    isoneutral.isoneutral_diffusion(ds_a, ds_a.variables.temp, True)
+    isoneutral.isoneutral_diffusion(ds_a, ds_a.variables.salt, False)
+    isoneutral.isoneutral_pre_and_diffusion_temp_salt(ds_b)
+    for k in PRE_KEYS + ("temp", "salt", "dtemp_iso", "dsalt_iso", "P_diss_iso"):
+        assert torch.equal(getattr(ds_a.variables, k), getattr(ds_b.variables, k)), k

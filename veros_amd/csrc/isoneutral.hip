@@ -249,6 +249,55 @@ __device__ __forceinline__ Cols cols_of(const double* staged, int nz) {
 // gradients are formed from them in registers. Offsets are 32-bit element
 // offsets (the host checks the arrays fit).
 // ---------------------------------------------------------------------------
+// Face-flux arithmetic (veros/core/isoneutral/diffusion.py:22-111), shared by
+// the flux kernel and the fused pre + flux kernel so both produce the same
+// values. A01/A23 are the (ip=0, kr=0..1) / (ip=1, kr=0..1) entries of Ai_ez or
+// Ai_nz; t0* the tracer at the cell (u = k+1, d = k-1, clamped at the ends),
+// t1* at the east or north neighbour.
+__device__ __forceinline__ double east_flux_core(double diffloc, double2 A01, double2 A23, double t0, double t0u,
+                                                 double t0d, double t1, double t1u, double t1d, double dzt,
+                                                 double cost, double dxu, double K11) {
+    double sumz = 0.0;
+    sumz = sumz + diffloc * A01.x * (t0 - t0d);  // kr=0, ip=0
+    sumz = sumz + diffloc * A23.x * (t1 - t1d);  // kr=0, ip=1
+    sumz = sumz + diffloc * A01.y * (t0u - t0);  // kr=1, ip=0
+    sumz = sumz + diffloc * A23.y * (t1u - t1);  // kr=1, ip=1
+    return sumz / (4.0 * dzt) + (t1 - t0) / (cost * dxu) * K11;
+}
+__device__ __forceinline__ double north_flux_core(double diffloc, double2 A01, double2 A23, double t0, double t0u,
+                                                  double t0d, double t1, double t1u, double t1d, double dzt,
+                                                  double cosu, double dyu, double K22) {
+    double sumz = 0.0;
+    sumz = sumz + diffloc * A01.x * (t0 - t0d);
+    sumz = sumz + diffloc * A23.x * (t1 - t1d);
+    sumz = sumz + diffloc * A01.y * (t0u - t0);
+    sumz = sumz + diffloc * A23.y * (t1u - t1);
+    return cosu * (sumz / (4.0 * dzt) + (t1 - t0) / dyu * K22);
+}
+// tc/tw/te/ts/tn = the tracer at the cell and its west/east/south/north
+// neighbours, levels k (0) and k+1 (1); cosu_s = cosu[j-1]
+__device__ __forceinline__ double top_flux_core(double diffloc, double2 B01, double2 B23, double2 C01, double2 C23,
+                                                double tc0, double tc1, double tw0, double tw1, double te0,
+                                                double te1, double ts0, double ts1, double tn0, double tn1,
+                                                double cost, double cosu, double cosu_s, double dxt, double dyt) {
+    double sumx = 0.0;
+    sumx = sumx + diffloc * B01.x / cost * (tc0 - tw0);  // ip=0, kr=0
+    sumx = sumx + diffloc * B01.y / cost * (tc1 - tw1);  // ip=0, kr=1
+    sumx = sumx + diffloc * B23.x / cost * (te0 - tc0);  // ip=1, kr=0
+    sumx = sumx + diffloc * B23.y / cost * (te1 - tc1);  // ip=1, kr=1
+    double sumy = 0.0;
+    sumy = sumy + diffloc * C01.x * cosu_s * (tc0 - ts0);
+    sumy = sumy + diffloc * C01.y * cosu_s * (tc1 - ts1);
+    sumy = sumy + diffloc * C23.x * cosu * (tn0 - tc0);
+    sumy = sumy + diffloc * C23.y * cosu * (tn1 - tc1);
+    return sumx / (4 * dxt) + sumy / (4 * dyt * cost);
+}
+
+// Face-flux outputs of the fused pre + flux kernel (temperature, salinity).
+struct PreFlux {
+    double *fe[2], *fn[2], *ft[2];
+};
+
 // The dry cells of isoneutral_diffusion_pre. A dry cell has every output
 // exactly zero: its own maskT factor is in maskU (Ai_ez, K_11), maskV (Ai_nz,
 // K_22) and maskW (Ai_bx, Ai_by, K_33), which multiply each entry and each
@@ -299,12 +348,14 @@ __global__ __launch_bounds__(256) void iso_pre_dry_kernel(Grid g, double* __rest
     }
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_PRE_MIN_WAVES))) void iso_pre_kernel(
-    Grid g, Prm p, Fields f, const double* __restrict__ K,
-                                                      double* __restrict__ Aez, double* __restrict__ Anz,
-                                                      double* __restrict__ Abx, double* __restrict__ Aby,
-                                                      double* __restrict__ K11, double* __restrict__ K22,
-                                                      double* __restrict__ K33) {
+// FUSED: also the face fluxes of temperature and salinity at level tau
+// from the tensor just formed (the flux kernel's work for the step that
+// follows pre), into `fo`.
+template <bool FUSED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FUSED ? 2 : VX_PRE_MIN_WAVES))) void iso_pre_kernel(
+    Grid g, Prm p, Fields f, const double* __restrict__ K, double* __restrict__ Aez, double* __restrict__ Anz,
+    double* __restrict__ Abx, double* __restrict__ Aby, double* __restrict__ K11, double* __restrict__ K22,
+    double* __restrict__ K33, PreFlux fo) {
     // Wet cells only (the dry ones are written by iso_pre_dry_kernel).
     // Block-local compaction: a block owns the columns (i, jb .. jb+cpb-1);
     // its wet cells (the wet suffixes of those columns) are handed to its
@@ -444,7 +495,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_PRE_MIN_
         } else {
             Aez[4u * c + 3] = out[3];
         }
-        K11[c] = sumz / (4.0 * dzt_k);
+        const double k11v = sumz / (4.0 * dzt_k);
+        K11[c] = k11v;
+        if constexpr (FUSED) {
+            // at level 0 the kr = 0 entries keep their values (their tracer
+            // differences vanish there)
+            const double a0 = hm ? out[0] : Aez[4u * c], a2 = hm ? out[2] : Aez[4u * c + 2];
+            const double2 A01 = make_double2(a0, out[1]), A23 = make_double2(a2, out[3]);
+            fo.fe[0][c] = east_flux_core(diffloc, A01, A23, Tc, hp ? Tcp : Tc, hm ? Tcm : Tc, Te, hp ? Tep : Te,
+                                         hm ? Tem : Te, dzt_k, cost_j, dxu_i, k11v);
+            fo.fe[1][c] = east_flux_core(diffloc, A01, A23, Sc, hp ? Scp : Sc, hm ? Scm : Sc, Se, hp ? Sep : Se,
+                                         hm ? Sem : Se, dzt_k, cost_j, dxu_i, k11v);
+        }
     }
 
     // --- north face (i, j+1/2, k), i in [2, X-2)   (:134-168)
@@ -480,7 +542,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_PRE_MIN_
             Anz[4u * c + 1] = out[1];
             Anz[4u * c + 3] = out[3];
         }
-        K22[c] = sumz / (4.0 * dzt_k);
+        const double k22v = sumz / (4.0 * dzt_k);
+        K22[c] = k22v;
+        if constexpr (FUSED) {
+            const double a0 = hm ? out[0] : Anz[4u * c], a2 = hm ? out[2] : Anz[4u * c + 2];
+            const double2 A01 = make_double2(a0, out[1]), A23 = make_double2(a2, out[3]);
+            fo.fn[0][c] = north_flux_core(diffloc, A01, A23, Tc, hp ? Tcp : Tc, hm ? Tcm : Tc, Tn, hp ? Tnp : Tn,
+                                          hm ? Tnm : Tn, dzt_k, cosu_j, dyu_j, k22v);
+            fo.fn[1][c] = north_flux_core(diffloc, A01, A23, Sc, hp ? Scp : Sc, hm ? Scm : Sc, Sn, hp ? Snp : Sn,
+                                          hm ? Snm : Sn, dzt_k, cosu_j, dyu_j, k22v);
+        }
     }
 
     // --- top face (i, j, k+1/2), i, j in [2, X/Y-2), k < nz-1   (:170-225)
@@ -535,6 +606,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VX_PRE_MIN_
         *reinterpret_cast<double2*>(Aby + 4u * c) = make_double2(oy[0], oy[1]);
         *reinterpret_cast<double2*>(Aby + 4u * c + 2) = make_double2(oy[2], oy[3]);
         K33[c] = sumx / (4 * dxt_i) + sumy / (4 * dyt_j * cost_j);
+        if constexpr (FUSED) {
+            const double2 B01 = make_double2(ox[0], ox[1]), B23 = make_double2(ox[2], ox[3]);
+            const double2 C01 = make_double2(oy[0], oy[1]), C23 = make_double2(oy[2], oy[3]);
+            fo.ft[0][c] = top_flux_core(Kc, B01, B23, C01, C23, Tc, Tcp, Tw, Twp, Te, Tep, Ts, Tsp, Tn, Tnp, cost_j,
+                                        cosu_j, cosu_s, dxt_i, dyt_j);
+            fo.ft[1][c] = top_flux_core(Kc, B01, B23, C01, C23, Sc, Scp, Sw, Swp, Se, Sep, Ss, Ssp, Sn, Snp, cost_j,
+                                        cosu_j, cosu_s, dxt_i, dyt_j);
+        }
+    }
+    if constexpr (FUSED) {
+        if (i >= 2 && j >= 2 && !hp) {  // no flux through the top level
+            fo.ft[0][c] = 0.0;
+            fo.ft[1][c] = 0.0;
+        }
     }
 }
 
@@ -605,14 +690,8 @@ __device__ __forceinline__ double flux_east(const FaceIn& in, const Cols& tc, in
     const int up = (k + 1 < nz) ? 1 : 0, dn = hm ? 1 : 0;  // pad_z_edges clamping
     const double diffloc = hm ? 0.25 * (k1_of<ISO>(in.K_c) + k1_of<ISO>(in.K_cm) + k1_of<ISO>(in.K_e) + k1_of<ISO>(in.K_em))
                               : 0.5 * (k1_of<ISO>(in.K_c) + k1_of<ISO>(in.K_e));
-    const double t0 = tc.c[k], t0u = tc.c[k + up], t0d = tc.c[k - dn];
-    const double t1 = tc.e[k], t1u = tc.e[k + up], t1d = tc.e[k - dn];
-    double sumz = 0.0;
-    sumz = sumz + diffloc * in.ez01.x * (t0 - t0d);  // kr=0, ip=0
-    sumz = sumz + diffloc * in.ez23.x * (t1 - t1d);  // kr=0, ip=1
-    sumz = sumz + diffloc * in.ez01.y * (t0u - t0);  // kr=1, ip=0
-    sumz = sumz + diffloc * in.ez23.y * (t1u - t1);  // kr=1, ip=1
-    return sumz / (4.0 * in.dzt) + (t1 - t0) / (in.cost * in.dxu) * in.K11;
+    return east_flux_core(diffloc, in.ez01, in.ez23, tc.c[k], tc.c[k + up], tc.c[k - dn], tc.e[k], tc.e[k + up],
+                          tc.e[k - dn], in.dzt, in.cost, in.dxu, in.K11);
 }
 
 // flux_north at (i, j, k), i in [2, X-3], j in [1, Y-3]   (:49-77)
@@ -622,36 +701,16 @@ __device__ __forceinline__ double flux_north(const FaceIn& in, const Cols& tc, i
     const int up = (k + 1 < nz) ? 1 : 0, dn = hm ? 1 : 0;
     const double diffloc = hm ? 0.25 * (k1_of<ISO>(in.K_c) + k1_of<ISO>(in.K_cm) + k1_of<ISO>(in.K_n) + k1_of<ISO>(in.K_nm))
                               : 0.5 * (k1_of<ISO>(in.K_c) + k1_of<ISO>(in.K_n));
-    const double t0 = tc.c[k], t0u = tc.c[k + up], t0d = tc.c[k - dn];
-    const double t1 = tc.n[k], t1u = tc.n[k + up], t1d = tc.n[k - dn];
-    double sumz = 0.0;
-    sumz = sumz + diffloc * in.nz01.x * (t0 - t0d);
-    sumz = sumz + diffloc * in.nz23.x * (t1 - t1d);
-    sumz = sumz + diffloc * in.nz01.y * (t0u - t0);
-    sumz = sumz + diffloc * in.nz23.y * (t1u - t1);
-    return in.cosu * (sumz / (4.0 * in.dzt) + (t1 - t0) / in.dyu * in.K22);
+    return north_flux_core(diffloc, in.nz01, in.nz23, tc.c[k], tc.c[k + up], tc.c[k - dn], tc.n[k], tc.n[k + up],
+                           tc.n[k - dn], in.dzt, in.cosu, in.dyu, in.K22);
 }
 
 // flux_top at an interior (i, j, k), k < nz-1   (:79-111)
 template <bool ISO>
 __device__ __forceinline__ double flux_top(const FaceIn& in, const Cols& tc, int k) {
-    const double diffloc = k2_of<ISO>(in.K_c);
-    const double tc0 = tc.c[k], tc1 = tc.c[k + 1];
-    const double tw0 = tc.w[k], tw1 = tc.w[k + 1];
-    const double te0 = tc.e[k], te1 = tc.e[k + 1];
-    const double ts0 = tc.s[k], ts1 = tc.s[k + 1];
-    const double tn0 = tc.n[k], tn1 = tc.n[k + 1];
-    double sumx = 0.0;
-    sumx = sumx + diffloc * in.bx01.x / in.cost * (tc0 - tw0);  // ip=0, kr=0
-    sumx = sumx + diffloc * in.bx01.y / in.cost * (tc1 - tw1);  // ip=0, kr=1
-    sumx = sumx + diffloc * in.bx23.x / in.cost * (te0 - tc0);  // ip=1, kr=0
-    sumx = sumx + diffloc * in.bx23.y / in.cost * (te1 - tc1);  // ip=1, kr=1
-    double sumy = 0.0;
-    sumy = sumy + diffloc * in.by01.x * in.cosu_s * (tc0 - ts0);
-    sumy = sumy + diffloc * in.by01.y * in.cosu_s * (tc1 - ts1);
-    sumy = sumy + diffloc * in.by23.x * in.cosu * (tn0 - tc0);
-    sumy = sumy + diffloc * in.by23.y * in.cosu * (tn1 - tc1);
-    return sumx / (4 * in.dxt) + sumy / (4 * in.dyt * in.cost);
+    return top_flux_core(k2_of<ISO>(in.K_c), in.bx01, in.bx23, in.by01, in.by23, tc.c[k], tc.c[k + 1], tc.w[k],
+                         tc.w[k + 1], tc.e[k], tc.e[k + 1], tc.s[k], tc.s[k + 1], tc.n[k], tc.n[k + 1], in.cost,
+                         in.cosu, in.cosu_s, in.dxt, in.dyt);
 }
 
 // Fluxes of NTR tracers, thread per cell (i, j) in [1, X-2) x [1, Y-2):
@@ -740,7 +799,10 @@ __device__ __forceinline__ double fn_at(const Grid& g, const TracerArgs& t, int 
     return (i >= 2 && i <= g.X - 3 && j >= 1 && j <= g.Y - 3) ? t.fn[o] : 0.0;
 }
 
-template <int TC, int NTR, bool IMPLICIT>
+// WETFLUX: the face fluxes exist for wet cells only (written by the fused pre
+// + flux kernel from the tensor it formed, in which every face touching a dry
+// cell carries a zero flux); a face whose other cell is dry reads as zero.
+template <int TC, int NTR, bool IMPLICIT, bool WETFLUX>
 __global__ __launch_bounds__(TC * kMaxNz) void iso_step_tile_kernel(Grid g, Prm p, StepArgs a) {
     extern __shared__ double lds[];
     const int nz = g.nz;
@@ -770,6 +832,11 @@ __global__ __launch_bounds__(TC * kMaxNz) void iso_step_tile_kernel(Grid g, Prm 
     // (flux_top at k = kbot-2) to the sweep's rhs.
     const bool wc = valid && wet(kb, k);
     const bool wu = valid && k + 1 < nz && wet(kb, k + 1);
+    bool west_ok = true, south_ok = true;
+    if constexpr (WETFLUX) {
+        west_ok = wc && wet(g.kbot[(i - 1) * g.Y + j], k);
+        south_ok = wc && wet(g.kbot[i * g.Y + j - 1], k);
+    }
 
     // ---- every global read of the step first: the block then waits on memory
     // once instead of once per phase
@@ -782,9 +849,9 @@ __global__ __launch_bounds__(TC * kMaxNz) void iso_step_tile_kernel(Grid g, Prm 
         Ic[t] = Ie[t] = Iw[t] = In[t] = Is[t] = Iu[t] = trp[t] = dtr0[t] = 0.0;
         if (wc) {
             fe_c[t] = fe_at(g, ta, i, j, c);
-            fe_w[t] = fe_at(g, ta, i - 1, j, c - st.si);
+            fe_w[t] = west_ok ? fe_at(g, ta, i - 1, j, c - st.si) : 0.0;
             fn_c[t] = fn_at(g, ta, i, j, c);
-            fn_s[t] = fn_at(g, ta, i, j - 1, c - st.sj);
+            fn_s[t] = south_ok ? fn_at(g, ta, i, j - 1, c - st.sj) : 0.0;
             if (interior) {
                 trp[t] = ta.tr[3u * c + a.taup1];
                 dtr0[t] = ta.dtr[c];
@@ -799,12 +866,12 @@ __global__ __launch_bounds__(TC * kMaxNz) void iso_step_tile_kernel(Grid g, Prm 
                 if (interior && k < nz - 1) Iu[t] = I[3u * (c + 1)];
             }
         }
-        if (interior && (wc || wu)) ft_c[t] = ta.ft[c];
+        if (interior && (WETFLUX ? wc : (wc || wu))) ft_c[t] = ta.ft[c];
     }
     double K33k = 0.0, K33m = 0.0;
     if (IMPLICIT && interior && wc) {
         K33k = a.K33[c];
-        if (k >= 1) K33m = a.K33[c - 1];
+        if (k >= kb) K33m = a.K33[c - 1];  // the edge row k = kbot-1 has no sub-diagonal
     }
     const double P0 = (energy && wc) ? a.P[c] : 0.0;
     double dxt_i = 0.0, cost_j = 1.0, dyt_j = 0.0;
@@ -1229,6 +1296,28 @@ AuxStream* aux_stream() {
     return &(per_device[dev] = a);
 }
 
+// Work launched on the returned stream runs after everything enqueued on `s`
+// so far; `s` itself when no helper stream can be had. The fork record and
+// the wait on it form one step under the lock, so calls from other host
+// threads cannot slip their own fork in between.
+hipStream_t fork_aux(hipStream_t s) {
+    std::lock_guard<std::mutex> lk(aux_mutex());
+    AuxStream* ax = aux_stream();
+    if (ax && hipEventRecord(ax->fork, s) == hipSuccess && hipStreamWaitEvent(ax->s, ax->fork, 0) == hipSuccess)
+        return ax->s;
+    return s;
+}
+// Work enqueued on `s` from here on runs after everything enqueued on the
+// helper stream `sd` (a no-op when `sd` is `s`).
+int join_aux(hipStream_t s, hipStream_t sd, const char* who) {
+    if (sd == s) return VX_OK;
+    std::lock_guard<std::mutex> lk(aux_mutex());
+    AuxStream* ax = aux_stream();
+    if (!ax || hipEventRecord(ax->join, sd) != hipSuccess || hipStreamWaitEvent(s, ax->join, 0) != hipSuccess)
+        return fail(VX_ERR_LAUNCH, "%s: joining the helper stream failed", who);
+    return VX_OK;
+}
+
 unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 unsigned blocks_xcd(int64_t n, int bs) { return (unsigned)(((n + bs - 1) / bs + 7) / 8 * 8); }
 
@@ -1244,47 +1333,6 @@ dim3 cell_grid(int ni, int nj, dim3 blk) {
 }  // namespace
 }  // namespace vx
 
-extern "C" int vx_iso_pre_f64(void* stream, const vx_grid* grid, const vx_iso_params* prm, const double* temp,
-                              const double* salt, int32_t tau, const double* K_iso, double* Ai_ez, double* Ai_nz,
-                              double* Ai_bx, double* Ai_by, double* K_11, double* K_22, double* K_33) {
-    using namespace vx;
-    Grid g;
-    int rc = make_grid(grid, g, "vx_iso_pre_f64");
-    if (rc) return rc;
-    if (!prm) return fail(VX_ERR_INVALID_ARGUMENT, "vx_iso_pre_f64: null params");
-    if (tau < 0 || tau > 2) return fail(VX_ERR_INVALID_ARGUMENT, "vx_iso_pre_f64: tau must be 0..2");
-    if (!temp || !salt || !K_iso || !Ai_ez || !Ai_nz || !Ai_bx || !Ai_by || !K_11 || !K_22 || !K_33)
-        return fail(VX_ERR_INVALID_ARGUMENT, "vx_iso_pre_f64: null array pointer");
-    hipStream_t s = (hipStream_t)stream;
-    Fields f{temp, salt, tau};
-    const Prm p = make_prm(prm);
-    if ((int64_t)g.X * g.Y * g.nz * 4 >= ((int64_t)1 << 32))
-        return fail(VX_ERR_UNSUPPORTED, "vx_iso_pre_f64: grid too large for 32-bit offsets");
-    const dim3 blk = cell_block(g.nz);
-    const int64_t ncol = (int64_t)(g.X - 3) * (g.Y - 3);  // one wave per column, 4 per block
-    // The dry-cell zero fill (store-bound) and the wet-cell slopes (latency-
-    // bound) write disjoint cells: the zero fill runs on a helper stream forked
-    // from and joined back into `s`, so the call stays ordered on `s`.
-    std::lock_guard<std::mutex> lk(aux_mutex());
-    AuxStream* ax = aux_stream();
-    hipStream_t sd = s;
-    if (ax && hipEventRecord(ax->fork, s) == hipSuccess && hipStreamWaitEvent(ax->s, ax->fork, 0) == hipSuccess)
-        sd = ax->s;
-    hipLaunchKernelGGL(iso_pre_dry_kernel, dim3(blocks_for(ncol, 4)), dim3(256), 0, sd, g, Ai_ez, Ai_nz, Ai_bx, Ai_by,
-                       K_11, K_22, K_33);
-    if ((rc = check_launch("vx_iso_pre_f64 (dry cells)"))) return rc;
-    const size_t pre_lds = (size_t)(blk.y + 1) * sizeof(int);
-    hipLaunchKernelGGL(iso_pre_kernel, cell_grid(g.X - 3, g.Y - 3, blk), blk, pre_lds, s, g, p, f, K_iso, Ai_ez, Ai_nz,
-                       Ai_bx, Ai_by, K_11, K_22, K_33);
-    if ((rc = check_launch("vx_iso_pre_f64"))) return rc;
-    hipLaunchKernelGGL(k33_top_zero_kernel, dim3(blocks_for((int64_t)g.X * g.Y, 256)), dim3(256), 0, s, g, K_33);
-    if ((rc = check_launch("vx_iso_pre_f64 (K_33 top)"))) return rc;
-    if (sd != s) {
-        if (hipEventRecord(ax->join, sd) != hipSuccess || hipStreamWaitEvent(s, ax->join, 0) != hipSuccess)
-            return fail(VX_ERR_LAUNCH, "vx_iso_pre_f64: joining the helper stream failed");
-    }
-    return VX_OK;
-}
 
 
 namespace vx {
@@ -1305,7 +1353,7 @@ int launch_step(const Grid& g, const Prm& p, const StepArgs& a, hipStream_t s, c
     if (rc) return rc;
     const dim3 blk(g.nz, TC);
     const size_t lds = (size_t)(3 + 4 * NTR) * TC * g.nz * sizeof(double);
-    hipLaunchKernelGGL((iso_step_tile_kernel<TC, NTR, IMPLICIT>), cell_grid(g.X - 2, g.Y - 2, blk), blk, lds, s, g,
+    hipLaunchKernelGGL((iso_step_tile_kernel<TC, NTR, IMPLICIT, false>), cell_grid(g.X - 2, g.Y - 2, blk), blk, lds, s, g,
                        p, a);
     return check_launch(who);
 }
@@ -1326,6 +1374,35 @@ bool tensor_ok(const double* Ai_ez, const double* Ai_nz, const double* Ai_bx, co
 }
 
 // three flux arrays per tracer
+// isoneutral_diffusion_pre on `s`: the dry-cell zero fill on a helper stream
+// (forked from and joined back into `s`), the wet-cell kernel and the top
+// level of K_33 on `s`; FUSED also writes the step's face fluxes into `fo`.
+// The dry-cell zero fill (store-bound) and the wet-cell slopes (latency-bound)
+// write disjoint cells: the zero fill runs on a helper stream forked from `s`.
+// With `dry_stream` null it is joined back into `s` before returning; else the
+// helper stream goes to *dry_stream and the caller joins it (join_aux) once
+// nothing more it enqueues on `s` reads the dry cells' outputs.
+template <bool FUSED>
+int launch_pre(const Grid& g, const Prm& p, const Fields& f, const double* K_iso, double* Ai_ez, double* Ai_nz,
+               double* Ai_bx, double* Ai_by, double* K_11, double* K_22, double* K_33, PreFlux fo, hipStream_t s,
+               const char* who, hipStream_t* dry_stream = nullptr) {
+    int rc = VX_OK;
+    const dim3 blk = cell_block(g.nz);
+    const int64_t ncol = (int64_t)(g.X - 3) * (g.Y - 3);  // one wave per column, 4 per block
+    const hipStream_t sd = fork_aux(s);
+    if (dry_stream) *dry_stream = sd;
+    hipLaunchKernelGGL(iso_pre_dry_kernel, dim3(blocks_for(ncol, 4)), dim3(256), 0, sd, g, Ai_ez, Ai_nz, Ai_bx, Ai_by,
+                       K_11, K_22, K_33);
+    if ((rc = check_launch(who))) return rc;
+    const size_t pre_lds = (size_t)(blk.y + 1) * sizeof(int);
+    hipLaunchKernelGGL(iso_pre_kernel<FUSED>, cell_grid(g.X - 3, g.Y - 3, blk), blk, pre_lds, s, g, p, f, K_iso, Ai_ez,
+                       Ai_nz, Ai_bx, Ai_by, K_11, K_22, K_33, fo);
+    if ((rc = check_launch(who))) return rc;
+    hipLaunchKernelGGL(k33_top_zero_kernel, dim3(blocks_for((int64_t)g.X * g.Y, 256)), dim3(256), 0, s, g, K_33);
+    if ((rc = check_launch(who))) return rc;
+    return dry_stream ? VX_OK : join_aux(s, sd, who);
+}
+
 void wire_workspace(const Grid& g, StepArgs& a, int ntr, void* ws) {
     const int64_t n3 = (int64_t)g.X * g.Y * g.nz;
     double* w = static_cast<double*>(ws);
@@ -1484,4 +1561,78 @@ extern "C" int vx_iso_diffusion_adjoint_f64(void* stream, const vx_grid* grid, c
     if ((rc = check_launch(who))) return rc;
     hipLaunchKernelGGL(iso_adj_scatter_kernel, cell_grid(g.X - 3, g.Y - 3, blk), blk, 0, s, g, a);
     return check_launch(who);
+}
+
+extern "C" int vx_iso_pre_diffusion_ts_f64(void* stream, const vx_grid* grid, const vx_iso_params* prm, double* temp,
+                                           double* salt, int32_t tau, int32_t taup1, const double* K_iso,
+                                           double* Ai_ez, double* Ai_nz, double* Ai_bx, double* Ai_by, double* K_11,
+                                           double* K_22, double* K_33, double* dtemp_iso, double* dsalt_iso,
+                                           const double* int_drhodT, const double* int_drhodS, double* P_diss,
+                                           void* workspace) {
+    using namespace vx;
+    const char* who = "vx_iso_pre_diffusion_ts_f64";
+    Grid g;
+    int rc = check_common(grid, prm, tau, taup1, g, who);
+    if (rc) return rc;
+    if (!temp || !salt || !K_iso || !tensor_ok(Ai_ez, Ai_nz, Ai_bx, Ai_by, K_11, K_22) || !K_33 || !dtemp_iso ||
+        !dsalt_iso)
+        return fail(VX_ERR_INVALID_ARGUMENT, "%s: null array pointer", who);
+    if (temp == salt) return fail(VX_ERR_INVALID_ARGUMENT, "%s: temp and salt must be distinct arrays", who);
+    const bool e = P_diss != nullptr;
+    if (e != (int_drhodT != nullptr) || e != (int_drhodS != nullptr))
+        return fail(VX_ERR_INVALID_ARGUMENT, "%s: P_diss, int_drhodT and int_drhodS must all be given or all be NULL",
+                    who);
+    if (!workspace) return fail(VX_ERR_INVALID_ARGUMENT, "%s: null workspace", who);
+    if ((int64_t)g.X * g.Y * g.nz * 4 >= ((int64_t)1 << 32))
+        return fail(VX_ERR_UNSUPPORTED, "%s: grid too large for 32-bit offsets", who);
+    if (g.nz > kMaxNz) return fail(VX_ERR_UNSUPPORTED, "%s: nz=%d too deep", who, g.nz);
+    StepArgs a{};
+    a.t[0] = {temp, dtemp_iso, int_drhodT};
+    a.t[1] = {salt, dsalt_iso, int_drhodS};
+    wire_workspace(g, a, 2, workspace);
+    a.tau = tau;
+    a.taup1 = taup1;
+    a.Kiso = K_iso;
+    a.Kskew = nullptr;
+    a.Aez = Ai_ez, a.Anz = Ai_nz, a.Abx = Ai_bx, a.Aby = Ai_by, a.K11 = K_11, a.K22 = K_22, a.K33 = K_33;
+    a.P = P_diss;
+    const Prm p = make_prm(prm);
+    hipStream_t s = (hipStream_t)stream;
+    const PreFlux fo{{a.t[0].fe, a.t[1].fe}, {a.t[0].fn, a.t[1].fn}, {a.t[0].ft, a.t[1].ft}};
+    // The step kernel reads no output of a dry cell (its fluxes come from the
+    // wet-cell kernel, and it reads K_33 only on wet cells), so the dry-cell
+    // zero fill overlaps the whole call and is joined at its end.
+    hipStream_t sd = s;
+    rc = launch_pre<true>(g, p, Fields{temp, salt, tau}, K_iso, Ai_ez, Ai_nz, Ai_bx, Ai_by, K_11, K_22, K_33, fo, s,
+                          who, &sd);
+    if (!rc) {
+        constexpr int TC = kTileColumns;
+        const dim3 blk(g.nz, TC);
+        const size_t lds = (size_t)(3 + 4 * 2) * TC * g.nz * sizeof(double);
+        hipLaunchKernelGGL((iso_step_tile_kernel<TC, 2, true, true>), cell_grid(g.X - 2, g.Y - 2, blk), blk, lds, s,
+                           g, p, a);
+        rc = check_launch(who);
+    }
+    const int rj = join_aux(s, sd, who);
+    return rc ? rc : rj;
+}
+
+extern "C" int vx_iso_pre_f64(void* stream, const vx_grid* grid, const vx_iso_params* prm, const double* temp,
+                              const double* salt, int32_t tau, const double* K_iso, double* Ai_ez, double* Ai_nz,
+                              double* Ai_bx, double* Ai_by, double* K_11, double* K_22, double* K_33) {
+    using namespace vx;
+    Grid g;
+    int rc = make_grid(grid, g, "vx_iso_pre_f64");
+    if (rc) return rc;
+    if (!prm) return fail(VX_ERR_INVALID_ARGUMENT, "vx_iso_pre_f64: null params");
+    if (tau < 0 || tau > 2) return fail(VX_ERR_INVALID_ARGUMENT, "vx_iso_pre_f64: tau must be 0..2");
+    if (!temp || !salt || !K_iso || !Ai_ez || !Ai_nz || !Ai_bx || !Ai_by || !K_11 || !K_22 || !K_33)
+        return fail(VX_ERR_INVALID_ARGUMENT, "vx_iso_pre_f64: null array pointer");
+    hipStream_t s = (hipStream_t)stream;
+    Fields f{temp, salt, tau};
+    const Prm p = make_prm(prm);
+    if ((int64_t)g.X * g.Y * g.nz * 4 >= ((int64_t)1 << 32))
+        return fail(VX_ERR_UNSUPPORTED, "vx_iso_pre_f64: grid too large for 32-bit offsets");
+    return launch_pre<false>(g, p, f, K_iso, Ai_ez, Ai_nz, Ai_bx, Ai_by, K_11, K_22, K_33, PreFlux{}, s,
+                             "vx_iso_pre_f64");
 }

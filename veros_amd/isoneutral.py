@@ -7,6 +7,10 @@
 * ``isoneutral_diffusion(state, tr, istemp)``, the routine that stores the
   kernel's outputs in ``state.variables`` (veros/core/isoneutral/diffusion.py:286-295)
 * ``isoneutral_skew_diffusion(state, tr, istemp)`` (veros/core/isoneutral/diffusion.py:298-307)
+* ``isoneutral_diffusion_temp_salt(state)``: the temperature then salinity
+  calls of veros/core/thermodynamics.py:431-432 in one pass
+* ``isoneutral_pre_and_diffusion_temp_salt(state)``: pre followed by both
+  calls (veros/core/thermodynamics.py:430-432) in one pass
 
 ``state`` is any object with ``.variables`` and ``.settings`` holding the veros
 variables and settings these functions read (a ``VerosState`` whose arrays
@@ -198,3 +202,36 @@ def isoneutral_diffusion_temp_salt(state):
 def isoneutral_skew_diffusion(state, tr, istemp):
     """Skew (GM) variant of the routine (veros/core/isoneutral/diffusion.py:298-307)."""
     _store(state, isoneutral_diffusion_kernel(state, tr, istemp, iso=False))
+
+
+def isoneutral_pre_and_diffusion_temp_salt(state):
+    """``isoneutral_diffusion_pre(state)``, then
+    ``isoneutral_diffusion(state, temp, True)`` and
+    ``isoneutral_diffusion(state, salt, False)`` (the sequence of
+    veros/core/thermodynamics.py:430-432), in place, in one pass: the face
+    fluxes are formed while the isoneutral tensor is computed. Same results
+    as the three calls."""
+    vs, st = state.variables, state.settings
+    X, Y, nz = vs.K_iso.shape
+    keep = []
+    g = grid_struct(vs, keep)
+    p = params_struct(st)
+    T = _check_array("temp", vs.temp, (X, Y, nz, 3))
+    S = _check_array("salt", vs.salt, (X, Y, nz, 3))
+    K = _check_array("K_iso", vs.K_iso, (X, Y, nz))
+    tens = [_check_array(n, getattr(vs, n), (X, Y, nz, 2, 2)) for n in ("Ai_ez", "Ai_nz", "Ai_bx", "Ai_by")]
+    tens += [_check_array(n, getattr(vs, n), (X, Y, nz)) for n in ("K_11", "K_22", "K_33")]
+    dT = _check_array("dtemp_iso", vs.dtemp_iso, (X, Y, nz))
+    dS = _check_array("dsalt_iso", vs.dsalt_iso, (X, Y, nz))
+    if st.enable_conserve_energy:
+        IT = _check_array("int_drhodT", vs.int_drhodT, (X, Y, nz, 3))
+        IS = _check_array("int_drhodS", vs.int_drhodS, (X, Y, nz, 3))
+        P = _check_array("P_diss_iso", vs.P_diss_iso, (X, Y, nz))
+    else:
+        IT = IS = P = None
+    L = _native.lib()
+    ws = workspace(L.vx_iso_workspace_bytes(g, 2), T.device)
+    rc = L.vx_iso_pre_diffusion_ts_f64(stream_handle(), g, p, ptr(T), ptr(S), int(vs.tau), int(vs.taup1), ptr(K),
+                                       *(ptr(t) for t in tens), ptr(dT), ptr(dS), ptr(IT), ptr(IS), ptr(P),
+                                       ptr(ws))
+    _native.check(rc, "isoneutral_diffusion_pre + isoneutral_diffusion (temp, salt)")
