@@ -180,8 +180,12 @@ struct CellIdx {
     bool valid;
 };
 
-__device__ __forceinline__ CellIdx cell_of(int i0, int ni, int j0, int nj) {
-    const int cpb = blockDim.y;
+// The block's row i and first column jb when blocks own cpb columns each.
+struct BlockCols {
+    int i, j;
+    bool valid;  // false for a padding block
+};
+__device__ __forceinline__ BlockCols block_cols(int i0, int ni, int j0, int nj, int cpb) {
     const int sw = kStripBlocks * cpb;                // strip width in columns
     const int nstrips = (nj + sw - 1) / sw;
     const int per_strip = ni * kStripBlocks;
@@ -190,11 +194,19 @@ __device__ __forceinline__ CellIdx cell_of(int i0, int ni, int j0, int nj) {
     const int rem = L - sidx * per_strip;
     const int ii = rem / kStripBlocks;
     const int b = rem - ii * kStripBlocks;
-    CellIdx r;
+    BlockCols r;
     r.i = i0 + ii;
-    r.j = j0 + sidx * sw + b * cpb + (int)threadIdx.y;
-    r.k = (int)threadIdx.x;
+    r.j = j0 + sidx * sw + b * cpb;
     r.valid = sidx < nstrips && r.j < j0 + nj;
+    return r;
+}
+__device__ __forceinline__ CellIdx cell_of(int i0, int ni, int j0, int nj) {
+    const BlockCols bc = block_cols(i0, ni, j0, nj, (int)blockDim.y);
+    CellIdx r;
+    r.i = bc.i;
+    r.j = bc.j + (int)threadIdx.y;
+    r.k = (int)threadIdx.x;
+    r.valid = bc.valid && r.j < j0 + nj;
     return r;
 }
 
@@ -324,67 +336,44 @@ __global__ __launch_bounds__(256) void iso_pre_dry_kernel(Grid g, double* __rest
                                                           double* __restrict__ K33) {
     const int lane = (int)(threadIdx.x & 63);
     const int64_t ni = g.X - 3, nj = g.Y - 3;
-    const int64_t col = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (col >= ni * nj) return;
-    const int i = 1 + (int)(col / nj), j = 1 + (int)(col - (col / nj) * nj);
-    const int nz = g.nz;
-    const int kb = g.kbot[i * g.Y + j];
-    const int nd = (kb >= 1 && kb <= nz) ? kb - 1 : nz;  // dry levels 0 .. nd-1
-    if (nd == 0) return;
-    const uint32_t c0 = ((uint32_t)i * (uint32_t)g.Y + (uint32_t)j) * (uint32_t)nz;
-    if (j >= 2) {
-        zero_ai_run(Aez + 4u * c0, nd, lane, true);
-        zero_run(K11 + c0, nd, lane);
-    }
-    if (i >= 2) {
-        zero_ai_run(Anz + 4u * c0, nd, lane, true);
-        zero_run(K22 + c0, nd, lane);
-    }
-    if (i >= 2 && j >= 2) {
-        const int nt = nd < nz - 1 ? nd : nz - 1;  // top faces: levels below nz-1
-        zero_ai_run(Abx + 4u * c0, nt, lane, false);
-        zero_ai_run(Aby + 4u * c0, nt, lane, false);
-        zero_run(K33 + c0, nt, lane);
+    const int64_t stride = (int64_t)gridDim.x * (blockDim.x >> 6);
+    // grid-stride over the columns (the grid may be smaller than one wave per
+    // column: a background fill beside other kernels)
+    for (int64_t col = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); col < ni * nj; col += stride) {
+        const int i = 1 + (int)(col / nj), j = 1 + (int)(col - (col / nj) * nj);
+        const int nz = g.nz;
+        const int kb = g.kbot[i * g.Y + j];
+        const int nd = (kb >= 1 && kb <= nz) ? kb - 1 : nz;  // dry levels 0 .. nd-1
+        if (nd == 0) continue;
+        const uint32_t c0 = ((uint32_t)i * (uint32_t)g.Y + (uint32_t)j) * (uint32_t)nz;
+        if (j >= 2) {
+            zero_ai_run(Aez + 4u * c0, nd, lane, true);
+            zero_run(K11 + c0, nd, lane);
+        }
+        if (i >= 2) {
+            zero_ai_run(Anz + 4u * c0, nd, lane, true);
+            zero_run(K22 + c0, nd, lane);
+        }
+        if (i >= 2 && j >= 2) {
+            const int nt = nd < nz - 1 ? nd : nz - 1;  // top faces: levels below nz-1
+            zero_ai_run(Abx + 4u * c0, nt, lane, false);
+            zero_ai_run(Aby + 4u * c0, nt, lane, false);
+            zero_run(K33 + c0, nt, lane);
+        }
     }
 }
 
-// FUSED: also the face fluxes of temperature and salinity at level tau
-// from the tensor just formed (the flux kernel's work for the step that
-// follows pre), into `fo`.
+// One wet cell (i, j, k) of isoneutral_diffusion_pre. FUSED: also the face
+// fluxes of temperature and salinity at level tau from the tensor just formed
+// (the flux kernel's work for the step that follows pre), into `fo`.
 template <bool FUSED>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FUSED ? 2 : VX_PRE_MIN_WAVES))) void iso_pre_kernel(
-    Grid g, Prm p, Fields f, const double* __restrict__ K, double* __restrict__ Aez, double* __restrict__ Anz,
-    double* __restrict__ Abx, double* __restrict__ Aby, double* __restrict__ K11, double* __restrict__ K22,
-    double* __restrict__ K33, PreFlux fo) {
-    // Wet cells only (the dry ones are written by iso_pre_dry_kernel).
-    // Block-local compaction: a block owns the columns (i, jb .. jb+cpb-1);
-    // its wet cells (the wet suffixes of those columns) are handed to its
-    // first threads, so the slope work fills whole waves.
-    extern __shared__ int pre_counts[];  // prefix sums of the columns' wet counts
-    const CellIdx ci0 = cell_of(1, g.X - 3, 1, g.Y - 3);
-    const int nz = g.nz, cpb = (int)blockDim.y;
-    const int jb = ci0.j - (int)threadIdx.y;
-    if (jb > g.Y - 3) return;  // padding block (uniform)
-    const int i = ci0.i;
-    const int tid = (int)(threadIdx.y * blockDim.x + threadIdx.x);
-    int* sW = pre_counts;
-    if (tid < cpb) {
-        const bool col_ok = jb + tid <= g.Y - 3;
-        const int kbq = col_ok ? g.kbot[i * g.Y + jb + tid] : 0;
-        sW[tid + 1] = (kbq >= 1 && kbq <= nz) ? nz - kbq + 1 : 0;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        sW[0] = 0;
-        for (int q = 0; q < cpb; ++q) sW[q + 1] += sW[q];
-    }
-    __syncthreads();
-    const int nwet = sW[cpb];
-    if (tid >= nwet) return;  // dry cells: iso_pre_dry_kernel
-    int q = 0;
-    while (sW[q + 1] <= tid) ++q;
-    const int k = nz - (sW[q + 1] - sW[q]) + (tid - sW[q]);  // wet suffix of column q
-    const int j = jb + q;
+__device__ __forceinline__ void pre_wet_cell(const Grid& g, const Prm& p, const Fields& f,
+                                             const double* __restrict__ K, double* __restrict__ Aez,
+                                             double* __restrict__ Anz, double* __restrict__ Abx,
+                                             double* __restrict__ Aby, double* __restrict__ K11,
+                                             double* __restrict__ K22, double* __restrict__ K33, const PreFlux& fo,
+                                             const int i, const int j, const int k) {
+    const int nz = g.nz;
     const uint32_t sj = (uint32_t)nz, si = (uint32_t)g.Y * (uint32_t)nz;
     const uint32_t c = (uint32_t)i * si + (uint32_t)j * sj + (uint32_t)k;
     const uint32_t ce = c + si, cw = c - si, cn = c + sj, cs = c - sj;
@@ -620,6 +609,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FUSED ? 2 :
             fo.ft[0][c] = 0.0;
             fo.ft[1][c] = 0.0;
         }
+    }
+}
+
+// The wet cells of isoneutral_diffusion_pre (the dry ones are written by
+// iso_pre_dry_kernel). Block-local compaction: a block owns the columns
+// (i, jb .. jb+ncb-1); their wet cells (the wet suffixes) are numbered in
+// column order and dealt round-robin to the block's threads, so the slope work
+// fills whole waves whatever the land fraction.
+template <bool FUSED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FUSED ? 2 : VX_PRE_MIN_WAVES))) void iso_pre_kernel(
+    Grid g, Prm p, Fields f, const double* __restrict__ K, double* __restrict__ Aez, double* __restrict__ Anz,
+    double* __restrict__ Abx, double* __restrict__ Aby, double* __restrict__ K11, double* __restrict__ K22,
+    double* __restrict__ K33, PreFlux fo, int ncb) {
+    extern __shared__ int pre_counts[];  // prefix sums of the columns' wet counts
+    const BlockCols bc = block_cols(1, g.X - 3, 1, g.Y - 3, ncb);
+    if (!bc.valid) return;  // padding block (uniform)
+    const int nz = g.nz, i = bc.i, jb = bc.j;
+    const int tid = (int)threadIdx.x, nthr = (int)blockDim.x;
+    int* sW = pre_counts;
+    for (int q = tid; q < ncb; q += nthr) {
+        const bool col_ok = jb + q <= g.Y - 3;
+        const int kbq = col_ok ? g.kbot[i * g.Y + jb + q] : 0;
+        sW[q + 1] = (kbq >= 1 && kbq <= nz) ? nz - kbq + 1 : 0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        sW[0] = 0;
+        for (int q = 0; q < ncb; ++q) sW[q + 1] += sW[q];
+    }
+    __syncthreads();
+    const int nwet = sW[ncb];
+    for (int t = tid; t < nwet; t += nthr) {
+        // the column q with sW[q] <= t < sW[q+1]: the last q with sW[q] <= t
+        int lo = 0, hi = ncb - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (sW[mid] <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const int k = nz - (sW[lo + 1] - sW[lo]) + (t - sW[lo]);  // wet suffix of column lo
+        pre_wet_cell<FUSED>(g, p, f, K, Aez, Anz, Abx, Aby, K11, K22, K33, fo, i, jb + lo, k);
     }
 }
 
@@ -1318,6 +1348,34 @@ int join_aux(hipStream_t s, hipStream_t sd, const char* who) {
     return VX_OK;
 }
 
+// Compute units of the current device (0 if it cannot be queried), cached
+// per device.
+int device_cu_count() {
+    static std::mutex m;
+    static std::map<int, int> per_device;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    std::lock_guard<std::mutex> lk(m);
+    auto it = per_device.find(dev);
+    if (it != per_device.end()) return it->second;
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) n = 0;
+    return per_device[dev] = n;
+}
+
+#ifndef VX_PRE_CELLS
+#define VX_PRE_CELLS 1920
+#endif
+// Columns per block of the wet-cell pre kernel: about VX_PRE_CELLS cells
+// (24 columns at nz = 80), so that a block's wet cells fill its 256 threads
+// for a few rounds at typical land fractions (measured on 1440x720x80 with 31%
+// wet cells: 240 cells per block 11.7 ms for pre + the temperature/salinity
+// step, 640 10.1 ms, 1920 9.8 ms).
+int pre_columns(int nz) {
+    const int n = (VX_PRE_CELLS + nz - 1) / nz;
+    return n < 1 ? 1 : (n > 64 ? 64 : n);
+}
+
 unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 unsigned blocks_xcd(int64_t n, int bs) { return (unsigned)(((n + bs - 1) / bs + 7) / 8 * 8); }
 
@@ -1387,16 +1445,30 @@ int launch_pre(const Grid& g, const Prm& p, const Fields& f, const double* K_iso
                double* Ai_bx, double* Ai_by, double* K_11, double* K_22, double* K_33, PreFlux fo, hipStream_t s,
                const char* who, hipStream_t* dry_stream = nullptr) {
     int rc = VX_OK;
-    const dim3 blk = cell_block(g.nz);
     const int64_t ncol = (int64_t)(g.X - 3) * (g.Y - 3);  // one wave per column, 4 per block
     const hipStream_t sd = fork_aux(s);
     if (dry_stream) *dry_stream = sd;
-    hipLaunchKernelGGL(iso_pre_dry_kernel, dim3(blocks_for(ncol, 4)), dim3(256), 0, sd, g, Ai_ez, Ai_nz, Ai_bx, Ai_by,
+    // Blocks per CU of the zero fill (0: one wave per column). Beside the
+    // whole fused call it runs in the background, one block per CU, at a
+    // modest store rate instead of flooding the memory system ahead of the
+    // wet-cell kernel; pre alone joins it at its end, so there it takes the
+    // full grid.
+#ifndef VX_DRY_BPC_PRE
+#define VX_DRY_BPC_PRE 0
+#endif
+#ifndef VX_DRY_BPC_FUSED
+#define VX_DRY_BPC_FUSED 1
+#endif
+    const unsigned bpc = dry_stream ? VX_DRY_BPC_FUSED : VX_DRY_BPC_PRE;
+    const unsigned dry_full = blocks_for(ncol, 4), ndry = bpc * (unsigned)device_cu_count();
+    const unsigned dry_grid = ndry > 0 && ndry < dry_full ? ndry : dry_full;
+    hipLaunchKernelGGL(iso_pre_dry_kernel, dim3(dry_grid), dim3(256), 0, sd, g, Ai_ez, Ai_nz, Ai_bx, Ai_by,
                        K_11, K_22, K_33);
     if ((rc = check_launch(who))) return rc;
-    const size_t pre_lds = (size_t)(blk.y + 1) * sizeof(int);
-    hipLaunchKernelGGL(iso_pre_kernel<FUSED>, cell_grid(g.X - 3, g.Y - 3, blk), blk, pre_lds, s, g, p, f, K_iso, Ai_ez,
-                       Ai_nz, Ai_bx, Ai_by, K_11, K_22, K_33, fo);
+    const int ncb = pre_columns(g.nz);
+    const size_t pre_lds = (size_t)(ncb + 1) * sizeof(int);
+    hipLaunchKernelGGL(iso_pre_kernel<FUSED>, cell_grid(g.X - 3, g.Y - 3, dim3(1, ncb)), dim3(256), pre_lds, s, g, p,
+                       f, K_iso, Ai_ez, Ai_nz, Ai_bx, Ai_by, K_11, K_22, K_33, fo, ncb);
     if ((rc = check_launch(who))) return rc;
     hipLaunchKernelGGL(k33_top_zero_kernel, dim3(blocks_for((int64_t)g.X * g.Y, 256)), dim3(256), 0, s, g, K_33);
     if ((rc = check_launch(who))) return rc;
