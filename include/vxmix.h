@@ -230,9 +230,12 @@ int vx_iso_diffusion_ts_f64(void* stream, const vx_grid* grid, const vx_iso_para
  * in one call: the outputs of vx_iso_pre_f64 followed by
  * vx_iso_diffusion_ts_f64 on the same arrays. The face fluxes of temperature
  * and salinity are formed from the isoneutral tensor while it is computed,
- * instead of reading it back. `workspace` as for vx_iso_diffusion_ts_f64
- * (vx_iso_workspace_bytes(grid, 2) bytes).
+ * instead of reading it back, and the tracer step runs over the wet cells
+ * only. `workspace` holds vx_iso_pre_diffusion_ts_workspace_bytes(grid)
+ * bytes (the two tracers' face fluxes and the wet-cell plan built from kbot).
  */
+int64_t vx_iso_pre_diffusion_ts_workspace_bytes(const vx_grid* grid);
+
 int vx_iso_pre_diffusion_ts_f64(void* stream, const vx_grid* grid, const vx_iso_params* prm,
                                 double* temp, double* salt, int32_t tau, int32_t taup1,
                                 const double* K_iso,

@@ -171,14 +171,18 @@ def test_shape_and_dtype_errors():
 
 
 @pytest.mark.parametrize("energy", [True, False])
-@pytest.mark.parametrize("shape,cyclic", [((31, 23, 19), True), ((14, 17, 3), False), ((9, 8, 2), True),
-                                          ((18, 11, 96), True)])
-def test_pre_and_temp_salt_fused_matches_three_calls(shape, cyclic, energy):
+@pytest.mark.parametrize("shape,cyclic,land", [((31, 23, 19), True, 0.25), ((14, 17, 3), False, 0.25),
+                                               ((9, 8, 2), True, 0.25), ((18, 11, 96), True, 0.25),
+                                               ((12, 10, 4), False, 0.25), ((15, 9, 20), True, 0.0),
+                                               ((11, 13, 7), False, 1.0), ((70, 9, 128), True, 0.3)])
+def test_pre_and_temp_salt_fused_matches_three_calls(shape, cyclic, land, energy):
     """isoneutral_pre_and_diffusion_temp_salt == isoneutral_diffusion_pre, then
     isoneutral_diffusion(temp), then (salt): every output, value for value, and
-    the oracle's sequence to 1e-12."""
+    the oracle's sequence to 1e-12. Covers a shallow grid, no land, all land (no wet
+    cell: the step has nothing to do) and the deepest supported grid (a group
+    of wet cells is then two full columns)."""
     from veros_amd import isoneutral
-    st = _random_state(11 + sum(shape), *shape, cyclic=cyclic)
+    st = _random_state(11 + sum(shape), *shape, cyclic=cyclic, land_frac=land)
     st.settings.enable_conserve_energy = energy
     ds_a, ds_b = _dev_state(st), _dev_state(st)
     isoneutral.isoneutral_diffusion_pre(ds_a)

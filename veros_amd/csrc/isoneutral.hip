@@ -1041,6 +1041,366 @@ __global__ __launch_bounds__(TC * kMaxNz) void iso_step_tile_kernel(Grid g, Prm 
 }
 
 // ---------------------------------------------------------------------------
+// Wet-cell tracer step of the fused pre + temperature/salinity call (the tile
+// kernel's work with the fused pre's wet-face fluxes). The wet columns of the
+// step's range [1, X-1) x [1, Y-1) are ranked in row-major order (rank r, a
+// column's wet cells being its levels kbot-1 .. nz-1), and their cells are
+// numbered in the same order (column r holds cells o_r .. o_r + w_r - 1). The
+// cells are cut into groups: group b takes the wet columns with
+// b*C <= o_r < (b+1)*C, C = kWetThreads - nz + 1, so a group holds at most
+// C + nz - 1 = kWetThreads cells (w_r <= nz). A block steps one group at a
+// time with a thread per cell, so no lane is spent on a dry cell; the plan is
+// built on the device from kbot by three small scan kernels.
+// ---------------------------------------------------------------------------
+constexpr int kWetThreads = 256;
+constexpr int kScanColsPerThread = 8;
+constexpr int kScanChunk = kWetThreads * kScanColsPerThread;  // columns per scan block
+
+struct WetPlan {
+    int* wetcol;  // [ncols]   column index (i-1)*(Y-2) + (j-1) of the wet column of rank r
+    int* wetoff;  // [ncols]   o_r, the number of wet cells before it
+    int* first;   // [ncols+2] first[b] = first rank of group b; first[ngroups] = number of wet columns
+    int* nb;      // [12]      ngroups, wet columns, wet cells, -, then a group counter per XCD
+    int2* chunk;  // [nchunks] (wet columns, wet cells) per scan chunk, then their exclusive sums
+};
+
+__device__ __forceinline__ int wet_count(const Grid& g, int64_t col, int64_t ncols) {
+    if (col >= ncols) return 0;
+    const int ny2 = g.Y - 2;
+    const int i = 1 + (int)(col / ny2), j = 1 + (int)(col - (col / ny2) * ny2);
+    const int kb = g.kbot[i * g.Y + j];
+    return (kb >= 1 && kb <= g.nz) ? g.nz - kb + 1 : 0;
+}
+
+// In-place inclusive sum over the block (Hillis-Steele; every thread calls it).
+__device__ __forceinline__ void block_scan2(int* s0, int* s1, int tid) {
+    for (int st = 1; st < kWetThreads; st <<= 1) {
+        const int a0 = tid >= st ? s0[tid - st] : 0, a1 = tid >= st ? s1[tid - st] : 0;
+        __syncthreads();
+        s0[tid] += a0;
+        s1[tid] += a1;
+        __syncthreads();
+    }
+}
+
+// per scan chunk of kScanChunk columns: (wet columns, wet cells)
+__global__ __launch_bounds__(kWetThreads) void wet_scan_count_kernel(Grid g, WetPlan w, int64_t ncols) {
+    __shared__ int s0[kWetThreads], s1[kWetThreads];
+    const int tid = (int)threadIdx.x;
+    const int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)tid * kScanColsPerThread;
+    int nc = 0, ncell = 0;
+    for (int q = 0; q < kScanColsPerThread; ++q) {
+        const int n = wet_count(g, base + q, ncols);
+        nc += n > 0 ? 1 : 0;
+        ncell += n;
+    }
+    s0[tid] = nc;
+    s1[tid] = ncell;
+    __syncthreads();
+    block_scan2(s0, s1, tid);
+    if (tid == kWetThreads - 1) w.chunk[blockIdx.x] = make_int2(s0[tid], s1[tid]);
+}
+
+// one block: exclusive sums of the chunk counts, and the totals
+__global__ __launch_bounds__(kWetThreads) void wet_scan_chunks_kernel(WetPlan w, int nchunks) {
+    __shared__ int s0[kWetThreads], s1[kWetThreads];
+    __shared__ int carry[2];
+    const int tid = (int)threadIdx.x;
+    if (tid == 0) carry[0] = carry[1] = 0;
+    __syncthreads();
+    for (int base = 0; base < nchunks; base += kWetThreads) {
+        const int q = base + tid;
+        const int2 v = q < nchunks ? w.chunk[q] : make_int2(0, 0);
+        s0[tid] = v.x;
+        s1[tid] = v.y;
+        __syncthreads();
+        block_scan2(s0, s1, tid);
+        const int c0 = carry[0], c1 = carry[1];
+        if (q < nchunks) w.chunk[q] = make_int2(c0 + s0[tid] - v.x, c1 + s1[tid] - v.y);
+        __syncthreads();
+        if (tid == kWetThreads - 1) {
+            carry[0] = c0 + s0[tid];
+            carry[1] = c1 + s1[tid];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        w.nb[0] = 0;  // set by wet_scan_columns_kernel when there is a wet column
+        w.nb[1] = carry[0];
+        w.nb[2] = carry[1];
+    }
+    if (tid < 8) w.nb[4 + tid] = 0;  // the step kernel's group counters
+}
+
+// ranks, offsets and the group starts
+__global__ __launch_bounds__(kWetThreads) void wet_scan_columns_kernel(Grid g, WetPlan w, int64_t ncols, int C) {
+    __shared__ int s0[kWetThreads], s1[kWetThreads];
+    const int tid = (int)threadIdx.x;
+    const int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)tid * kScanColsPerThread;
+    int n[kScanColsPerThread];
+    int nc = 0, ncell = 0;
+#pragma unroll
+    for (int q = 0; q < kScanColsPerThread; ++q) {
+        n[q] = wet_count(g, base + q, ncols);
+        nc += n[q] > 0 ? 1 : 0;
+        ncell += n[q];
+    }
+    s0[tid] = nc;
+    s1[tid] = ncell;
+    __syncthreads();
+    block_scan2(s0, s1, tid);
+    const int2 co = w.chunk[blockIdx.x];
+    int r = co.x + s0[tid] - nc, o = co.y + s1[tid] - ncell;
+    const int nwc = w.nb[1];
+#pragma unroll
+    for (int q = 0; q < kScanColsPerThread; ++q) {
+        if (n[q] > 0) {
+            w.wetcol[r] = (int)(base + q);
+            w.wetoff[r] = o;
+            if (r == 0) w.first[0] = 0;
+            // the group boundary m = b*C inside (o, o + n]: the columns before
+            // group b are ranks 0 .. r
+            const int m = (o / C + 1) * C;
+            if (m <= o + n[q]) w.first[m / C] = r + 1;
+            if (r == nwc - 1) {
+                const int ng = o / C + 1;
+                w.first[ng] = nwc;
+                w.nb[0] = ng;
+            }
+            ++r;
+            o += n[q];
+        }
+    }
+}
+
+// Persistent grid (a multiple of 8 blocks): the blocks of XCD x (blockIdx % 8)
+// share the groups [x, x+1) * ngroups / 8, so neighbouring rows share an L2,
+// and take them one at a time from that XCD's counter.
+#ifndef VX_WET_WAVES
+#define VX_WET_WAVES 5  // 96 VGPRs: five blocks per CU
+#endif
+template <int NTR>
+__global__ __launch_bounds__(kWetThreads) __attribute__((amdgpu_waves_per_eu(VX_WET_WAVES))) void iso_step_wet_kernel(
+    Grid g, Prm p, StepArgs a, WetPlan w) {
+    extern __shared__ double lds[];
+    constexpr int T = kWetThreads;
+    // LDS: rows A, B, C (shared by the tracers), then per tracer F (flux_top),
+    // D (lateral dissipation), R (rhs -> dp -> new), Q (cp), each T cells;
+    // then the group's column table
+    double* sA = lds;
+    double* sB = lds + T;
+    double* sC = lds + 2 * T;
+    double* sT = lds + 3 * T;
+    int* sOff = reinterpret_cast<int*>(lds + (3 + 4 * NTR) * T);  // [T+1] first cell of each column
+    int* sCol = sOff + (T + 1);                                   // [T]   its column index
+    const int nz = g.nz, ny2 = g.Y - 2;
+    const int tid = (int)threadIdx.x;
+    const int ngroups = w.nb[0];
+    const int xcd = (int)(blockIdx.x & 7);
+    const int gbeg = (int)((int64_t)ngroups * xcd / 8), gend = (int)((int64_t)ngroups * (xcd + 1) / 8);
+    const Strides st{(uint32_t)g.Y * (uint32_t)nz, (uint32_t)nz};
+    const bool energy = a.P != nullptr;
+    const double dt = p.dt_tracer;
+    __shared__ int sGrp;
+    for (;;) {
+        __syncthreads();  // the previous group's LDS (and sGrp) is no longer read
+        if (tid == 0) sGrp = gbeg + atomicAdd(&w.nb[4 + xcd], 1);
+        __syncthreads();
+        const int grp = sGrp;
+        if (grp >= gend) break;  // uniform: every block ends once its XCD's share is taken
+        const int r0 = w.first[grp], r1 = w.first[grp + 1];
+        const int ncolg = r1 - r0;
+        const int o0 = w.wetoff[r0];
+        if (tid < ncolg) {
+            sOff[tid] = w.wetoff[r0 + tid] - o0;
+            sCol[tid] = w.wetcol[r0 + tid];
+        }
+        if (tid == 0) sOff[ncolg] = (grp + 1 < ngroups ? w.wetoff[r1] : w.nb[2]) - o0;
+        __syncthreads();
+        const int ncell = sOff[ncolg];
+        const bool have = tid < ncell;
+        // this thread's cell: the column q with sOff[q] <= tid < sOff[q+1]
+        int q = 0;
+        if (have) {
+            int hi = ncolg - 1;
+            while (q < hi) {
+                const int mid = (q + hi + 1) >> 1;
+                if (sOff[mid] <= tid) q = mid;
+                else hi = mid - 1;
+            }
+        }
+        const int lo = tid;
+        const int col = have ? sCol[q] : 0;
+        const int i = 1 + col / ny2, j = 1 + (col - (col / ny2) * ny2);
+        const int kb = have ? nz - (sOff[q + 1] - sOff[q]) + 1 : 1;
+        const int k = kb - 1 + (tid - sOff[q]);
+        const bool interior = have && i >= 2 && i <= g.X - 3 && j >= 2 && j <= g.Y - 3;
+        const uint32_t c = (uint32_t)i * st.si + (uint32_t)j * st.sj + (uint32_t)k;
+        bool west_ok = false, south_ok = false;
+        if (have) {
+            west_ok = wet(g.kbot[(i - 1) * g.Y + j], k);
+            south_ok = wet(g.kbot[i * g.Y + j - 1], k);
+        }
+
+        // ---- every global read first
+        double fe_c[NTR], fe_w[NTR], fn_c[NTR], fn_s[NTR], ft_c[NTR];
+        double Ic[NTR], Ie[NTR], Iw[NTR], In[NTR], Is[NTR], Iu[NTR], trp[NTR], dtr0[NTR];
+#pragma unroll
+        for (int t = 0; t < NTR; ++t) {
+            const TracerArgs& ta = a.t[t];
+            fe_c[t] = fe_w[t] = fn_c[t] = fn_s[t] = ft_c[t] = 0.0;
+            Ic[t] = Ie[t] = Iw[t] = In[t] = Is[t] = Iu[t] = trp[t] = dtr0[t] = 0.0;
+            if (have) {
+                fe_c[t] = fe_at(g, ta, i, j, c);
+                fe_w[t] = west_ok ? fe_at(g, ta, i - 1, j, c - st.si) : 0.0;
+                fn_c[t] = fn_at(g, ta, i, j, c);
+                fn_s[t] = south_ok ? fn_at(g, ta, i, j - 1, c - st.sj) : 0.0;
+                if (interior) {
+                    trp[t] = ta.tr[3u * c + a.taup1];
+                    dtr0[t] = ta.dtr[c];
+                    ft_c[t] = ta.ft[c];
+                }
+                if (energy) {
+                    const double* __restrict__ I = ta.I + a.tau;
+                    Ic[t] = I[3u * c];
+                    Ie[t] = I[3u * (c + st.si)];
+                    Iw[t] = I[3u * (c - st.si)];
+                    In[t] = I[3u * (c + st.sj)];
+                    Is[t] = I[3u * (c - st.sj)];
+                    if (interior && k < nz - 1) Iu[t] = I[3u * (c + 1)];
+                }
+            }
+        }
+        double K33k = 0.0, K33m = 0.0;
+        if (interior) {
+            K33k = a.K33[c];
+            if (k >= kb) K33m = a.K33[c - 1];  // the edge row k = kbot-1 has no sub-diagonal
+        }
+        const double P0 = (energy && have) ? a.P[c] : 0.0;
+        double dxt_i = 0.0, cost_j = 1.0, dyt_j = 0.0, dzt_k = 1.0, dzw_k = 1.0, dzw_lo = 1.0;
+        if (have) {
+            dxt_i = g.dxt[i];
+            cost_j = g.cost[j];
+            dyt_j = g.dyt[j];
+            dzt_k = g.dzt[k];
+            dzw_k = g.dzw[k];
+            dzw_lo = g.dzw[k >= 1 ? k - 1 : 0];  // pad_z_edges(dzw)[k]
+        }
+
+        // ---- phase 1a: flux_top and lateral dissipation -> LDS, K33 rows -> LDS
+        if (have) {
+#pragma unroll
+            for (int t = 0; t < NTR; ++t) {
+                double* sTt = sT + t * 4 * T;
+                sTt[lo] = ft_c[t];
+                if (energy) {
+                    // compute_dissipation (veros/core/diffusion.py:13-33)
+                    const double de = (Ie[t] - Ic[t]) * fe_c[t] + (Ic[t] - Iw[t]) * fe_w[t];
+                    const double dn = (In[t] - Ic[t]) * fn_c[t] + (Ic[t] - Is[t]) * fn_s[t];
+                    const double g05 = 0.5 * p.grav / p.rho_0;
+                    sTt[T + lo] = g05 * de / (dxt_i * cost_j) + g05 * dn / (dyt_j * cost_j);
+                }
+            }
+        }
+        if (interior) {
+            // rows of the K33 system (veros/core/isoneutral/diffusion.py:154-164); the
+            // edge row k = kbot-1 takes b_edge and a = 0 (veros/core/operators.py:72)
+            const double delta = (k < nz - 1) ? dt / dzw_k * K33k : 0.0;
+            const double delta_m = (k >= 1) ? dt / dzw_lo * K33m : 0.0;
+            const double dz = dzt_k;
+            const bool edge = k == kb - 1;
+            sA[lo] = edge ? 0.0 : -delta_m / dz;
+            sB[lo] = edge ? 1 + delta / dz : 1 + (delta + delta_m) / dz;
+            sC[lo] = (k < nz - 1) ? -delta / dz : 0.0;
+        }
+        __syncthreads();
+
+        // ---- phase 1b: divergence and rhs (veros/core/isoneutral/diffusion.py:116-139,196-197)
+        double dtrk[NTR], rhs[NTR];
+#pragma unroll
+        for (int t = 0; t < NTR; ++t) {
+            dtrk[t] = rhs[t] = 0.0;
+            if (interior) {
+                double* sTt = sT + t * 4 * T;
+                // below the lowest wet cell the top-face flux is zero (its maskW)
+                const double ftm = (k >= kb) ? sTt[lo - 1] : 0.0;
+                const double hor = (fe_c[t] - fe_w[t]) / (cost_j * dxt_i) +
+                                   (fn_c[t] - fn_s[t]) / (cost_j * dyt_j);
+                const double vert = (k == 0) ? ft_c[t] / dzt_k : (ft_c[t] - ftm) / dzt_k;
+                dtrk[t] = hor + vert;
+                rhs[t] = trp[t] + dt * dtrk[t];
+                sTt[2 * T + lo] = rhs[t];
+            }
+        }
+        __syncthreads();
+
+        // ---- phase 2: a lane per (interior column, tracer) sweeps in LDS
+        for (int lin = tid; lin < ncolg * NTR; lin += T) {
+            const int cq = lin % ncolg, t = lin / ncolg;
+            const int ccol = sCol[cq];
+            const int ci = 1 + ccol / ny2, cj = 1 + (ccol - (ccol / ny2) * ny2);
+            if (ci < 2 || ci > g.X - 3 || cj < 2 || cj > g.Y - 3) continue;
+            const int b0 = sOff[cq], b1 = sOff[cq + 1];
+            double* R = sT + t * 4 * T + 2 * T;
+            double* Q = sT + t * 4 * T + 3 * T;
+            double cpp = 0.0, dpp = 0.0;
+#pragma unroll 8
+            for (int kk = b0; kk < b1; ++kk) {
+                const double ak = sA[kk];
+                const double r = rcp_nr(sB[kk] - ak * cpp);
+                cpp = sC[kk] * r;
+                dpp = (R[kk] - ak * dpp) * r;
+                Q[kk] = cpp;
+                R[kk] = dpp;
+            }
+            double xn = 0.0;
+#pragma unroll 8
+            for (int kk = b1 - 1; kk >= b0; --kk) {
+                xn = R[kk] - Q[kk] * xn;
+                R[kk] = xn;
+            }
+        }
+        __syncthreads();
+
+        // ---- phase 3
+        if (have) {
+            double Pk = P0;
+#pragma unroll
+            for (int t = 0; t < NTR; ++t) {
+                const double* sTt = sT + t * 4 * T;
+                if (interior) {
+                    const double newk = sTt[2 * T + lo];
+                    a.t[t].tr[3u * c + a.taup1] = newk;
+                    a.t[t].dtr[c] = (dtr0[t] + dtrk[t]) + (newk - rhs[t]) / dt;
+                }
+                if (energy) {
+                    // dissipation_on_wgrid (veros/core/diffusion.py:39-62)
+                    const double d0 = sTt[T + lo];
+                    double dw;
+                    if (k == nz - 1) {
+                        dw = d0;
+                    } else {
+                        const double d1 = sTt[T + lo + 1];
+                        const double avg = 0.5 * (d0 + d1);
+                        const bool edge = k == kb - 1;
+                        dw = (avg + 0.5 * (d0 * dzw_lo / dzw_k)) * (edge ? 1.0 : 0.0) + avg * (edge ? 0.0 : 1.0);
+                    }
+                    Pk = Pk + dw;
+                    // vertical mixing term (veros/core/isoneutral/diffusion.py:254-279)
+                    if (interior && k < nz - 1) {
+                        const double fxa = (-Iu[t] + Ic[t]) / dzw_k;
+                        const double gr = -p.grav / p.rho_0;
+                        const double dtrz = sTt[2 * T + lo + 1] - sTt[2 * T + lo];
+                        Pk = Pk + gr * fxa * (ft_c[t] * 1.0 + K33k * dtrz / dzw_k * 1.0);
+                    }
+                }
+            }
+            if (energy) a.P[c] = Pk;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Adjoint of one isoneutral tracer step (iso=True, energy branch off) with the
 // isoneutral tensor (Ai_*, K_11, K_22, K_33) held fixed:
 //   T_new = M^-1 (T[taup1] + dt * D(T[tau], K_iso))   on interior wet cells,
@@ -1475,6 +1835,27 @@ int launch_pre(const Grid& g, const Prm& p, const Fields& f, const double* K_iso
     return dry_stream ? VX_OK : join_aux(s, sd, who);
 }
 
+// The wet-cell plan after the two tracers' fluxes in the fused call's workspace.
+int64_t wet_plan_ncols(int64_t X, int64_t Y) { return (X - 2) * (Y - 2); }
+int64_t wet_plan_nchunks(int64_t X, int64_t Y) { return (wet_plan_ncols(X, Y) + kScanChunk - 1) / kScanChunk; }
+int64_t wet_plan_bytes(int64_t X, int64_t Y) {
+    const int64_t ints = 3 * wet_plan_ncols(X, Y) + 2 + 12;
+    return (ints * (int64_t)sizeof(int) + 7) / 8 * 8 + wet_plan_nchunks(X, Y) * (int64_t)sizeof(int2);
+}
+WetPlan wire_wet_plan(const Grid& g, void* ws) {
+    const int64_t ncols = wet_plan_ncols(g.X, g.Y);
+    char* base = static_cast<char*>(ws) + 6 * (int64_t)g.X * g.Y * g.nz * (int64_t)sizeof(double);
+    WetPlan w;
+    int* ip = reinterpret_cast<int*>(base);
+    w.wetcol = ip;
+    w.wetoff = ip + ncols;
+    w.first = ip + 2 * ncols;
+    w.nb = ip + 3 * ncols + 2;
+    const int64_t ints = 3 * ncols + 2 + 12;
+    w.chunk = reinterpret_cast<int2*>(base + (ints * (int64_t)sizeof(int) + 7) / 8 * 8);
+    return w;
+}
+
 void wire_workspace(const Grid& g, StepArgs& a, int ntr, void* ws) {
     const int64_t n3 = (int64_t)g.X * g.Y * g.nz;
     double* w = static_cast<double*>(ws);
@@ -1581,6 +1962,11 @@ extern "C" int64_t vx_iso_workspace_bytes(const vx_grid* grid, int32_t ntracers)
     return 3 * (int64_t)ntracers * (grid->nx + 4) * (grid->ny + 4) * (int64_t)grid->nz * (int64_t)sizeof(double);
 }
 
+extern "C" int64_t vx_iso_pre_diffusion_ts_workspace_bytes(const vx_grid* grid) {
+    if (!grid || grid->nx < 1 || grid->ny < 1 || grid->nz < 1) return -1;
+    return vx_iso_workspace_bytes(grid, 2) + vx::wet_plan_bytes(grid->nx + 4, grid->ny + 4);
+}
+
 extern "C" int64_t vx_iso_adjoint_workspace_bytes(const vx_grid* grid) {
     if (!grid) return -1;
     return 4 * (grid->nx + 4) * (grid->ny + 4) * (int64_t)grid->nz * (int64_t)sizeof(double);
@@ -1671,6 +2057,17 @@ extern "C" int vx_iso_pre_diffusion_ts_f64(void* stream, const vx_grid* grid, co
     const Prm p = make_prm(prm);
     hipStream_t s = (hipStream_t)stream;
     const PreFlux fo{{a.t[0].fe, a.t[1].fe}, {a.t[0].fn, a.t[1].fn}, {a.t[0].ft, a.t[1].ft}};
+    // the wet-cell plan of the step (reads kbot only)
+    const WetPlan wp = wire_wet_plan(g, workspace);
+    const int64_t ncols = wet_plan_ncols(g.X, g.Y);
+    const int nchunks = (int)wet_plan_nchunks(g.X, g.Y);
+    const int C = kWetThreads - g.nz + 1;  // >= nz as nz <= kMaxNz
+    hipLaunchKernelGGL(wet_scan_count_kernel, dim3(nchunks), dim3(kWetThreads), 0, s, g, wp, ncols);
+    if ((rc = check_launch(who))) return rc;
+    hipLaunchKernelGGL(wet_scan_chunks_kernel, dim3(1), dim3(kWetThreads), 0, s, wp, nchunks);
+    if ((rc = check_launch(who))) return rc;
+    hipLaunchKernelGGL(wet_scan_columns_kernel, dim3(nchunks), dim3(kWetThreads), 0, s, g, wp, ncols, C);
+    if ((rc = check_launch(who))) return rc;
     // The step kernel reads no output of a dry cell (its fluxes come from the
     // wet-cell kernel, and it reads K_33 only on wet cells), so the dry-cell
     // zero fill overlaps the whole call and is joined at its end.
@@ -1678,11 +2075,16 @@ extern "C" int vx_iso_pre_diffusion_ts_f64(void* stream, const vx_grid* grid, co
     rc = launch_pre<true>(g, p, Fields{temp, salt, tau}, K_iso, Ai_ez, Ai_nz, Ai_bx, Ai_by, K_11, K_22, K_33, fo, s,
                           who, &sd);
     if (!rc) {
-        constexpr int TC = kTileColumns;
-        const dim3 blk(g.nz, TC);
-        const size_t lds = (size_t)(3 + 4 * 2) * TC * g.nz * sizeof(double);
-        hipLaunchKernelGGL((iso_step_tile_kernel<TC, 2, true, true>), cell_grid(g.X - 2, g.Y - 2, blk), blk, lds, s,
-                           g, p, a);
+#ifndef VX_WET_BPC
+#define VX_WET_BPC 5
+#endif
+        // persistent: VX_WET_BPC blocks per CU (as many as fit), a multiple
+        // of 8 in all (measured on 1440x720x80: 3 blocks per CU 9.0 ms for
+        // the call, 4 8.45 ms, 5 at five waves per SIMD 8.29 ms)
+        const int ncu = device_cu_count();
+        const unsigned nblk = (unsigned)(((ncu > 0 ? ncu : 256) * VX_WET_BPC + 7) / 8 * 8);
+        const size_t lds = (size_t)(3 + 4 * 2) * kWetThreads * sizeof(double) + (2 * kWetThreads + 1) * sizeof(int);
+        hipLaunchKernelGGL(iso_step_wet_kernel<2>, dim3(nblk), dim3(kWetThreads), lds, s, g, p, a, wp);
         rc = check_launch(who);
     }
     const int rj = join_aux(s, sd, who);

@@ -230,7 +230,7 @@ def isoneutral_pre_and_diffusion_temp_salt(state):
     else:
         IT = IS = P = None
     L = _native.lib()
-    ws = workspace(L.vx_iso_workspace_bytes(g, 2), T.device)
+    ws = workspace(L.vx_iso_pre_diffusion_ts_workspace_bytes(g), T.device)
     rc = L.vx_iso_pre_diffusion_ts_f64(stream_handle(), g, p, ptr(T), ptr(S), int(vs.tau), int(vs.taup1), ptr(K),
                                        *(ptr(t) for t in tens), ptr(dT), ptr(dS), ptr(IT), ptr(IS), ptr(P),
                                        ptr(ws))
