@@ -174,6 +174,13 @@ __device__ __forceinline__ double rden(double drodz) { return rcp_nr(fmin(0.0, d
 #define VX_STRIP_BLOCKS 16
 #endif
 constexpr int kStripBlocks = VX_STRIP_BLOCKS;
+// the wet-cell pre kernel's blocks are wider (pre_columns): fewer per strip
+// (4 x 24 columns at nz = 80; measured on 1440x720x80, 16 per strip made the
+// fused pre + temperature/salinity call 1% slower, 8 5% slower)
+#ifndef VX_PRE_STRIP_BLOCKS
+#define VX_PRE_STRIP_BLOCKS 4
+#endif
+constexpr int kPreStripBlocks = VX_PRE_STRIP_BLOCKS;
 
 struct CellIdx {
     int i, j, k;
@@ -185,15 +192,15 @@ struct BlockCols {
     int i, j;
     bool valid;  // false for a padding block
 };
-__device__ __forceinline__ BlockCols block_cols(int i0, int ni, int j0, int nj, int cpb) {
-    const int sw = kStripBlocks * cpb;                // strip width in columns
+__device__ __forceinline__ BlockCols block_cols(int i0, int ni, int j0, int nj, int cpb, int sb = kStripBlocks) {
+    const int sw = sb * cpb;                          // strip width in columns
     const int nstrips = (nj + sw - 1) / sw;
-    const int per_strip = ni * kStripBlocks;
+    const int per_strip = ni * sb;
     const int L = (int)xcd_block(gridDim.x);
     const int sidx = L / per_strip;
     const int rem = L - sidx * per_strip;
-    const int ii = rem / kStripBlocks;
-    const int b = rem - ii * kStripBlocks;
+    const int ii = rem / sb;
+    const int b = rem - ii * sb;
     BlockCols r;
     r.i = i0 + ii;
     r.j = j0 + sidx * sw + b * cpb;
@@ -623,7 +630,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FUSED ? 2 :
     double* __restrict__ Abx, double* __restrict__ Aby, double* __restrict__ K11, double* __restrict__ K22,
     double* __restrict__ K33, PreFlux fo, int ncb) {
     extern __shared__ int pre_counts[];  // prefix sums of the columns' wet counts
-    const BlockCols bc = block_cols(1, g.X - 3, 1, g.Y - 3, ncb);
+    const BlockCols bc = block_cols(1, g.X - 3, 1, g.Y - 3, ncb, kPreStripBlocks);
     if (!bc.valid) return;  // padding block (uniform)
     const int nz = g.nz, i = bc.i, jb = bc.j;
     const int tid = (int)threadIdx.x, nthr = (int)blockDim.x;
@@ -829,10 +836,7 @@ __device__ __forceinline__ double fn_at(const Grid& g, const TracerArgs& t, int 
     return (i >= 2 && i <= g.X - 3 && j >= 1 && j <= g.Y - 3) ? t.fn[o] : 0.0;
 }
 
-// WETFLUX: the face fluxes exist for wet cells only (written by the fused pre
-// + flux kernel from the tensor it formed, in which every face touching a dry
-// cell carries a zero flux); a face whose other cell is dry reads as zero.
-template <int TC, int NTR, bool IMPLICIT, bool WETFLUX>
+template <int TC, int NTR, bool IMPLICIT>
 __global__ __launch_bounds__(TC * kMaxNz) void iso_step_tile_kernel(Grid g, Prm p, StepArgs a) {
     extern __shared__ double lds[];
     const int nz = g.nz;
@@ -862,11 +866,6 @@ __global__ __launch_bounds__(TC * kMaxNz) void iso_step_tile_kernel(Grid g, Prm 
     // (flux_top at k = kbot-2) to the sweep's rhs.
     const bool wc = valid && wet(kb, k);
     const bool wu = valid && k + 1 < nz && wet(kb, k + 1);
-    bool west_ok = true, south_ok = true;
-    if constexpr (WETFLUX) {
-        west_ok = wc && wet(g.kbot[(i - 1) * g.Y + j], k);
-        south_ok = wc && wet(g.kbot[i * g.Y + j - 1], k);
-    }
 
     // ---- every global read of the step first: the block then waits on memory
     // once instead of once per phase
@@ -879,9 +878,9 @@ __global__ __launch_bounds__(TC * kMaxNz) void iso_step_tile_kernel(Grid g, Prm 
         Ic[t] = Ie[t] = Iw[t] = In[t] = Is[t] = Iu[t] = trp[t] = dtr0[t] = 0.0;
         if (wc) {
             fe_c[t] = fe_at(g, ta, i, j, c);
-            fe_w[t] = west_ok ? fe_at(g, ta, i - 1, j, c - st.si) : 0.0;
+            fe_w[t] = fe_at(g, ta, i - 1, j, c - st.si);
             fn_c[t] = fn_at(g, ta, i, j, c);
-            fn_s[t] = south_ok ? fn_at(g, ta, i, j - 1, c - st.sj) : 0.0;
+            fn_s[t] = fn_at(g, ta, i, j - 1, c - st.sj);
             if (interior) {
                 trp[t] = ta.tr[3u * c + a.taup1];
                 dtr0[t] = ta.dtr[c];
@@ -896,7 +895,7 @@ __global__ __launch_bounds__(TC * kMaxNz) void iso_step_tile_kernel(Grid g, Prm 
                 if (interior && k < nz - 1) Iu[t] = I[3u * (c + 1)];
             }
         }
-        if (interior && (WETFLUX ? wc : (wc || wu))) ft_c[t] = ta.ft[c];
+        if (interior && (wc || wu)) ft_c[t] = ta.ft[c];
     }
     double K33k = 0.0, K33m = 0.0;
     if (IMPLICIT && interior && wc) {
@@ -1043,8 +1042,9 @@ __global__ __launch_bounds__(TC * kMaxNz) void iso_step_tile_kernel(Grid g, Prm 
 // ---------------------------------------------------------------------------
 // Wet-cell tracer step of the fused pre + temperature/salinity call (the tile
 // kernel's work with the fused pre's wet-face fluxes). The wet columns of the
-// step's range [1, X-1) x [1, Y-1) are ranked in row-major order (rank r, a
-// column's wet cells being its levels kbot-1 .. nz-1), and their cells are
+// step's range [1, X-1) x [1, Y-1) are ranked in the strip order of
+// wet_col_ij (rank r, a column's wet cells being its levels kbot-1 .. nz-1),
+// and their cells are
 // numbered in the same order (column r holds cells o_r .. o_r + w_r - 1). The
 // cells are cut into groups: group b takes the wet columns with
 // b*C <= o_r < (b+1)*C, C = kWetThreads - nz + 1, so a group holds at most
@@ -1057,17 +1057,37 @@ constexpr int kScanColsPerThread = 8;
 constexpr int kScanChunk = kWetThreads * kScanColsPerThread;  // columns per scan block
 
 struct WetPlan {
-    int* wetcol;  // [ncols]   column index (i-1)*(Y-2) + (j-1) of the wet column of rank r
+    int* wetcol;  // [ncols]   enumeration index (wet_col_ij) of the wet column of rank r
     int* wetoff;  // [ncols]   o_r, the number of wet cells before it
     int* first;   // [ncols+2] first[b] = first rank of group b; first[ngroups] = number of wet columns
     int* nb;      // [12]      ngroups, wet columns, wet cells, -, then a group counter per XCD
     int2* chunk;  // [nchunks] (wet columns, wet cells) per scan chunk, then their exclusive sums
 };
 
+// The columns of the step range are enumerated in strips of kWetStrip
+// columns along j, each strip walked row by row along i, so that the rows a
+// group's stencil reaches (i-1, i+1) are stepped close in time and share an
+// L2; the columns past Y-2 in the last strip are padding without wet cells.
+#ifndef VX_WET_STRIP
+#define VX_WET_STRIP 64
+#endif
+constexpr int kWetStrip = VX_WET_STRIP;
+__host__ __device__ inline int64_t wet_enum_cols(int64_t X, int64_t Y) {
+    return (Y - 2 + kWetStrip - 1) / kWetStrip * (X - 2) * kWetStrip;
+}
+__device__ __forceinline__ void wet_col_ij(const Grid& g, int col, int& i, int& j) {
+    const int per = (g.X - 2) * kWetStrip;
+    const int sidx = col / per;
+    const int rem = col - sidx * per;
+    const int ii = rem / kWetStrip;
+    i = 1 + ii;
+    j = 1 + sidx * kWetStrip + (rem - ii * kWetStrip);
+}
 __device__ __forceinline__ int wet_count(const Grid& g, int64_t col, int64_t ncols) {
     if (col >= ncols) return 0;
-    const int ny2 = g.Y - 2;
-    const int i = 1 + (int)(col / ny2), j = 1 + (int)(col - (col / ny2) * ny2);
+    int i, j;
+    wet_col_ij(g, (int)col, i, j);
+    if (j > g.Y - 2) return 0;
     const int kb = g.kbot[i * g.Y + j];
     return (kb >= 1 && kb <= g.nz) ? g.nz - kb + 1 : 0;
 }
@@ -1193,7 +1213,7 @@ __global__ __launch_bounds__(kWetThreads) __attribute__((amdgpu_waves_per_eu(VX_
     double* sT = lds + 3 * T;
     int* sOff = reinterpret_cast<int*>(lds + (3 + 4 * NTR) * T);  // [T+1] first cell of each column
     int* sCol = sOff + (T + 1);                                   // [T]   its column index
-    const int nz = g.nz, ny2 = g.Y - 2;
+    const int nz = g.nz;
     const int tid = (int)threadIdx.x;
     const int ngroups = w.nb[0];
     const int xcd = (int)(blockIdx.x & 7);
@@ -1230,8 +1250,8 @@ __global__ __launch_bounds__(kWetThreads) __attribute__((amdgpu_waves_per_eu(VX_
             }
         }
         const int lo = tid;
-        const int col = have ? sCol[q] : 0;
-        const int i = 1 + col / ny2, j = 1 + (col - (col / ny2) * ny2);
+        int i, j;
+        wet_col_ij(g, have ? sCol[q] : 0, i, j);
         const int kb = have ? nz - (sOff[q + 1] - sOff[q]) + 1 : 1;
         const int k = kb - 1 + (tid - sOff[q]);
         const bool interior = have && i >= 2 && i <= g.X - 3 && j >= 2 && j <= g.Y - 3;
@@ -1337,8 +1357,8 @@ __global__ __launch_bounds__(kWetThreads) __attribute__((amdgpu_waves_per_eu(VX_
         // ---- phase 2: a lane per (interior column, tracer) sweeps in LDS
         for (int lin = tid; lin < ncolg * NTR; lin += T) {
             const int cq = lin % ncolg, t = lin / ncolg;
-            const int ccol = sCol[cq];
-            const int ci = 1 + ccol / ny2, cj = 1 + (ccol - (ccol / ny2) * ny2);
+            int ci, cj;
+            wet_col_ij(g, sCol[cq], ci, cj);
             if (ci < 2 || ci > g.X - 3 || cj < 2 || cj > g.Y - 3) continue;
             const int b0 = sOff[cq], b1 = sOff[cq + 1];
             double* R = sT + t * 4 * T + 2 * T;
@@ -1742,10 +1762,10 @@ unsigned blocks_xcd(int64_t n, int bs) { return (unsigned)(((n + bs - 1) / bs + 
 // block of nz levels x cpb columns (about 256 threads); 1-D grid of ni row
 // groups, rounded up to a multiple of 8 for the XCD remap
 dim3 cell_block(int nz) { return dim3(nz, nz >= 256 ? 1 : 256 / nz); }
-dim3 cell_grid(int ni, int nj, dim3 blk) {
-    const int64_t sw = (int64_t)kStripBlocks * blk.y;
+dim3 cell_grid(int ni, int nj, dim3 blk, int sb = kStripBlocks) {
+    const int64_t sw = (int64_t)sb * blk.y;
     const int64_t nstrips = (nj + sw - 1) / sw;
-    return dim3((unsigned)((nstrips * ni * kStripBlocks + 7) / 8 * 8));
+    return dim3((unsigned)((nstrips * ni * sb + 7) / 8 * 8));
 }
 
 }  // namespace
@@ -1771,7 +1791,7 @@ int launch_step(const Grid& g, const Prm& p, const StepArgs& a, hipStream_t s, c
     if (rc) return rc;
     const dim3 blk(g.nz, TC);
     const size_t lds = (size_t)(3 + 4 * NTR) * TC * g.nz * sizeof(double);
-    hipLaunchKernelGGL((iso_step_tile_kernel<TC, NTR, IMPLICIT, false>), cell_grid(g.X - 2, g.Y - 2, blk), blk, lds, s, g,
+    hipLaunchKernelGGL((iso_step_tile_kernel<TC, NTR, IMPLICIT>), cell_grid(g.X - 2, g.Y - 2, blk), blk, lds, s, g,
                        p, a);
     return check_launch(who);
 }
@@ -1827,7 +1847,8 @@ int launch_pre(const Grid& g, const Prm& p, const Fields& f, const double* K_iso
     if ((rc = check_launch(who))) return rc;
     const int ncb = pre_columns(g.nz);
     const size_t pre_lds = (size_t)(ncb + 1) * sizeof(int);
-    hipLaunchKernelGGL(iso_pre_kernel<FUSED>, cell_grid(g.X - 3, g.Y - 3, dim3(1, ncb)), dim3(256), pre_lds, s, g, p,
+    hipLaunchKernelGGL(iso_pre_kernel<FUSED>, cell_grid(g.X - 3, g.Y - 3, dim3(1, ncb), kPreStripBlocks), dim3(256),
+                       pre_lds, s, g, p,
                        f, K_iso, Ai_ez, Ai_nz, Ai_bx, Ai_by, K_11, K_22, K_33, fo, ncb);
     if ((rc = check_launch(who))) return rc;
     hipLaunchKernelGGL(k33_top_zero_kernel, dim3(blocks_for((int64_t)g.X * g.Y, 256)), dim3(256), 0, s, g, K_33);
@@ -1836,7 +1857,7 @@ int launch_pre(const Grid& g, const Prm& p, const Fields& f, const double* K_iso
 }
 
 // The wet-cell plan after the two tracers' fluxes in the fused call's workspace.
-int64_t wet_plan_ncols(int64_t X, int64_t Y) { return (X - 2) * (Y - 2); }
+int64_t wet_plan_ncols(int64_t X, int64_t Y) { return wet_enum_cols(X, Y); }
 int64_t wet_plan_nchunks(int64_t X, int64_t Y) { return (wet_plan_ncols(X, Y) + kScanChunk - 1) / kScanChunk; }
 int64_t wet_plan_bytes(int64_t X, int64_t Y) {
     const int64_t ints = 3 * wet_plan_ncols(X, Y) + 2 + 12;
