@@ -30,7 +30,7 @@ HBM_PEAK_GBS = 8000.0
 # and WRITE_SIZE rocprofv3 passes recorded in profiles/ (read bytes = 2.0 x
 # FETCH_SIZE, write bytes = 1.0 x WRITE_SIZE, calibrated on a 1 GiB device copy)
 TS_TRAFFIC_BYTES = {(1440, 720, 80): 21_040_000_000}
-PRE_TS_TRAFFIC_BYTES = {}
+PRE_TS_TRAFFIC_BYTES = {(1440, 720, 80): 27_142_421_504}
 # algorithmic bytes per interior cell: the fused temperature + salinity step
 # with the energy branch reads K_iso, Ai_* and K_11/22/33 once (8 + 128 + 24),
 # P_diss read and written (16), and per tracer tr[tau], tr[taup1] read and
@@ -337,13 +337,14 @@ def main():
     }
     gbs = {k: b / (t * 1e-3) / 1e9 for k, (b, t) in kern.items()}
     # roofline: pre + the temperature + salinity diffusion in one C-ABI call,
-    # the largest launch of the step (four kernels: the dry-column zero fill on
-    # a helper stream beside the wet-cell slopes + face fluxes, the top-level
-    # K_33 zero, then the divergence + K33 solve + updates)
+    # the largest launch of the step (the wet-cell plan's three scan kernels,
+    # the dry-column zero fill on a helper stream beside the wet-cell slopes +
+    # face fluxes, the top-level K_33 zero, then the divergence + K33 solve +
+    # updates over the wet cells)
     dom = "iso_pre_and_diffusion_TS"
     roof = {"bound": "hbm",
-            "kernel": "vx_iso_pre_diffusion_ts_f64 = iso_pre_dry_kernel || iso_pre_kernel<true> + "
-                      "k33_top_zero_kernel + iso_step_tile_kernel<3,2,true,true>",
+            "kernel": "vx_iso_pre_diffusion_ts_f64 = wet_scan_{count,chunks,columns}_kernel + "
+                      "(iso_pre_dry_kernel || iso_pre_kernel<true> + k33_top_zero_kernel + iso_step_wet_kernel<2>)",
             "achieved": round(gbs[dom], 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(gbs[dom] / HBM_PEAK_GBS, 4),
             "traffic": PRE_TS_TRAFFIC_BYTES.get((nx, ny, nz)),
