@@ -52,6 +52,30 @@ def test_diffusion_golden(path, trn, iso):
         assert rel_err(_np(getattr(out, k)), g[f"{tag}_{k}"]) <= TOL, k
 
 
+@pytest.mark.parametrize("path", golden_files("iso_"), ids=lambda p: p.stem)
+def test_pre_and_temp_salt_fused_golden(path):
+    """The one call (pre, then temperature, then salinity) against the
+    reference's recorded outputs: pre's tensor; the temperature and salinity
+    steps, each recorded from the same inputs with that tensor (the salinity
+    step reads nothing the temperature step writes, except P_diss_iso); and
+    P_diss_iso as the input plus both recorded increments."""
+    from veros_amd import isoneutral
+    g = load_golden(path)
+    ds = _dev_state(state_from_golden(g))
+    isoneutral.isoneutral_pre_and_diffusion_temp_salt(ds)
+    vs = ds.variables
+    for k in PRE_KEYS:
+        assert rel_err(_np(getattr(vs, k)), g["pre_" + k]) <= TOL, k
+    for tr, keys in (("temp", ("temp", "dtemp_iso")), ("salt", ("salt", "dsalt_iso"))):
+        for k in keys:
+            assert rel_err(_np(getattr(vs, k)), g[f"iso_{tr}_{k}"]) <= TOL, k
+    if "iso_temp_P_diss_iso" in g:
+        P0 = g["in_P_diss_iso"]
+        ref = P0 + (g["iso_temp_P_diss_iso"] - P0) + (g["iso_salt_P_diss_iso"] - P0)
+        scale = max(np.abs(ref).max(), np.abs(g["iso_temp_P_diss_iso"] - P0).max())
+        assert np.abs(_np(vs.P_diss_iso) - ref).max() <= TOL * scale
+
+
 def _random_state(seed, nx, ny, nz, **kw):
     rng = np.random.default_rng(seed)
     opts = dict(dx_deg=1.0, dy_deg=1.0, y_origin=-50.0, dz_top=10.0, dz_bottom=80.0, land_frac=0.25,
