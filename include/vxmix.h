@@ -193,7 +193,8 @@ int vx_iso_pre_f64(void* stream, const vx_grid* grid, const vx_iso_params* prm,
  * (enable_conserve_energy), P_diss accumulates the dissipation of dynamic
  * enthalpy computed from int_drhodX[..., tau] (veros/core/diffusion.py:9-62).
  * `workspace` must hold vx_iso_workspace_bytes(grid, 1) bytes (the face
- * fluxes); calls that share a workspace must be ordered on one stream.
+ * fluxes and the wet-cell plan built from kbot); calls that share a workspace
+ * must be ordered on one stream.
  */
 int64_t vx_iso_workspace_bytes(const vx_grid* grid, int32_t ntracers);
 int vx_iso_diffusion_f64(void* stream, const vx_grid* grid, const vx_iso_params* prm,
@@ -232,7 +233,8 @@ int vx_iso_diffusion_ts_f64(void* stream, const vx_grid* grid, const vx_iso_para
  * and salinity are formed from the isoneutral tensor while it is computed,
  * instead of reading it back, and the tracer step runs over the wet cells
  * only. `workspace` holds vx_iso_pre_diffusion_ts_workspace_bytes(grid)
- * bytes (the two tracers' face fluxes and the wet-cell plan built from kbot).
+ * bytes (the two tracers' face fluxes and the wet-cell plan built from kbot;
+ * the same size as vx_iso_workspace_bytes(grid, 2)).
  */
 int64_t vx_iso_pre_diffusion_ts_workspace_bytes(const vx_grid* grid);
 

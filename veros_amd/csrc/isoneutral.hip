@@ -1,11 +1,14 @@
 // Isoneutral triad-tensor stencils for gfx950 (float64).
 //
-//   iso_pre_kernel       isoneutral_diffusion_pre  veros/core/isoneutral/isoneutral.py:18-229
+//   iso_pre_dry_kernel, iso_pre_kernel
+//                        isoneutral_diffusion_pre  veros/core/isoneutral/isoneutral.py:18-229
 //   iso_flux_kernel      _calc_tracer_fluxes       veros/core/isoneutral/diffusion.py:9-113
-//   iso_column_tile_kernel  _calc_explicit_part, _calc_implicit_part, the tracer and
-//                        dtr_iso updates and the energy branch
+//   wet_scan_*_kernel    the wet-cell plan of a step (from kbot)
+//   iso_step_wet_kernel  _calc_explicit_part, _calc_implicit_part, the tracer and
+//                        dtr_iso updates and the energy branch, over the wet cells
 //                                                  veros/core/isoneutral/diffusion.py:116-283,
 //                                                  veros/core/diffusion.py:9-62
+//   iso_adj_*_kernel     the adjoint of one isoneutral tracer step
 //
 // Layout: veros state arrays with 2-cell halos, z contiguous, k = 0 deepest.
 // Cell-parallel kernels put consecutive lanes on consecutive levels of one
@@ -18,13 +21,10 @@
 
 #include "vx_common.hpp"
 
-// Tuning knobs (compile-time): minimum waves per SIMD requested for the pre
-// kernel (caps its VGPRs), and interior columns per block of the column-tile kernel.
+// Tuning knob (compile-time): minimum waves per SIMD requested for the pre
+// kernel (caps its VGPRs).
 #ifndef VX_PRE_MIN_WAVES
 #define VX_PRE_MIN_WAVES 3
-#endif
-#ifndef VX_TILE_COLUMNS
-#define VX_TILE_COLUMNS 3
 #endif
 
 namespace vx {
@@ -668,15 +668,16 @@ __global__ void k33_top_zero_kernel(Grid g, double* __restrict__ K33) {
 }
 
 // ---------------------------------------------------------------------------
-// Fused tracer step (veros/core/isoneutral/diffusion.py:9-283 with
+// Tracer step (veros/core/isoneutral/diffusion.py:9-283 with
 // veros/core/diffusion.py:9-62), NTR tracers sharing one pass over the
-// isoneutral tensor. Block = nz levels x TC columns (consecutive j of one i
-// row), covering (i, j) in [1, X-1) x [1, Y-1).
-//   iso_flux_kernel (thread per cell): the face fluxes -> workspace
-//   tile kernel phase 1 (thread per cell): lateral dissipation on T points,
-//     the flux divergence, rhs = tr[taup1] + dt*dtr, and the K33 rows -> LDS
+// isoneutral tensor, covering (i, j) in [1, X-1) x [1, Y-1):
+//   iso_flux_kernel (thread per cell): the face fluxes -> workspace (in the
+//     fused pre + temperature/salinity call, iso_pre_kernel<true> instead)
+//   iso_step_wet_kernel phase 1 (thread per wet cell): lateral dissipation on
+//     T points, the flux divergence, rhs = tr[taup1] + dt*dtr, and the K33
+//     rows -> LDS
 //   phase 2 (lane per interior column and tracer): Thomas sweep in LDS
-//   phase 3 (thread per cell): tracer, dtr_iso and P_diss updates
+//   phase 3 (thread per wet cell): tracer, dtr_iso and P_diss updates
 // ---------------------------------------------------------------------------
 struct TracerArgs {
     double* tr;           // (X, Y, nz, 3)
@@ -756,7 +757,7 @@ __device__ __forceinline__ double flux_top(const FaceIn& in, const Cols& tc, int
 // Every use of a face flux is multiplied by the maskT of a cell next to the
 // face (the divergence, the lateral dissipation and the vertical mixing term
 // of that cell), so a face between two dry cells is neither computed nor
-// written; the tile kernel reads only faces that touch a wet cell.
+// written; the step kernel reads only faces that touch a wet cell.
 template <int NTR, bool ISO>
 __global__ __launch_bounds__(256) void iso_flux_kernel(Grid g, StepArgs a) {
     extern __shared__ double lds[];
@@ -836,217 +837,13 @@ __device__ __forceinline__ double fn_at(const Grid& g, const TracerArgs& t, int 
     return (i >= 2 && i <= g.X - 3 && j >= 1 && j <= g.Y - 3) ? t.fn[o] : 0.0;
 }
 
-template <int TC, int NTR, bool IMPLICIT>
-__global__ __launch_bounds__(TC * kMaxNz) void iso_step_tile_kernel(Grid g, Prm p, StepArgs a) {
-    extern __shared__ double lds[];
-    const int nz = g.nz;
-    const int tcnz = TC * nz;
-    // LDS: rows A, B, C (shared by the tracers), then per tracer F (flux_top),
-    // D (lateral dissipation), R (rhs -> dp -> new), Q (cp)
-    double* sA = lds;
-    double* sB = lds + tcnz;
-    double* sC = lds + 2 * tcnz;
-    double* sT = lds + 3 * tcnz;  // per tracer blocks of 4 * tcnz
-
-    const CellIdx ci = cell_of(1, g.X - 2, 1, g.Y - 2);
-    const int i = ci.i, j = ci.j, k = ci.k;
-    const int tc = (int)threadIdx.y;
-    const int lo = tc * nz + k;
-    const bool valid = ci.valid;
-    const bool interior = valid && i >= 2 && i <= g.X - 3 && j >= 2 && j <= g.Y - 3;
-    const Strides st{(uint32_t)g.Y * (uint32_t)nz, (uint32_t)nz};
-    const uint32_t c = (uint32_t)i * st.si + (uint32_t)j * st.sj + (uint32_t)k;
-    const int kb = valid ? g.kbot[i * g.Y + j] : 0;
-    const bool energy = a.P != nullptr;
-    const double dt = p.dt_tracer;
-    const double mt = m01(kb, k);
-    // Dry cells (maskT = 0) keep tr, dtr and P unchanged: their flux
-    // divergence, dissipation and vertical mixing term are multiplied by
-    // maskT. Their lanes only hand the bottom face flux of the lowest wet cell
-    // (flux_top at k = kbot-2) to the sweep's rhs.
-    const bool wc = valid && wet(kb, k);
-    const bool wu = valid && k + 1 < nz && wet(kb, k + 1);
-
-    // ---- every global read of the step first: the block then waits on memory
-    // once instead of once per phase
-    double fe_c[NTR], fe_w[NTR], fn_c[NTR], fn_s[NTR], ft_c[NTR];
-    double Ic[NTR], Ie[NTR], Iw[NTR], In[NTR], Is[NTR], Iu[NTR], trp[NTR], dtr0[NTR];
-#pragma unroll
-    for (int t = 0; t < NTR; ++t) {
-        const TracerArgs& ta = a.t[t];
-        fe_c[t] = fe_w[t] = fn_c[t] = fn_s[t] = ft_c[t] = 0.0;
-        Ic[t] = Ie[t] = Iw[t] = In[t] = Is[t] = Iu[t] = trp[t] = dtr0[t] = 0.0;
-        if (wc) {
-            fe_c[t] = fe_at(g, ta, i, j, c);
-            fe_w[t] = fe_at(g, ta, i - 1, j, c - st.si);
-            fn_c[t] = fn_at(g, ta, i, j, c);
-            fn_s[t] = fn_at(g, ta, i, j - 1, c - st.sj);
-            if (interior) {
-                trp[t] = ta.tr[3u * c + a.taup1];
-                dtr0[t] = ta.dtr[c];
-            }
-            if (energy) {
-                const double* __restrict__ I = ta.I + a.tau;
-                Ic[t] = I[3u * c];
-                Ie[t] = I[3u * (c + st.si)];
-                Iw[t] = I[3u * (c - st.si)];
-                In[t] = I[3u * (c + st.sj)];
-                Is[t] = I[3u * (c - st.sj)];
-                if (interior && k < nz - 1) Iu[t] = I[3u * (c + 1)];
-            }
-        }
-        if (interior && (wc || wu)) ft_c[t] = ta.ft[c];
-    }
-    double K33k = 0.0, K33m = 0.0;
-    if (IMPLICIT && interior && wc) {
-        K33k = a.K33[c];
-        if (k >= kb) K33m = a.K33[c - 1];  // the edge row k = kbot-1 has no sub-diagonal
-    }
-    const double P0 = (energy && wc) ? a.P[c] : 0.0;
-    double dxt_i = 0.0, cost_j = 1.0, dyt_j = 0.0;
-    if (valid) {
-        dxt_i = g.dxt[i];
-        cost_j = g.cost[j];
-        dyt_j = g.dyt[j];
-    }
-    const double dzt_k = g.dzt[k], dzw_k = g.dzw[k];
-    const double dzw_lo = g.dzw[k >= 1 ? k - 1 : 0];  // pad_z_edges(dzw)[k]
-
-    // ---- phase 1a: flux_top and lateral dissipation -> LDS, K33 rows -> LDS
-#pragma unroll
-    for (int t = 0; t < NTR; ++t) {
-        double* sTt = sT + t * 4 * tcnz;
-        sTt[lo] = ft_c[t];
-        if (valid && energy) {
-            double diss = 0.0;
-            if (wc) {
-                // compute_dissipation (veros/core/diffusion.py:13-33)
-                const double de = (Ie[t] - Ic[t]) * fe_c[t] + (Ic[t] - Iw[t]) * fe_w[t];
-                const double dn = (In[t] - Ic[t]) * fn_c[t] + (Ic[t] - Is[t]) * fn_s[t];
-                const double g05 = 0.5 * p.grav / p.rho_0;
-                diss = g05 * de / (dxt_i * cost_j) + g05 * dn / (dyt_j * cost_j);
-            }
-            sTt[tcnz + lo] = diss;
-        }
-    }
-    if (IMPLICIT && interior && wc) {
-        // rows of the K33 system (veros/core/isoneutral/diffusion.py:154-164); the
-        // edge row k = kbot-1 takes b_edge and a = 0 (veros/core/operators.py:72)
-        const double delta = (k < nz - 1) ? dt / dzw_k * K33k : 0.0;
-        const double delta_m = (k >= 1) ? dt / dzw_lo * K33m : 0.0;
-        const double dz = dzt_k;
-        const bool edge = k == kb - 1;
-        sA[lo] = edge ? 0.0 : -delta_m / dz;
-        sB[lo] = edge ? 1 + delta / dz : 1 + (delta + delta_m) / dz;
-        sC[lo] = (k < nz - 1) ? -delta / dz : 0.0;
-    }
-    __syncthreads();
-
-    // ---- phase 1b: divergence and rhs (veros/core/isoneutral/diffusion.py:116-139,196-197)
-    double dtrk[NTR], rhs[NTR];
-#pragma unroll
-    for (int t = 0; t < NTR; ++t) {
-        dtrk[t] = rhs[t] = 0.0;
-        if (interior && wc) {
-            double* sTt = sT + t * 4 * tcnz;
-            const double ftm = (k >= 1) ? sTt[lo - 1] : 0.0;
-            const double hor = (fe_c[t] - fe_w[t]) / (cost_j * dxt_i) +
-                               (fn_c[t] - fn_s[t]) / (cost_j * dyt_j);
-            const double vert = (k == 0) ? ft_c[t] / dzt_k : (ft_c[t] - ftm) / dzt_k;
-            dtrk[t] = mt * hor + mt * vert;
-            rhs[t] = trp[t] + dt * dtrk[t];
-            sTt[2 * tcnz + lo] = rhs[t];
-        }
-    }
-
-    // ---- phase 2: one lane per (interior column, tracer) sweeps in LDS
-    if constexpr (IMPLICIT) {
-        __syncthreads();
-        const int lin = (int)(threadIdx.y * blockDim.x + threadIdx.x);
-        if (lin < TC * NTR) {
-            const int col = lin % TC, t = lin / TC;
-            const int cj = j - tc + col;  // j - tc is the tile's first column
-            const bool col_ok = i >= 2 && i <= g.X - 3 && cj >= 2 && cj <= g.Y - 3;
-            const int kbl = col_ok ? g.kbot[i * g.Y + cj] : 0;
-            if (col_ok && kbl >= 1 && kbl <= nz) {
-                const int k0 = kbl - 1;
-                const double* Ar = sA + col * nz;
-                const double* Br = sB + col * nz;
-                const double* Cr = sC + col * nz;
-                double* R = sT + t * 4 * tcnz + 2 * tcnz + col * nz;
-                double* Q = sT + t * 4 * tcnz + 3 * tcnz + col * nz;
-                double cpp = 0.0, dpp = 0.0;
-#pragma unroll 8
-                for (int kk = k0; kk < nz; ++kk) {
-                    const double ak = Ar[kk];
-                    const double r = rcp_nr(Br[kk] - ak * cpp);
-                    cpp = Cr[kk] * r;
-                    dpp = (R[kk] - ak * dpp) * r;
-                    Q[kk] = cpp;
-                    R[kk] = dpp;
-                }
-                double xn = 0.0;
-#pragma unroll 8
-                for (int kk = nz - 1; kk >= k0; --kk) {
-                    xn = R[kk] - Q[kk] * xn;
-                    R[kk] = xn;
-                }
-            }
-        }
-    }
-    __syncthreads();
-
-    // ---- phase 3
-    if (!wc) return;
-    double Pk = P0;
-#pragma unroll
-    for (int t = 0; t < NTR; ++t) {
-        const double* sTt = sT + t * 4 * tcnz;
-        if (interior) {
-            const double newk = sTt[2 * tcnz + lo];
-            a.t[t].tr[3u * c + a.taup1] = newk;
-            a.t[t].dtr[c] = (dtr0[t] + dtrk[t]) + (newk - rhs[t]) / dt;
-        }
-        if (energy) {
-            // dissipation_on_wgrid (veros/core/diffusion.py:39-62)
-            const double d0 = sTt[tcnz + lo];
-            const bool land = kb >= 1;  // create_water_masks(kbot): ks = kbot - 1 >= 0
-            double dw;
-            if (k == nz - 1) {
-                dw = d0 * (land ? 1.0 : 0.0);
-            } else {
-                const double d1 = sTt[tcnz + lo + 1];
-                const double avg = 0.5 * (d0 + d1);
-                const bool edge = land && k == kb - 1;
-                const bool water = land && k > kb - 1;
-                dw = (avg + 0.5 * (d0 * dzw_lo / dzw_k)) * (edge ? 1.0 : 0.0) + avg * (water ? 1.0 : 0.0);
-            }
-            Pk = Pk + dw;
-            // vertical mixing term (veros/core/isoneutral/diffusion.py:254-279)
-            if (interior && k < nz - 1) {
-                const double mw = m01(kb, k) * m01(kb, k + 1);
-                const double fxa = (-Iu[t] + Ic[t]) / dzw_k;
-                const double gr = -p.grav / p.rho_0;
-                if constexpr (IMPLICIT) {
-                    const double dtrz = sTt[2 * tcnz + lo + 1] - sTt[2 * tcnz + lo];
-                    Pk = Pk + gr * fxa * (ft_c[t] * mw + K33k * dtrz / dzw_k * mw);
-                } else {
-                    Pk = Pk + gr * fxa * ft_c[t] * mw;
-                }
-            }
-        }
-    }
-    if (energy) a.P[c] = Pk;
-}
 
 // ---------------------------------------------------------------------------
-// Wet-cell tracer step of the fused pre + temperature/salinity call (the tile
-// kernel's work with the fused pre's wet-face fluxes). The wet columns of the
-// step's range [1, X-1) x [1, Y-1) are ranked in the strip order of
-// wet_col_ij (rank r, a column's wet cells being its levels kbot-1 .. nz-1),
-// and their cells are
-// numbered in the same order (column r holds cells o_r .. o_r + w_r - 1). The
-// cells are cut into groups: group b takes the wet columns with
+// Wet-cell tracer step. The wet columns of the step's range [1, X-1) x
+// [1, Y-1) are ranked in the strip order of wet_col_ij (rank r, a column's wet
+// cells being its levels kbot-1 .. nz-1), and their cells are numbered in the
+// same order (column r holds cells o_r .. o_r + w_r - 1). The cells are cut
+// into groups: group b takes the wet columns with
 // b*C <= o_r < (b+1)*C, C = kWetThreads - nz + 1, so a group holds at most
 // C + nz - 1 = kWetThreads cells (w_r <= nz). A block steps one group at a
 // time with a thread per cell, so no lane is spent on a dry cell; the plan is
@@ -1196,10 +993,16 @@ __global__ __launch_bounds__(kWetThreads) void wet_scan_columns_kernel(Grid g, W
 // Persistent grid (a multiple of 8 blocks): the blocks of XCD x (blockIdx % 8)
 // share the groups [x, x+1) * ngroups / 8, so neighbouring rows share an L2,
 // and take them one at a time from that XCD's counter.
+// IMPLICIT: the K33 part (isoneutral step); else the explicit step only (skew).
+// WETFLUX: the face fluxes exist for wet cells only (the fused pre formed them
+// from the tensor it wrote, in which every face touching a dry cell carries a
+// zero flux), so a face whose other cell is dry reads as zero; else they come
+// from iso_flux_kernel, which writes every face of a wet cell, including the
+// top face of the dry cell below the lowest wet one.
 #ifndef VX_WET_WAVES
 #define VX_WET_WAVES 5  // 96 VGPRs: five blocks per CU
 #endif
-template <int NTR>
+template <int NTR, bool IMPLICIT, bool WETFLUX>
 __global__ __launch_bounds__(kWetThreads) __attribute__((amdgpu_waves_per_eu(VX_WET_WAVES))) void iso_step_wet_kernel(
     Grid g, Prm p, StepArgs a, WetPlan w) {
     extern __shared__ double lds[];
@@ -1256,19 +1059,19 @@ __global__ __launch_bounds__(kWetThreads) __attribute__((amdgpu_waves_per_eu(VX_
         const int k = kb - 1 + (tid - sOff[q]);
         const bool interior = have && i >= 2 && i <= g.X - 3 && j >= 2 && j <= g.Y - 3;
         const uint32_t c = (uint32_t)i * st.si + (uint32_t)j * st.sj + (uint32_t)k;
-        bool west_ok = false, south_ok = false;
-        if (have) {
+        bool west_ok = have, south_ok = have;
+        if (WETFLUX && have) {
             west_ok = wet(g.kbot[(i - 1) * g.Y + j], k);
             south_ok = wet(g.kbot[i * g.Y + j - 1], k);
         }
 
         // ---- every global read first
-        double fe_c[NTR], fe_w[NTR], fn_c[NTR], fn_s[NTR], ft_c[NTR];
+        double fe_c[NTR], fe_w[NTR], fn_c[NTR], fn_s[NTR], ft_c[NTR], ft_b[NTR];
         double Ic[NTR], Ie[NTR], Iw[NTR], In[NTR], Is[NTR], Iu[NTR], trp[NTR], dtr0[NTR];
 #pragma unroll
         for (int t = 0; t < NTR; ++t) {
             const TracerArgs& ta = a.t[t];
-            fe_c[t] = fe_w[t] = fn_c[t] = fn_s[t] = ft_c[t] = 0.0;
+            fe_c[t] = fe_w[t] = fn_c[t] = fn_s[t] = ft_c[t] = ft_b[t] = 0.0;
             Ic[t] = Ie[t] = Iw[t] = In[t] = Is[t] = Iu[t] = trp[t] = dtr0[t] = 0.0;
             if (have) {
                 fe_c[t] = fe_at(g, ta, i, j, c);
@@ -1279,6 +1082,8 @@ __global__ __launch_bounds__(kWetThreads) __attribute__((amdgpu_waves_per_eu(VX_
                     trp[t] = ta.tr[3u * c + a.taup1];
                     dtr0[t] = ta.dtr[c];
                     ft_c[t] = ta.ft[c];
+                    // the top face of the dry cell below the lowest wet cell
+                    if (!WETFLUX && k == kb - 1 && k >= 1) ft_b[t] = ta.ft[c - 1];
                 }
                 if (energy) {
                     const double* __restrict__ I = ta.I + a.tau;
@@ -1292,7 +1097,7 @@ __global__ __launch_bounds__(kWetThreads) __attribute__((amdgpu_waves_per_eu(VX_
             }
         }
         double K33k = 0.0, K33m = 0.0;
-        if (interior) {
+        if (IMPLICIT && interior) {
             K33k = a.K33[c];
             if (k >= kb) K33m = a.K33[c - 1];  // the edge row k = kbot-1 has no sub-diagonal
         }
@@ -1322,7 +1127,7 @@ __global__ __launch_bounds__(kWetThreads) __attribute__((amdgpu_waves_per_eu(VX_
                 }
             }
         }
-        if (interior) {
+        if (IMPLICIT && interior) {
             // rows of the K33 system (veros/core/isoneutral/diffusion.py:154-164); the
             // edge row k = kbot-1 takes b_edge and a = 0 (veros/core/operators.py:72)
             const double delta = (k < nz - 1) ? dt / dzw_k * K33k : 0.0;
@@ -1342,8 +1147,9 @@ __global__ __launch_bounds__(kWetThreads) __attribute__((amdgpu_waves_per_eu(VX_
             dtrk[t] = rhs[t] = 0.0;
             if (interior) {
                 double* sTt = sT + t * 4 * T;
-                // below the lowest wet cell the top-face flux is zero (its maskW)
-                const double ftm = (k >= kb) ? sTt[lo - 1] : 0.0;
+                // below the lowest wet cell: with the fused pre's fluxes the
+                // top face there is zero (its maskW), else it was read above
+                const double ftm = (k >= kb) ? sTt[lo - 1] : ft_b[t];
                 const double hor = (fe_c[t] - fe_w[t]) / (cost_j * dxt_i) +
                                    (fn_c[t] - fn_s[t]) / (cost_j * dyt_j);
                 const double vert = (k == 0) ? ft_c[t] / dzt_k : (ft_c[t] - ftm) / dzt_k;
@@ -1355,7 +1161,7 @@ __global__ __launch_bounds__(kWetThreads) __attribute__((amdgpu_waves_per_eu(VX_
         __syncthreads();
 
         // ---- phase 2: a lane per (interior column, tracer) sweeps in LDS
-        for (int lin = tid; lin < ncolg * NTR; lin += T) {
+        for (int lin = tid; IMPLICIT && lin < ncolg * NTR; lin += T) {
             const int cq = lin % ncolg, t = lin / ncolg;
             int ci, cj;
             wet_col_ij(g, sCol[cq], ci, cj);
@@ -1410,8 +1216,12 @@ __global__ __launch_bounds__(kWetThreads) __attribute__((amdgpu_waves_per_eu(VX_
                     if (interior && k < nz - 1) {
                         const double fxa = (-Iu[t] + Ic[t]) / dzw_k;
                         const double gr = -p.grav / p.rho_0;
-                        const double dtrz = sTt[2 * T + lo + 1] - sTt[2 * T + lo];
-                        Pk = Pk + gr * fxa * (ft_c[t] * 1.0 + K33k * dtrz / dzw_k * 1.0);
+                        if constexpr (IMPLICIT) {
+                            const double dtrz = sTt[2 * T + lo + 1] - sTt[2 * T + lo];
+                            Pk = Pk + gr * fxa * (ft_c[t] * 1.0 + K33k * dtrz / dzw_k * 1.0);
+                        } else {
+                            Pk = Pk + gr * fxa * ft_c[t] * 1.0;
+                        }
                     }
                 }
             }
@@ -1776,24 +1586,73 @@ dim3 cell_grid(int ni, int nj, dim3 blk, int sb = kStripBlocks) {
 namespace vx {
 namespace {
 
-constexpr int kTileColumns = VX_TILE_COLUMNS;
+
+// The wet-cell plan after the tracers' face fluxes in a step's workspace.
+int64_t wet_plan_ncols(int64_t X, int64_t Y) { return wet_enum_cols(X, Y); }
+int64_t wet_plan_nchunks(int64_t X, int64_t Y) { return (wet_plan_ncols(X, Y) + kScanChunk - 1) / kScanChunk; }
+int64_t wet_plan_bytes(int64_t X, int64_t Y) {
+    const int64_t ints = 3 * wet_plan_ncols(X, Y) + 2 + 12;
+    return (ints * (int64_t)sizeof(int) + 7) / 8 * 8 + wet_plan_nchunks(X, Y) * (int64_t)sizeof(int2);
+}
+WetPlan wire_wet_plan(const Grid& g, void* ws, int ntr) {
+    const int64_t ncols = wet_plan_ncols(g.X, g.Y);
+    char* base = static_cast<char*>(ws) + 3 * ntr * (int64_t)g.X * g.Y * g.nz * (int64_t)sizeof(double);
+    WetPlan w;
+    int* ip = reinterpret_cast<int*>(base);
+    w.wetcol = ip;
+    w.wetoff = ip + ncols;
+    w.first = ip + 2 * ncols;
+    w.nb = ip + 3 * ncols + 2;
+    const int64_t ints = 3 * ncols + 2 + 12;
+    w.chunk = reinterpret_cast<int2*>(base + (ints * (int64_t)sizeof(int) + 7) / 8 * 8);
+    return w;
+}
+
+// The wet-cell plan of the step from kbot (three small kernels on `s`).
+int launch_wet_plan(const Grid& g, const WetPlan& wp, hipStream_t s, const char* who) {
+    const int64_t ncols = wet_plan_ncols(g.X, g.Y);
+    const int nchunks = (int)wet_plan_nchunks(g.X, g.Y);
+    const int C = kWetThreads - g.nz + 1;  // >= nz as nz <= kMaxNz
+    hipLaunchKernelGGL(wet_scan_count_kernel, dim3(nchunks), dim3(kWetThreads), 0, s, g, wp, ncols);
+    int rc = check_launch(who);
+    if (rc) return rc;
+    hipLaunchKernelGGL(wet_scan_chunks_kernel, dim3(1), dim3(kWetThreads), 0, s, wp, nchunks);
+    if ((rc = check_launch(who))) return rc;
+    hipLaunchKernelGGL(wet_scan_columns_kernel, dim3(nchunks), dim3(kWetThreads), 0, s, g, wp, ncols, C);
+    return check_launch(who);
+}
+
+#ifndef VX_WET_BPC
+#define VX_WET_BPC 5
+#endif
+// The tracer step over the wet cells: persistent, VX_WET_BPC blocks per CU (as
+// many as fit), a multiple of 8 in all (measured on 1440x720x80: 3 blocks per
+// CU 9.0 ms for the fused pre + temperature/salinity call, 4 8.45 ms, 5 at
+// five waves per SIMD 8.29 ms).
+template <int NTR, bool IMPLICIT, bool WETFLUX>
+int launch_wet_step(const Grid& g, const Prm& p, const StepArgs& a, const WetPlan& wp, hipStream_t s,
+                    const char* who) {
+    const int ncu = device_cu_count();
+    const unsigned nblk = (unsigned)(((ncu > 0 ? ncu : 256) * VX_WET_BPC + 7) / 8 * 8);
+    const size_t lds = (size_t)(3 + 4 * NTR) * kWetThreads * sizeof(double) + (2 * kWetThreads + 1) * sizeof(int);
+    hipLaunchKernelGGL((iso_step_wet_kernel<NTR, IMPLICIT, WETFLUX>), dim3(nblk), dim3(kWetThreads), lds, s, g, p, a,
+                       wp);
+    return check_launch(who);
+}
 
 template <int NTR, bool IMPLICIT>
-int launch_step(const Grid& g, const Prm& p, const StepArgs& a, hipStream_t s, const char* who) {
-    constexpr int TC = kTileColumns;
+int launch_step(const Grid& g, const Prm& p, const StepArgs& a, void* workspace, hipStream_t s, const char* who) {
     if ((int64_t)g.X * g.Y * g.nz * 4 >= ((int64_t)1 << 32))
         return fail(VX_ERR_UNSUPPORTED, "%s: grid too large for 32-bit offsets", who);
     if (g.nz > kMaxNz) return fail(VX_ERR_UNSUPPORTED, "%s: nz=%d too deep", who, g.nz);
+    const WetPlan wp = wire_wet_plan(g, workspace, NTR);
+    int rc = launch_wet_plan(g, wp, s, who);
+    if (rc) return rc;
     const dim3 fblk = cell_block(g.nz);
     const size_t flux_lds = NTR * (size_t)(3 * fblk.y + 2) * g.nz * sizeof(double);
     hipLaunchKernelGGL((iso_flux_kernel<NTR, IMPLICIT>), cell_grid(g.X - 3, g.Y - 3, fblk), fblk, flux_lds, s, g, a);
-    int rc = check_launch(who);
-    if (rc) return rc;
-    const dim3 blk(g.nz, TC);
-    const size_t lds = (size_t)(3 + 4 * NTR) * TC * g.nz * sizeof(double);
-    hipLaunchKernelGGL((iso_step_tile_kernel<TC, NTR, IMPLICIT>), cell_grid(g.X - 2, g.Y - 2, blk), blk, lds, s, g,
-                       p, a);
-    return check_launch(who);
+    if ((rc = check_launch(who))) return rc;
+    return launch_wet_step<NTR, IMPLICIT, false>(g, p, a, wp, s, who);
 }
 
 int check_common(const vx_grid* grid, const vx_iso_params* prm, int32_t tau, int32_t taup1, Grid& g,
@@ -1856,27 +1715,6 @@ int launch_pre(const Grid& g, const Prm& p, const Fields& f, const double* K_iso
     return dry_stream ? VX_OK : join_aux(s, sd, who);
 }
 
-// The wet-cell plan after the two tracers' fluxes in the fused call's workspace.
-int64_t wet_plan_ncols(int64_t X, int64_t Y) { return wet_enum_cols(X, Y); }
-int64_t wet_plan_nchunks(int64_t X, int64_t Y) { return (wet_plan_ncols(X, Y) + kScanChunk - 1) / kScanChunk; }
-int64_t wet_plan_bytes(int64_t X, int64_t Y) {
-    const int64_t ints = 3 * wet_plan_ncols(X, Y) + 2 + 12;
-    return (ints * (int64_t)sizeof(int) + 7) / 8 * 8 + wet_plan_nchunks(X, Y) * (int64_t)sizeof(int2);
-}
-WetPlan wire_wet_plan(const Grid& g, void* ws) {
-    const int64_t ncols = wet_plan_ncols(g.X, g.Y);
-    char* base = static_cast<char*>(ws) + 6 * (int64_t)g.X * g.Y * g.nz * (int64_t)sizeof(double);
-    WetPlan w;
-    int* ip = reinterpret_cast<int*>(base);
-    w.wetcol = ip;
-    w.wetoff = ip + ncols;
-    w.first = ip + 2 * ncols;
-    w.nb = ip + 3 * ncols + 2;
-    const int64_t ints = 3 * ncols + 2 + 12;
-    w.chunk = reinterpret_cast<int2*>(base + (ints * (int64_t)sizeof(int) + 7) / 8 * 8);
-    return w;
-}
-
 void wire_workspace(const Grid& g, StepArgs& a, int ntr, void* ws) {
     const int64_t n3 = (int64_t)g.X * g.Y * g.nz;
     double* w = static_cast<double*>(ws);
@@ -1913,7 +1751,7 @@ extern "C" int vx_iso_diffusion_f64(void* stream, const vx_grid* grid, const vx_
     a.Kskew = nullptr;
     a.Aez = Ai_ez, a.Anz = Ai_nz, a.Abx = Ai_bx, a.Aby = Ai_by, a.K11 = K_11, a.K22 = K_22, a.K33 = K_33;
     a.P = P_diss;
-    return launch_step<1, true>(g, make_prm(prm), a, (hipStream_t)stream, who);
+    return launch_step<1, true>(g, make_prm(prm), a, workspace, (hipStream_t)stream, who);
 }
 
 extern "C" int vx_iso_diffusion_ts_f64(void* stream, const vx_grid* grid, const vx_iso_params* prm, double* temp,
@@ -1947,7 +1785,7 @@ extern "C" int vx_iso_diffusion_ts_f64(void* stream, const vx_grid* grid, const 
     a.Kskew = nullptr;
     a.Aez = Ai_ez, a.Anz = Ai_nz, a.Abx = Ai_bx, a.Aby = Ai_by, a.K11 = K_11, a.K22 = K_22, a.K33 = K_33;
     a.P = P_diss;
-    return launch_step<2, true>(g, make_prm(prm), a, (hipStream_t)stream, who);
+    return launch_step<2, true>(g, make_prm(prm), a, workspace, (hipStream_t)stream, who);
 }
 
 extern "C" int vx_iso_skew_diffusion_f64(void* stream, const vx_grid* grid, const vx_iso_params* prm,
@@ -1975,17 +1813,17 @@ extern "C" int vx_iso_skew_diffusion_f64(void* stream, const vx_grid* grid, cons
     a.Kskew = K_gm;
     a.Aez = Ai_ez, a.Anz = Ai_nz, a.Abx = Ai_bx, a.Aby = Ai_by, a.K11 = K_11, a.K22 = K_22, a.K33 = nullptr;
     a.P = P_diss_skew;
-    return launch_step<1, false>(g, make_prm(prm), a, (hipStream_t)stream, who);
+    return launch_step<1, false>(g, make_prm(prm), a, workspace, (hipStream_t)stream, who);
 }
 
 extern "C" int64_t vx_iso_workspace_bytes(const vx_grid* grid, int32_t ntracers) {
-    if (!grid || ntracers < 1 || ntracers > 2) return -1;
-    return 3 * (int64_t)ntracers * (grid->nx + 4) * (grid->ny + 4) * (int64_t)grid->nz * (int64_t)sizeof(double);
+    if (!grid || ntracers < 1 || ntracers > 2 || grid->nx < 1 || grid->ny < 1 || grid->nz < 1) return -1;
+    return 3 * (int64_t)ntracers * (grid->nx + 4) * (grid->ny + 4) * (int64_t)grid->nz * (int64_t)sizeof(double) +
+           vx::wet_plan_bytes(grid->nx + 4, grid->ny + 4);
 }
 
 extern "C" int64_t vx_iso_pre_diffusion_ts_workspace_bytes(const vx_grid* grid) {
-    if (!grid || grid->nx < 1 || grid->ny < 1 || grid->nz < 1) return -1;
-    return vx_iso_workspace_bytes(grid, 2) + vx::wet_plan_bytes(grid->nx + 4, grid->ny + 4);
+    return vx_iso_workspace_bytes(grid, 2);
 }
 
 extern "C" int64_t vx_iso_adjoint_workspace_bytes(const vx_grid* grid) {
@@ -2079,35 +1917,15 @@ extern "C" int vx_iso_pre_diffusion_ts_f64(void* stream, const vx_grid* grid, co
     hipStream_t s = (hipStream_t)stream;
     const PreFlux fo{{a.t[0].fe, a.t[1].fe}, {a.t[0].fn, a.t[1].fn}, {a.t[0].ft, a.t[1].ft}};
     // the wet-cell plan of the step (reads kbot only)
-    const WetPlan wp = wire_wet_plan(g, workspace);
-    const int64_t ncols = wet_plan_ncols(g.X, g.Y);
-    const int nchunks = (int)wet_plan_nchunks(g.X, g.Y);
-    const int C = kWetThreads - g.nz + 1;  // >= nz as nz <= kMaxNz
-    hipLaunchKernelGGL(wet_scan_count_kernel, dim3(nchunks), dim3(kWetThreads), 0, s, g, wp, ncols);
-    if ((rc = check_launch(who))) return rc;
-    hipLaunchKernelGGL(wet_scan_chunks_kernel, dim3(1), dim3(kWetThreads), 0, s, wp, nchunks);
-    if ((rc = check_launch(who))) return rc;
-    hipLaunchKernelGGL(wet_scan_columns_kernel, dim3(nchunks), dim3(kWetThreads), 0, s, g, wp, ncols, C);
-    if ((rc = check_launch(who))) return rc;
+    const WetPlan wp = wire_wet_plan(g, workspace, 2);
+    if ((rc = launch_wet_plan(g, wp, s, who))) return rc;
     // The step kernel reads no output of a dry cell (its fluxes come from the
     // wet-cell kernel, and it reads K_33 only on wet cells), so the dry-cell
     // zero fill overlaps the whole call and is joined at its end.
     hipStream_t sd = s;
     rc = launch_pre<true>(g, p, Fields{temp, salt, tau}, K_iso, Ai_ez, Ai_nz, Ai_bx, Ai_by, K_11, K_22, K_33, fo, s,
                           who, &sd);
-    if (!rc) {
-#ifndef VX_WET_BPC
-#define VX_WET_BPC 5
-#endif
-        // persistent: VX_WET_BPC blocks per CU (as many as fit), a multiple
-        // of 8 in all (measured on 1440x720x80: 3 blocks per CU 9.0 ms for
-        // the call, 4 8.45 ms, 5 at five waves per SIMD 8.29 ms)
-        const int ncu = device_cu_count();
-        const unsigned nblk = (unsigned)(((ncu > 0 ? ncu : 256) * VX_WET_BPC + 7) / 8 * 8);
-        const size_t lds = (size_t)(3 + 4 * 2) * kWetThreads * sizeof(double) + (2 * kWetThreads + 1) * sizeof(int);
-        hipLaunchKernelGGL(iso_step_wet_kernel<2>, dim3(nblk), dim3(kWetThreads), lds, s, g, p, a, wp);
-        rc = check_launch(who);
-    }
+    if (!rc) rc = launch_wet_step<2, true, true>(g, p, a, wp, s, who);
     const int rj = join_aux(s, sd, who);
     return rc ? rc : rj;
 }
