@@ -829,6 +829,145 @@ __global__ __launch_bounds__(256) void iso_flux_kernel(Grid g, StepArgs a) {
     }
 }
 
+// The same fluxes over the cells that need one, compacted like the wet-cell
+// pre kernel: a column (i, j) needs levels from the lowest of its own wet
+// cells, the top face of the dry cell below them, and the wet cells of its east
+// and north neighbours, up to nz-1 (one contiguous run); a block owns ncb
+// columns of row i and deals their runs round-robin to its threads. The
+// tracer stencil is read straight from global memory (every read issued
+// before the first flux), so no lane of a dry region is launched.
+__device__ __forceinline__ int first_wet(int kb, int nz) { return (kb >= 1 && kb <= nz) ? kb - 1 : nz; }
+__device__ __forceinline__ int flux_run_start(const Grid& g, int i, int j) {
+    const int nz = g.nz;
+    const int fc = first_wet(g.kbot[i * g.Y + j], nz);
+    int s0 = fc < nz ? (fc >= 1 ? fc - 1 : 0) : nz;             // top face of the dry cell below
+    const int fe = j >= 2 ? first_wet(g.kbot[(i + 1) * g.Y + j], nz) : nz;
+    const int fn = i >= 2 ? first_wet(g.kbot[i * g.Y + j + 1], nz) : nz;
+    s0 = fe < s0 ? fe : s0;
+    return fn < s0 ? fn : s0;
+}
+
+template <int NTR, bool ISO>
+__global__ __launch_bounds__(256) void iso_flux_wet_kernel(Grid g, StepArgs a, int ncb) {
+    extern __shared__ int flux_counts[];  // prefix sums of the columns' run lengths
+    const BlockCols bc = block_cols(1, g.X - 3, 1, g.Y - 3, ncb, kPreStripBlocks);
+    if (!bc.valid) return;  // padding block (uniform)
+    const int nz = g.nz, i = bc.i, jb = bc.j;
+    const int tid = (int)threadIdx.x, nthr = (int)blockDim.x;
+    int* sW = flux_counts;
+    for (int q = tid; q < ncb; q += nthr) sW[q + 1] = jb + q <= g.Y - 3 ? nz - flux_run_start(g, i, jb + q) : 0;
+    __syncthreads();
+    if (tid == 0) {
+        sW[0] = 0;
+        for (int q = 0; q < ncb; ++q) sW[q + 1] += sW[q];
+    }
+    __syncthreads();
+    const int ncell = sW[ncb];
+    const uint32_t si = (uint32_t)g.Y * (uint32_t)nz, sj = (uint32_t)nz;
+    const double* __restrict__ Kf = ISO ? a.Kiso : a.Kskew;
+    for (int t = tid; t < ncell; t += nthr) {
+        int q = 0, hi = ncb - 1;
+        while (q < hi) {
+            const int mid = (q + hi + 1) >> 1;
+            if (sW[mid] <= t) q = mid;
+            else hi = mid - 1;
+        }
+        const int j = jb + q;
+        const int k = nz - (sW[q + 1] - sW[q]) + (t - sW[q]);
+        const int kb = g.kbot[i * g.Y + j], kb_e = g.kbot[(i + 1) * g.Y + j], kb_n = g.kbot[i * g.Y + j + 1];
+        const bool wc = wet(kb, k);
+        const bool need_e = j >= 2 && (wc || wet(kb_e, k));
+        const bool need_n = i >= 2 && (wc || wet(kb_n, k));
+        const bool need_t = i >= 2 && j >= 2 && (wc || (k + 1 < nz && wet(kb, k + 1)));
+        if (!(need_e || need_n || need_t)) continue;
+        const uint32_t c = (uint32_t)i * si + (uint32_t)j * sj + (uint32_t)k;
+        const bool hm = k >= 1, hp = k + 1 < nz;
+        const bool top = need_t && hp;
+        const uint32_t up = hp ? 1u : 0u, dn = hm ? 1u : 0u;  // pad_z_edges clamping
+        // every global read before the first flux
+        FaceIn in{};
+        in.K_c = Kf[c];
+        in.K_cm = hm ? Kf[c - 1] : 0.0;
+        in.dzt = g.dzt[k];
+        in.cost = g.cost[j];
+        in.dxu = g.dxu[i];
+        in.cosu = g.cosu[j];
+        in.cosu_s = g.cosu[j - 1];
+        in.dyu = g.dyu[j];
+        in.dxt = g.dxt[i];
+        in.dyt = g.dyt[j];
+        if (need_e) {
+            in.K_e = Kf[c + si];
+            in.K_em = hm ? Kf[c + si - 1] : 0.0;
+            in.ez01 = *reinterpret_cast<const double2*>(a.Aez + 4u * c);
+            in.ez23 = *reinterpret_cast<const double2*>(a.Aez + 4u * c + 2);
+            in.K11 = a.K11[c];
+        }
+        if (need_n) {
+            in.K_n = Kf[c + sj];
+            in.K_nm = hm ? Kf[c + sj - 1] : 0.0;
+            in.nz01 = *reinterpret_cast<const double2*>(a.Anz + 4u * c);
+            in.nz23 = *reinterpret_cast<const double2*>(a.Anz + 4u * c + 2);
+            in.K22 = a.K22[c];
+        }
+        if (top) {
+            in.bx01 = *reinterpret_cast<const double2*>(a.Abx + 4u * c);
+            in.bx23 = *reinterpret_cast<const double2*>(a.Abx + 4u * c + 2);
+            in.by01 = *reinterpret_cast<const double2*>(a.Aby + 4u * c);
+            in.by23 = *reinterpret_cast<const double2*>(a.Aby + 4u * c + 2);
+        }
+        // tracer at level tau: centre and east/north columns at k, k+up, k-dn;
+        // west/south at k, k+1 (top faces only)
+        double Tc[NTR], Tcu[NTR], Tcd[NTR], Te[NTR], Teu[NTR], Ted[NTR], Tn[NTR], Tnu[NTR], Tnd[NTR];
+        double Tw[NTR], Twu[NTR], Ts[NTR], Tsu[NTR];
+#pragma unroll
+        for (int r = 0; r < NTR; ++r) {
+            const double* __restrict__ T = a.t[r].tr + a.tau;
+            Tc[r] = T[3u * c];
+            Tcu[r] = T[3u * (c + up)];
+            Tcd[r] = T[3u * (c - dn)];
+            Te[r] = Teu[r] = Ted[r] = Tn[r] = Tnu[r] = Tnd[r] = Tw[r] = Twu[r] = Ts[r] = Tsu[r] = 0.0;
+            if (need_e || top) {
+                Te[r] = T[3u * (c + si)];
+                Teu[r] = T[3u * (c + si + up)];
+            }
+            if (need_e) Ted[r] = T[3u * (c + si - dn)];
+            if (need_n || top) {
+                Tn[r] = T[3u * (c + sj)];
+                Tnu[r] = T[3u * (c + sj + up)];
+            }
+            if (need_n) Tnd[r] = T[3u * (c + sj - dn)];
+            if (top) {
+                Tw[r] = T[3u * (c - si)];
+                Twu[r] = T[3u * (c - si + 1)];
+                Ts[r] = T[3u * (c - sj)];
+                Tsu[r] = T[3u * (c - sj + 1)];
+            }
+        }
+        // flux_east (:22-47), flux_north (:49-77), flux_top (:79-111)
+        const double dl_e = hm ? 0.25 * (k1_of<ISO>(in.K_c) + k1_of<ISO>(in.K_cm) + k1_of<ISO>(in.K_e) +
+                                         k1_of<ISO>(in.K_em))
+                               : 0.5 * (k1_of<ISO>(in.K_c) + k1_of<ISO>(in.K_e));
+        const double dl_n = hm ? 0.25 * (k1_of<ISO>(in.K_c) + k1_of<ISO>(in.K_cm) + k1_of<ISO>(in.K_n) +
+                                         k1_of<ISO>(in.K_nm))
+                               : 0.5 * (k1_of<ISO>(in.K_c) + k1_of<ISO>(in.K_n));
+#pragma unroll
+        for (int r = 0; r < NTR; ++r) {
+            if (need_e)
+                a.t[r].fe[c] = east_flux_core(dl_e, in.ez01, in.ez23, Tc[r], Tcu[r], Tcd[r], Te[r], Teu[r], Ted[r],
+                                              in.dzt, in.cost, in.dxu, in.K11);
+            if (need_n)
+                a.t[r].fn[c] = north_flux_core(dl_n, in.nz01, in.nz23, Tc[r], Tcu[r], Tcd[r], Tn[r], Tnu[r], Tnd[r],
+                                               in.dzt, in.cosu, in.dyu, in.K22);
+            if (need_t)
+                a.t[r].ft[c] = top ? top_flux_core(k2_of<ISO>(in.K_c), in.bx01, in.bx23, in.by01, in.by23, Tc[r],
+                                                   Tcu[r], Tw[r], Twu[r], Te[r], Teu[r], Ts[r], Tsu[r], Tn[r], Tnu[r],
+                                                   in.cost, in.cosu, in.cosu_s, in.dxt, in.dyt)
+                                   : 0.0;
+        }
+    }
+}
+
 // Workspace fluxes with their valid index ranges (0 elsewhere).
 __device__ __forceinline__ double fe_at(const Grid& g, const TracerArgs& t, int i, int j, uint32_t o) {
     return (i >= 1 && i <= g.X - 3 && j >= 2 && j <= g.Y - 3) ? t.fe[o] : 0.0;
@@ -1648,9 +1787,19 @@ int launch_step(const Grid& g, const Prm& p, const StepArgs& a, void* workspace,
     const WetPlan wp = wire_wet_plan(g, workspace, NTR);
     int rc = launch_wet_plan(g, wp, s, who);
     if (rc) return rc;
-    const dim3 fblk = cell_block(g.nz);
-    const size_t flux_lds = NTR * (size_t)(3 * fblk.y + 2) * g.nz * sizeof(double);
-    hipLaunchKernelGGL((iso_flux_kernel<NTR, IMPLICIT>), cell_grid(g.X - 3, g.Y - 3, fblk), fblk, flux_lds, s, g, a);
+#ifndef VX_FLUX_WET
+#define VX_FLUX_WET 1
+#endif
+    if (VX_FLUX_WET) {
+        const int ncb = pre_columns(g.nz);
+        hipLaunchKernelGGL((iso_flux_wet_kernel<NTR, IMPLICIT>), cell_grid(g.X - 3, g.Y - 3, dim3(1, ncb), kPreStripBlocks),
+                           dim3(256), (size_t)(ncb + 1) * sizeof(int), s, g, a, ncb);
+    } else {
+        const dim3 fblk = cell_block(g.nz);
+        const size_t flux_lds = NTR * (size_t)(3 * fblk.y + 2) * g.nz * sizeof(double);
+        hipLaunchKernelGGL((iso_flux_kernel<NTR, IMPLICIT>), cell_grid(g.X - 3, g.Y - 3, fblk), fblk, flux_lds, s, g,
+                           a);
+    }
     if ((rc = check_launch(who))) return rc;
     return launch_wet_step<NTR, IMPLICIT, false>(g, p, a, wp, s, who);
 }
