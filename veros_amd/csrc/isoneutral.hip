@@ -219,49 +219,6 @@ __device__ __forceinline__ CellIdx cell_of(int i0, int ni, int j0, int nj) {
 
 __device__ __forceinline__ double m01(int kb, int k) { return (kb > 0 && kb - 1 <= k) ? 1.0 : 0.0; }
 
-// Stencil columns of a cell block, staged in LDS. A block holds the columns
-// (i, jb .. jb+cpb-1) (cpb = blockDim.y); its stencil reaches the columns
-//   q = 0 .. cpb+1        : (i,   jb-1 .. jb+cpb)   south, own, north
-//   q = cpb+2 .. 2cpb+1   : (i-1, jb .. jb+cpb-1)   west
-//   q = 2cpb+2 .. 3cpb+1  : (i+1, jb .. jb+cpb-1)   east
-// Staging level `lev` of an (X, Y, nz, 3) field once per block replaces the
-// 13 strided loads per cell and field with LDS reads (each global line is
-// fetched once per block instead of once per stencil point). j is clamped
-// into the array: columns past the grid edge are staged but never read.
-__device__ __forceinline__ int stencil_columns(int cpb) { return 3 * cpb + 2; }
-
-__device__ __forceinline__ void stage_level(const Grid& g, const double* __restrict__ f, int lev, int i, int jb,
-                                            double* __restrict__ dst) {
-    // block = (nz levels) x (cpb columns): thread (k, ty) stages level k of
-    // the columns ty, ty + cpb, ...
-    const int nz = g.nz, cpb = (int)blockDim.y, k = (int)threadIdx.x;
-    const int ncol = stencil_columns(cpb);
-    for (int q = (int)threadIdx.y; q < ncol; q += cpb) {
-        int ii = i, jj;
-        if (q < cpb + 2) {
-            jj = jb - 1 + q;
-        } else if (q < 2 * cpb + 2) {
-            ii = i - 1;
-            jj = jb + q - cpb - 2;
-        } else {
-            ii = i + 1;
-            jj = jb + q - 2 * cpb - 2;
-        }
-        jj = jj < 0 ? 0 : (jj >= g.Y ? g.Y - 1 : jj);
-        dst[q * nz + k] = f[3u * (((uint32_t)ii * (uint32_t)g.Y + (uint32_t)jj) * (uint32_t)nz + (uint32_t)k) + (uint32_t)lev];
-    }
-}
-
-// Column pointers of one thread into a staged field.
-struct Cols {
-    const double *c, *e, *w, *n, *s;
-};
-__device__ __forceinline__ Cols cols_of(const double* staged, int nz) {
-    const int ty = (int)threadIdx.y, cpb = (int)blockDim.y;
-    return {staged + (1 + ty) * nz, staged + (2 * cpb + 2 + ty) * nz, staged + (cpb + 2 + ty) * nz,
-            staged + (2 + ty) * nz, staged + ty * nz};
-}
-
 // ---------------------------------------------------------------------------
 // isoneutral_diffusion_pre: thread per cell (i, j, k), i in [1, X-2), j in [1, Y-2).
 // The 13 tracer points of the stencil are loaded once per field; all face
@@ -719,118 +676,15 @@ __device__ __forceinline__ double k1_of(double K) { return ISO ? K : 0.0 - K; }
 template <bool ISO>
 __device__ __forceinline__ double k2_of(double K) { return ISO ? K : 0.0 + K; }
 
-// The face fluxes read the tracer (level tau) from the block's staged
-// stencil columns `tc`.
-// flux_east at (i, j, k), i in [1, X-3], j in [2, Y-3]   (:22-47)
-template <bool ISO>
-__device__ __forceinline__ double flux_east(const FaceIn& in, const Cols& tc, int k, int nz) {
-    const bool hm = k >= 1;
-    const int up = (k + 1 < nz) ? 1 : 0, dn = hm ? 1 : 0;  // pad_z_edges clamping
-    const double diffloc = hm ? 0.25 * (k1_of<ISO>(in.K_c) + k1_of<ISO>(in.K_cm) + k1_of<ISO>(in.K_e) + k1_of<ISO>(in.K_em))
-                              : 0.5 * (k1_of<ISO>(in.K_c) + k1_of<ISO>(in.K_e));
-    return east_flux_core(diffloc, in.ez01, in.ez23, tc.c[k], tc.c[k + up], tc.c[k - dn], tc.e[k], tc.e[k + up],
-                          tc.e[k - dn], in.dzt, in.cost, in.dxu, in.K11);
-}
-
-// flux_north at (i, j, k), i in [2, X-3], j in [1, Y-3]   (:49-77)
-template <bool ISO>
-__device__ __forceinline__ double flux_north(const FaceIn& in, const Cols& tc, int k, int nz) {
-    const bool hm = k >= 1;
-    const int up = (k + 1 < nz) ? 1 : 0, dn = hm ? 1 : 0;
-    const double diffloc = hm ? 0.25 * (k1_of<ISO>(in.K_c) + k1_of<ISO>(in.K_cm) + k1_of<ISO>(in.K_n) + k1_of<ISO>(in.K_nm))
-                              : 0.5 * (k1_of<ISO>(in.K_c) + k1_of<ISO>(in.K_n));
-    return north_flux_core(diffloc, in.nz01, in.nz23, tc.c[k], tc.c[k + up], tc.c[k - dn], tc.n[k], tc.n[k + up],
-                           tc.n[k - dn], in.dzt, in.cosu, in.dyu, in.K22);
-}
-
-// flux_top at an interior (i, j, k), k < nz-1   (:79-111)
-template <bool ISO>
-__device__ __forceinline__ double flux_top(const FaceIn& in, const Cols& tc, int k) {
-    return top_flux_core(k2_of<ISO>(in.K_c), in.bx01, in.bx23, in.by01, in.by23, tc.c[k], tc.c[k + 1], tc.w[k],
-                         tc.w[k + 1], tc.e[k], tc.e[k + 1], tc.s[k], tc.s[k + 1], tc.n[k], tc.n[k + 1], in.cost,
-                         in.cosu, in.cosu_s, in.dxt, in.dyt);
-}
-
-// Fluxes of NTR tracers, thread per cell (i, j) in [1, X-2) x [1, Y-2):
-// flux_east for j >= 2, flux_north for i >= 2, flux_top for i, j >= 2 (0 at the
-// top level). The tracers share every read of the isoneutral tensor.
-// Every use of a face flux is multiplied by the maskT of a cell next to the
-// face (the divergence, the lateral dissipation and the vertical mixing term
-// of that cell), so a face between two dry cells is neither computed nor
-// written; the step kernel reads only faces that touch a wet cell.
-template <int NTR, bool ISO>
-__global__ __launch_bounds__(256) void iso_flux_kernel(Grid g, StepArgs a) {
-    extern __shared__ double lds[];
-    const CellIdx ci = cell_of(1, g.X - 3, 1, g.Y - 3);
-    const int jb = ci.j - (int)threadIdx.y;  // the block's first column
-    if (jb > g.Y - 3) return;                 // padding block (uniform)
-    const int nz = g.nz;
-    const int i = ci.i, j = ci.j, k = ci.k;
-    int kb = 0, kb_e = 0, kb_n = 0;
-    if (ci.valid) {
-        kb = g.kbot[i * g.Y + j];
-        kb_e = g.kbot[(i + 1) * g.Y + j];
-        kb_n = g.kbot[i * g.Y + j + 1];
-    }
-    const int per = stencil_columns((int)blockDim.y) * nz;
-#pragma unroll
-    for (int t = 0; t < NTR; ++t) stage_level(g, a.t[t].tr, a.tau, i, jb, lds + t * per);
-    __syncthreads();
-    if (!ci.valid) return;
-    const bool wc = wet(kb, k);
-    const bool need_e = j >= 2 && (wc || wet(kb_e, k));
-    const bool need_n = i >= 2 && (wc || wet(kb_n, k));
-    const bool need_t = i >= 2 && j >= 2 && (wc || (k + 1 < nz && wet(kb, k + 1)));
-    if (!(need_e || need_n || need_t)) return;
-
-    // every global read before the first flux
-    const uint32_t si = (uint32_t)g.Y * (uint32_t)nz, sj = (uint32_t)nz;
-    const uint32_t c = (uint32_t)i * si + (uint32_t)j * sj + (uint32_t)k;
-    const bool hm = k >= 1;
-    const double* __restrict__ Kf = ISO ? a.Kiso : a.Kskew;
-    FaceIn in{};
-    in.K_c = Kf[c];
-    in.K_cm = hm ? Kf[c - 1] : 0.0;
-    in.dzt = g.dzt[k];
-    in.cost = g.cost[j];
-    in.dxu = g.dxu[i];
-    in.cosu = g.cosu[j];
-    in.cosu_s = g.cosu[j - 1];
-    in.dyu = g.dyu[j];
-    in.dxt = g.dxt[i];
-    in.dyt = g.dyt[j];
-    if (need_e) {
-        in.K_e = Kf[c + si];
-        in.K_em = hm ? Kf[c + si - 1] : 0.0;
-        in.ez01 = *reinterpret_cast<const double2*>(a.Aez + 4u * c);      // ip=0: kr=0, 1
-        in.ez23 = *reinterpret_cast<const double2*>(a.Aez + 4u * c + 2);  // ip=1: kr=0, 1
-        in.K11 = a.K11[c];
-    }
-    if (need_n) {
-        in.K_n = Kf[c + sj];
-        in.K_nm = hm ? Kf[c + sj - 1] : 0.0;
-        in.nz01 = *reinterpret_cast<const double2*>(a.Anz + 4u * c);
-        in.nz23 = *reinterpret_cast<const double2*>(a.Anz + 4u * c + 2);
-        in.K22 = a.K22[c];
-    }
-    const bool top = need_t && k < nz - 1;
-    if (top) {
-        in.bx01 = *reinterpret_cast<const double2*>(a.Abx + 4u * c);
-        in.bx23 = *reinterpret_cast<const double2*>(a.Abx + 4u * c + 2);
-        in.by01 = *reinterpret_cast<const double2*>(a.Aby + 4u * c);
-        in.by23 = *reinterpret_cast<const double2*>(a.Aby + 4u * c + 2);
-    }
-#pragma unroll
-    for (int t = 0; t < NTR; ++t) {
-        const Cols tc = cols_of(lds + t * per, nz);
-        if (need_e) a.t[t].fe[c] = flux_east<ISO>(in, tc, k, nz);
-        if (need_n) a.t[t].fn[c] = flux_north<ISO>(in, tc, k, nz);
-        if (need_t) a.t[t].ft[c] = top ? flux_top<ISO>(in, tc, k) : 0.0;
-    }
-}
-
-// The same fluxes over the cells that need one, compacted like the wet-cell
-// pre kernel: a column (i, j) needs levels from the lowest of its own wet
+// Face fluxes of NTR tracers (veros/core/isoneutral/diffusion.py:9-113):
+// flux_east for (i, j) in [1, X-3] x [2, Y-3], flux_north for [2, X-3] x
+// [1, Y-3], flux_top for [2, X-3] x [2, Y-3] (0 at the top level); the tracers
+// share every read of the isoneutral tensor. Every use of a face flux is
+// multiplied by the maskT of a cell next to the face (the divergence, the
+// lateral dissipation and the vertical mixing term of that cell), so only the
+// faces touching a wet cell are formed and written; the step kernel reads no
+// other. The cells that need one are compacted like the wet-cell pre kernel's:
+// a column (i, j) needs levels from the lowest of its own wet
 // cells, the top face of the dry cell below them, and the wet cells of its east
 // and north neighbours, up to nz-1 (one contiguous run); a block owns ncb
 // columns of row i and deals their runs round-robin to its threads. The
@@ -848,7 +702,7 @@ __device__ __forceinline__ int flux_run_start(const Grid& g, int i, int j) {
 }
 
 template <int NTR, bool ISO>
-__global__ __launch_bounds__(256) void iso_flux_wet_kernel(Grid g, StepArgs a, int ncb) {
+__global__ __launch_bounds__(256) void iso_flux_kernel(Grid g, StepArgs a, int ncb) {
     extern __shared__ int flux_counts[];  // prefix sums of the columns' run lengths
     const BlockCols bc = block_cols(1, g.X - 3, 1, g.Y - 3, ncb, kPreStripBlocks);
     if (!bc.valid) return;  // padding block (uniform)
@@ -1787,19 +1641,9 @@ int launch_step(const Grid& g, const Prm& p, const StepArgs& a, void* workspace,
     const WetPlan wp = wire_wet_plan(g, workspace, NTR);
     int rc = launch_wet_plan(g, wp, s, who);
     if (rc) return rc;
-#ifndef VX_FLUX_WET
-#define VX_FLUX_WET 1
-#endif
-    if (VX_FLUX_WET) {
-        const int ncb = pre_columns(g.nz);
-        hipLaunchKernelGGL((iso_flux_wet_kernel<NTR, IMPLICIT>), cell_grid(g.X - 3, g.Y - 3, dim3(1, ncb), kPreStripBlocks),
-                           dim3(256), (size_t)(ncb + 1) * sizeof(int), s, g, a, ncb);
-    } else {
-        const dim3 fblk = cell_block(g.nz);
-        const size_t flux_lds = NTR * (size_t)(3 * fblk.y + 2) * g.nz * sizeof(double);
-        hipLaunchKernelGGL((iso_flux_kernel<NTR, IMPLICIT>), cell_grid(g.X - 3, g.Y - 3, fblk), fblk, flux_lds, s, g,
-                           a);
-    }
+    const int ncb = pre_columns(g.nz);
+    hipLaunchKernelGGL((iso_flux_kernel<NTR, IMPLICIT>), cell_grid(g.X - 3, g.Y - 3, dim3(1, ncb), kPreStripBlocks),
+                       dim3(256), (size_t)(ncb + 1) * sizeof(int), s, g, a, ncb);
     if ((rc = check_launch(who))) return rc;
     return launch_wet_step<NTR, IMPLICIT, false>(g, p, a, wp, s, who);
 }
