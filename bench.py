@@ -101,6 +101,14 @@ def build_workload(nx, ny, nz, seed=1002):
         vs[k] = torch.zeros((X, Y, nz), **f64)
     for k, v in grid.items():
         vs[k] = torch.tensor(v, **f64)
+    # vertical mixing inputs (veros_amd.workloads.add_vertmix_fields): kappaH ~
+    # LogU(1e-5, 1e-2), surface forcing ~ 1e-5 N(0, 1) (salt 1e-6)
+    lo, hi = math.log(1e-5), math.log(1e-2)
+    vs["kappaH"] = torch.exp(lo + (hi - lo) * torch.rand((X, Y, nz), generator=g, **f64))
+    vs["forc_temp_surface"] = 1e-5 * torch.randn((X, Y), generator=g, **f64)
+    vs["forc_salt_surface"] = 1e-6 * torch.randn((X, Y), generator=g, **f64)
+    vs["dtemp_vmix"] = torch.zeros((X, Y, nz), **f64)
+    vs["dsalt_vmix"] = torch.zeros((X, Y, nz), **f64)
     from types import SimpleNamespace
     state = SimpleNamespace(variables=SimpleNamespace(**vs), settings=settings)
 
@@ -242,7 +250,7 @@ def main():
         sys.exit(2)
     import torch
 
-    from veros_amd import isoneutral, utilities
+    from veros_amd import isoneutral, thermodynamics, utilities
 
     torch.cuda.set_device(0)
     nx, ny, nz = args.nx, args.ny, args.nz
@@ -273,6 +281,9 @@ def main():
     def run_pre_diff():
         isoneutral.isoneutral_pre_and_diffusion_temp_salt(state)
 
+    def run_vertmix():
+        thermodynamics.vertmix_tempsalt(state)
+
     def run_diff_two_calls():
         isoneutral.isoneutral_diffusion(state, vs.temp, True)
         isoneutral.isoneutral_diffusion(state, vs.salt, False)
@@ -291,7 +302,8 @@ def main():
     parts = {}
     for name, fn in (("tdma_fp64", lambda: run_tdma("fp64")), ("tdma_mixed", lambda: run_tdma("mixed")),
                      ("iso_pre", run_pre), ("iso_diffusion_TS", run_diff),
-                     ("iso_diffusion_T_then_S", run_diff_two_calls), ("iso_pre_and_diffusion_TS", run_pre_diff)):
+                     ("iso_diffusion_T_then_S", run_diff_two_calls), ("iso_pre_and_diffusion_TS", run_pre_diff),
+                     ("vertmix_TS", run_vertmix)):
         for _ in range(args.warmup):
             fn()
         parts[name] = timed(fn, max(5, args.steps))
@@ -322,6 +334,11 @@ def main():
     B_diff_two = 2 * (224 * N + 4 * (nx + 4) * (ny + 4))
     B_diff = TS_BYTES_PER_CELL * N + 4 * (nx + 4) * (ny + 4)
     B_pre_diff = PRE_TS_BYTES_PER_CELL * N + 4 * (nx + 4) * (ny + 4)
+    # vertmix_tempsalt: kappaH read and temp/salt[taup1] read and written on the
+    # wet cells (8 + 2 * 16), the two tendencies written on every cell
+    # (2 * 8 over X*Y*nz), kbot and the two forcings per column (4 + 16)
+    _, W_v = tdma_bytes(vs.kbot[2:-2, 2:-2], nz)
+    B_vmix = 40 * W_v + 16 * (nx + 4) * (ny + 4) * nz + 20 * (nx + 4) * (ny + 4)
     rates = {
         "tdma_mcolumns_per_s": {p: C / (parts[f"tdma_{p}"][0] * 1e-3) / 1e6 for p in ("fp64", "mixed")},
         "iso_mcells_per_s": N / (parts["iso_pre_and_diffusion_TS"][0] * 1e-3) / 1e6,
@@ -334,6 +351,7 @@ def main():
         "iso_diffusion_TS": (B_diff, parts["iso_diffusion_TS"][0]),
         "iso_diffusion_T_then_S": (B_diff_two, parts["iso_diffusion_T_then_S"][0]),
         "iso_pre_and_diffusion_TS": (B_pre_diff, parts["iso_pre_and_diffusion_TS"][0]),
+        "vertmix_TS": (B_vmix, parts["vertmix_TS"][0]),
     }
     gbs = {k: b / (t * 1e-3) / 1e9 for k, (b, t) in kern.items()}
     # roofline: pre + the temperature + salinity diffusion in one C-ABI call,

@@ -248,6 +248,23 @@ int vx_iso_pre_diffusion_ts_f64(void* stream, const vx_grid* grid, const vx_iso_
                                 void* workspace);
 
 /*
+ * vertmix_tempsalt (veros/core/thermodynamics.py:248-300): implicit vertical
+ * mixing of temperature and salinity with diffusivity kappaH (X, Y, nz) and
+ * the surface forcing forc_temp_surface / forc_salt_surface (X, Y) on the top
+ * level, in place on temp/salt[..., taup1] over the wet cells of the interior
+ * columns (grid->kbot, dzt, dzw are read; the other grid arrays may be NULL).
+ * dtemp_vmix / dsalt_vmix (X, Y, nz) are overwritten everywhere with
+ * (new - old) / dt_tracer (zero off the wet cells). With enable_cyclic_x the
+ * x halos of level taup1 are then copied as enforce_boundaries does on one
+ * process (veros/core/utilities.py:7-21; needs nx >= 4).
+ */
+int vx_vertmix_tempsalt_f64(void* stream, const vx_grid* grid, double dt_tracer,
+                            double* temp, double* salt, int32_t taup1,
+                            const double* kappaH,
+                            const double* forc_temp_surface, const double* forc_salt_surface,
+                            double* dtemp_vmix, double* dsalt_vmix, int32_t enable_cyclic_x);
+
+/*
  * isoneutral_skew_diffusion for one tracer (veros/core/isoneutral/diffusion.py:172-207,
  * 210-283, 298-307, iso=False): the same flux stencil with K_iso = 0 and
  * K_skew = K_gm, the explicit update of tracer[..., taup1] and dtr_iso, no

@@ -112,6 +112,28 @@ def test_iso_argument_errors_without_gpu():
     assert rc == _native.VX_ERR_INVALID_ARGUMENT  # tau out of range
 
 
+def test_vertmix_argument_errors_without_gpu():
+    L = _native.lib()
+    g = _native.VxGrid()
+    g.nx, g.ny, g.nz = 4, 4, 8
+
+    def call(grid=g, dt=3600.0, temp=1, salt=2, taup1=1, kap=1, cyc=0):
+        return L.vx_vertmix_tempsalt_f64(None, grid, dt, temp, salt, taup1, kap, 1, 1, 3, 4, cyc)
+
+    assert call() == _native.VX_ERR_INVALID_ARGUMENT                # null kbot/dzt/dzw
+    g.kbot, g.dzt, g.dzw = 16, 16, 16
+    assert call(grid=None) == _native.VX_ERR_INVALID_ARGUMENT
+    assert call(taup1=3) == _native.VX_ERR_INVALID_ARGUMENT
+    assert call(temp=2) == _native.VX_ERR_INVALID_ARGUMENT           # temp and salt alias
+    assert call(kap=None) == _native.VX_ERR_INVALID_ARGUMENT
+    assert call(dt=0.0) == _native.VX_ERR_INVALID_ARGUMENT
+    g.nz = _native.max_nz() + 1
+    assert call() == _native.VX_ERR_UNSUPPORTED
+    g.nz, g.nx = 8, 3
+    assert call(cyc=1) == _native.VX_ERR_UNSUPPORTED                 # cyclic halo copy needs nx >= 4
+    assert "vx_vertmix_tempsalt_f64" in L.vx_last_error().decode()
+
+
 def test_batch_plan_argument_errors_without_gpu():
     import ctypes
     L = _native.lib()

@@ -334,6 +334,29 @@ def mixing_config4(seed=1004, nx=180, ny=90, nz=40):
     return st, p, w
 
 
+def add_vertmix_fields(rng, state, *, kappa_lo=1e-5, kappa_hi=1e-2, forc_scale=1e-5, random_aux=False):
+    """Add the fields vertmix_tempsalt reads and writes to an isoneutral state:
+    ``kappaH ~ LogU(kappa_lo, kappa_hi)`` m^2/s per cell (the vertical
+    diffusivity of the standalone TDMA configs), surface forcing
+    ``forc_temp_surface``, ``forc_salt_surface ~ forc_scale * N(0, 1)`` per
+    column (K m/s, g/kg m/s), and the outputs ``dtemp_vmix``, ``dsalt_vmix``
+    (noise with ``random_aux``, so tests see that every cell is written)."""
+    vs = state.variables
+    X, Y, nz = vs.K_iso.shape
+    vs.kappaH = np.exp(rng.uniform(np.log(kappa_lo), np.log(kappa_hi), size=(X, Y, nz)))
+    vs.forc_temp_surface = forc_scale * rng.standard_normal((X, Y))
+    vs.forc_salt_surface = 0.1 * forc_scale * rng.standard_normal((X, Y))
+    vs.dtemp_vmix = rng.standard_normal((X, Y, nz)) if random_aux else np.zeros((X, Y, nz))
+    vs.dsalt_vmix = rng.standard_normal((X, Y, nz)) if random_aux else np.zeros((X, Y, nz))
+    return state
+
+
+def vertmix_config2(seed=1002, nx=1440, ny=720, nz=80):
+    """The 1440x720x80 grid of the mixing step with the vertical-mixing fields."""
+    st = iso_config2(seed, nx, ny, nz)
+    return add_vertmix_fields(np.random.default_rng(seed + 200), st)
+
+
 def config2_settings(nx=1440, ny=720, nz=80):
     """Linear EOS rho = 1025*(1 - 2e-4*T + 7.6e-4*(S-35)) and ACC mixing constants."""
     return MixSettings(nx=nx, ny=ny, nz=nz, drhodT=-1025.0 * 2e-4, drhodS=1025.0 * 7.6e-4)
