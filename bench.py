@@ -25,11 +25,10 @@ REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO))
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per call of vx_iso_diffusion_ts_f64 (its two kernels) and of
-# vx_iso_pre_diffusion_ts_f64 (its four) on 1440x720x80, from the FETCH_SIZE
-# and WRITE_SIZE rocprofv3 passes recorded in profiles/ (read bytes = 2.0 x
-# FETCH_SIZE, write bytes = 1.0 x WRITE_SIZE, calibrated on a 1 GiB device copy)
-TS_TRAFFIC_BYTES = {(1440, 720, 80): 21_040_000_000}
+# HBM bytes per call of vx_iso_pre_diffusion_ts_f64 (its seven kernels) on
+# 1440x720x80, from the FETCH_SIZE and WRITE_SIZE rocprofv3 passes recorded in
+# profiles/ (read bytes = 2.0 x FETCH_SIZE, write bytes = 1.0 x WRITE_SIZE,
+# calibrated on a 1 GiB device copy)
 PRE_TS_TRAFFIC_BYTES = {(1440, 720, 80): 27_142_421_504}
 # algorithmic bytes per interior cell: the fused temperature + salinity step
 # with the energy branch reads K_iso, Ai_* and K_11/22/33 once (8 + 128 + 24),
