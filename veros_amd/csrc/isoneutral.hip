@@ -281,16 +281,22 @@ struct PreFlux {
 // contiguous run per array, so a wave per column zero-fills those runs with
 // lane-contiguous stores (full cache lines) over the index ranges the
 // reference writes; iso_pre_kernel handles the wet cells.
+// The zeros are stored nontemporal: nothing in the call reads them back, and
+// streaming them past the caches leaves the L2 to the wet-cell kernels beside
+// them (measured on 1440x720x80: pre 4.9 -> 4.6 ms, pre + the
+// temperature/salinity step in one call 8.3 -> 7.5 ms; the same stores for the
+// wet cells' tensor made pre slower).
+typedef double vx_d2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void zero_run(double* __restrict__ p, int n, int lane) {
-    for (int m = lane; m < n; m += 64) p[m] = 0.0;
+    for (int m = lane; m < n; m += 64) __builtin_nontemporal_store(0.0, p + m);
 }
 // Ai entries 4 c + 2 ip + kr of the cells 0 .. nd-1 of a column; at level 0
 // only the kr = 1 entries are written (the kr = 0 ones keep their values).
 __device__ __forceinline__ void zero_ai_run(double* __restrict__ p, int nd, int lane, bool keep_level0_kr0) {
     const int n2 = 2 * nd;  // double2 pieces
     for (int m = lane; m < n2; m += 64) {
-        if (keep_level0_kr0 && m < 2) p[2 * m + 1] = 0.0;
-        else *reinterpret_cast<double2*>(p + 2 * m) = make_double2(0.0, 0.0);
+        if (keep_level0_kr0 && m < 2) __builtin_nontemporal_store(0.0, p + 2 * m + 1);
+        else __builtin_nontemporal_store((vx_d2){0.0, 0.0}, reinterpret_cast<vx_d2*>(p + 2 * m));
     }
 }
 
