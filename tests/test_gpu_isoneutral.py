@@ -235,3 +235,35 @@ def test_pre_and_temp_salt_fused_config1():
     isoneutral.isoneutral_pre_and_diffusion_temp_salt(ds_b)
     for k in PRE_KEYS + ("temp", "salt", "dtemp_iso", "dsalt_iso", "P_diss_iso"):
         assert torch.equal(getattr(ds_a.variables, k), getattr(ds_b.variables, k)), k
+
+
+def test_pre_and_temp_salt_fused_full_grid_equals_separate_calls():
+    """1440x720x80 (the mixed step's grid, 35% land, kbot U{5..80}): the one
+    call equals pre followed by the temperature + salinity call value for
+    value on every output, and a vertically and horizontally uniform tracer
+    is left unchanged up to the rounding of the implicit K33 solve (all its
+    face fluxes vanish, and the K33 rows sum to one)."""
+    import sys
+    from pathlib import Path
+    sys.path.insert(0, str(Path(__file__).resolve().parent.parent))
+    import bench
+    from veros_amd import isoneutral
+    from types import SimpleNamespace
+    st_a, _ = bench.build_workload(1440, 720, 80)
+    va = st_a.variables
+    vb = SimpleNamespace(**{k: (v.clone() if isinstance(v, torch.Tensor) else v) for k, v in vars(va).items()})
+    st_b = SimpleNamespace(variables=vb, settings=st_a.settings)
+    isoneutral.isoneutral_diffusion_pre(st_a)
+    isoneutral.isoneutral_diffusion_temp_salt(st_a)
+    isoneutral.isoneutral_pre_and_diffusion_temp_salt(st_b)
+    torch.cuda.synchronize()
+    for k in PRE_KEYS + ("temp", "salt", "dtemp_iso", "dsalt_iso", "P_diss_iso"):
+        assert torch.equal(getattr(va, k), getattr(vb, k)), k
+    del st_a, va
+    # uniform salinity: no flux, so salt[taup1] and dsalt_iso stay as they were
+    vb.salt.fill_(35.0)
+    vb.dsalt_iso.zero_()
+    isoneutral.isoneutral_pre_and_diffusion_temp_salt(st_b)
+    torch.cuda.synchronize()
+    assert (vb.salt[..., vb.taup1] - 35.0).abs().max().item() <= 35.0 * 1e-13
+    assert vb.dsalt_iso.abs().max().item() <= 35.0 * 1e-13 / st_b.settings.dt_tracer
