@@ -29,7 +29,7 @@ HBM_PEAK_GBS = 8000.0
 # 1440x720x80, from the FETCH_SIZE and WRITE_SIZE rocprofv3 passes recorded in
 # profiles/ (read bytes = 2.0 x FETCH_SIZE, write bytes = 1.0 x WRITE_SIZE,
 # calibrated on a 1 GiB device copy)
-PRE_TS_TRAFFIC_BYTES = {(1440, 720, 80): 27_142_421_504}
+PRE_TS_TRAFFIC_BYTES = {(1440, 720, 80): 27_204_216_832}
 # algorithmic bytes per interior cell: the fused temperature + salinity step
 # with the energy branch reads K_iso, Ai_* and K_11/22/33 once (8 + 128 + 24),
 # P_diss read and written (16), and per tracer tr[tau], tr[taup1] read and
