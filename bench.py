@@ -244,8 +244,9 @@ def main():
     if args.batched:
         return run_batched(args)
     if args.gpus != 1:
-        print(json.dumps({"error": "this build targets one GPU; --gpus above 1 is not supported "
-                                   "(columns shard with no communication, see DESIGN.md)"}))
+        print(json.dumps({"error": "one GPU only: --gpus above 1 is not supported (the TDMA columns would "
+                                   "shard with no communication, the stencils by 2-D blocks with a 2-cell "
+                                   "halo exchange of T and S per step; not built)"}))
         sys.exit(2)
     import torch
 
