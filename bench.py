@@ -217,11 +217,6 @@ def run_batched(args):
             for s0 in systems:
                 utilities.solve_implicit_kbot(s0["a"], s0["b"], s0["c"], s0["d"], s0["kbot"], precision=prec,
                                               refine_steps=1, out=s0["x"])
-        if prec == "mixed":
-            # batched mixed plans are disabled (vx_tdma_plan_create mode 1): sequential launches only
-            res[prec] = {"sequential_64_launches_ms": round(timed(sequential, args.steps), 4),
-                         "plan": "disabled"}
-            continue
         plan = TdmaBatch(systems, prec)
         r = {"sequential_64_launches_ms": timed(sequential, args.steps),
              "plan_untuned_ms": timed(plan.run, args.steps)}

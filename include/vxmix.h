@@ -126,8 +126,9 @@ typedef struct vx_tdma_system {
 
 typedef struct vx_tdma_plan vx_tdma_plan;
 
-/* mode 0: float64 Thomas (vx_tdma_f64). mode 1 (mixed with one refinement
- * step) is reserved and currently returns VX_ERR_UNSUPPORTED.
+/* mode 0: float64 Thomas (vx_tdma_f64), one persistent launch per nz bucket.
+ * mode 1: float32 sweeps with one float64 refinement step (vx_tdma_mixed),
+ * one launch per system (captured together by vx_tdma_plan_graph_launch).
  * Validation errors leave *out NULL. */
 int vx_tdma_plan_create(const vx_tdma_system* systems, int32_t nsys, int32_t mode, vx_tdma_plan** out);
 /* Solve every system: one launch per non-empty nz bucket, on `stream`. */

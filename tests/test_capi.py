@@ -141,7 +141,7 @@ def test_batch_plan_argument_errors_without_gpu():
     arr = (_native.VxTdmaSystem * 1)()
     assert L.vx_tdma_plan_create(arr, 0, 0, ctypes.byref(h)) == _native.VX_ERR_INVALID_ARGUMENT
     assert L.vx_tdma_plan_create(arr, 1, 5, ctypes.byref(h)) == _native.VX_ERR_INVALID_ARGUMENT
-    assert L.vx_tdma_plan_create(arr, 1, 1, ctypes.byref(h)) == _native.VX_ERR_UNSUPPORTED  # mixed: disabled
+    assert L.vx_tdma_plan_create(arr, 1, 1, ctypes.byref(h)) == _native.VX_ERR_INVALID_ARGUMENT  # nz = 0
     arr[0].nz = 200
     arr[0].nx = arr[0].ny = 2
     assert L.vx_tdma_plan_create(arr, 1, 0, ctypes.byref(h)) == _native.VX_ERR_UNSUPPORTED

@@ -40,11 +40,7 @@ def _oracle(s):
                                  b_edge=f(s.get("b_edge")), d_edge=f(s.get("d_edge")))
 
 
-MIXED_DISABLED = ("the mixed batched kernel took a device memory fault (nz bucket 80) whose cause is not "
-                  "found; mode 1 is disabled in vx_tdma_plan_create")
-
-
-@pytest.mark.parametrize("precision", ["fp64", pytest.param("mixed", marks=pytest.mark.skip(reason=MIXED_DISABLED))])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_plan_matches_single_and_oracle(precision):
     from veros_amd.batched import TdmaBatch
     sys = _systems()
@@ -61,10 +57,11 @@ def test_plan_matches_single_and_oracle(precision):
     plan.close()
 
 
-def test_tune_and_graph_replays():
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_tune_and_graph_replays(precision):
     from veros_amd.batched import TdmaBatch
     sys = _systems(seed=9)
-    plan = TdmaBatch(sys, "fp64")
+    plan = TdmaBatch(sys, precision)
     info = plan.tune()
     for b in info["buckets"]:
         assert b["block"] in (64, 128, 256) and b["blocks_per_cu"] in (0, 1, 2, 4, 8)
@@ -76,7 +73,7 @@ def test_tune_and_graph_replays():
         plan.graph_launch()
         torch.cuda.synchronize()
         for q, s in enumerate(sys):
-            assert torch.equal(s["x"], _single(s, "fp64")), (rep, q)
+            assert torch.equal(s["x"], _single(s, precision)), (rep, q)
     plan.close()
 
 
@@ -89,5 +86,5 @@ def test_plan_rejects_bad_systems():
         TdmaBatch([sys[0], bad])
     with pytest.raises(ValueError):
         TdmaBatch(sys, "bf16")
-    with pytest.raises(NotImplementedError):
-        TdmaBatch(sys, "mixed")
+    with pytest.raises(TypeError):
+        TdmaBatch([sys[0], bad], "mixed")

@@ -124,6 +124,8 @@ struct vx_tdma_plan {
     hipStream_t capture_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // mode 1: the systems, each solved by the single-grid mixed kernel
+    std::vector<vx_tdma_system> mixed;
 };
 
 namespace vx {
@@ -153,15 +155,14 @@ extern "C" int vx_tdma_plan_create(const vx_tdma_system* systems, int32_t nsys, 
     if (!systems || nsys < 1) return fail(VX_ERR_INVALID_ARGUMENT, "%s: need at least one system", who);
     if (mode < 0 || mode > 1)
         return fail(VX_ERR_INVALID_ARGUMENT, "%s: mode must be 0 (fp64) or 1 (mixed, one refinement step)", who);
-    // The mixed batched kernel (tdma_batch_kernel<NZB, 1>) took a device memory
-    // fault in its nz-bucket-80 instantiation although its index arithmetic
-    // stays in bounds in a host build under AddressSanitizer; until the cause is
-    // found it is not built. Mixed solves remain available one grid at a time
-    // (vx_tdma_mixed).
-    if (mode == 1)
-        return fail(VX_ERR_UNSUPPORTED, "%s: mode 1 (mixed) is disabled for batched plans; use vx_tdma_mixed per "
-                    "grid or mode 0", who);
+    // Mode 1 (mixed) does not use a persistent batched kernel: that kernel
+    // (tdma_batch_kernel<NZB, 1>) took a device memory fault in its
+    // nz-bucket-80 instantiation although its index arithmetic stays in bounds
+    // in a host build under AddressSanitizer, and is not built. A mode-1 plan
+    // runs the single-grid mixed kernel (vx_tdma_mixed, one refinement step)
+    // once per system, on the caller's stream or captured into one hipGraph.
     std::vector<std::vector<SysDesc>> per(kNumBuckets);
+    std::vector<vx_tdma_system> mixed;
     for (int q = 0; q < nsys; ++q) {
         const vx_tdma_system& y = systems[q];
         if (y.nz < 1 || y.nx < 0 || y.ny < 0)
@@ -183,9 +184,11 @@ extern "C" int vx_tdma_plan_create(const vx_tdma_system* systems, int32_t nsys, 
                      ? 1
                      : 0;
         per[(y.nz + 15) / 16 - 1].push_back(sd);
+        if (mode == 1) mixed.push_back(y);
     }
     auto* P = new vx_tdma_plan();
     P->mode = mode;
+    P->mixed = std::move(mixed);
     int dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
@@ -217,7 +220,7 @@ extern "C" int vx_tdma_plan_create(const vx_tdma_system* systems, int32_t nsys, 
         }
         P->buckets.push_back(b);
     }
-    if (!table.empty()) {
+    if (!table.empty() && mode == 0) {
         if (hipMalloc(&P->d_sys, table.size() * sizeof(SysDesc)) != hipSuccess) {
             delete P;
             return fail(VX_ERR_LAUNCH, "%s: hipMalloc of the system table failed", who);
@@ -232,17 +235,34 @@ extern "C" int vx_tdma_plan_create(const vx_tdma_system* systems, int32_t nsys, 
     return VX_OK;
 }
 
+namespace vx {
+namespace {
+int run_plan(vx_tdma_plan* plan, hipStream_t s, const char* who) {
+    if (plan->mode == 1) {
+        for (const vx_tdma_system& y : plan->mixed) {
+            const int rc = vx_tdma_mixed(s, y.a, y.b, y.c, y.d, y.b_edge, y.d_edge, y.kbot, y.kbot_sx, y.kbot_sy,
+                                         y.x, y.nx, y.ny, y.nz, y.sx, y.sy, 1);
+            if (rc) return rc;
+        }
+        return check_launch(who);
+    }
+    for (const Bucket& b : plan->buckets) launch_bucket(plan, b, b.bs, b.factor, s);
+    return check_launch(who);
+}
+}  // namespace
+}  // namespace vx
+
 extern "C" int vx_tdma_plan_run(vx_tdma_plan* plan, void* stream) {
     using namespace vx;
     if (!plan) return fail(VX_ERR_INVALID_ARGUMENT, "vx_tdma_plan_run: null plan");
-    for (const Bucket& b : plan->buckets) launch_bucket(plan, b, b.bs, b.factor, (hipStream_t)stream);
-    return check_launch("vx_tdma_plan_run");
+    return run_plan(plan, (hipStream_t)stream, "vx_tdma_plan_run");
 }
 
 extern "C" int vx_tdma_plan_tune(vx_tdma_plan* plan, void* stream) {
     using namespace vx;
     const char* who = "vx_tdma_plan_tune";
     if (!plan) return fail(VX_ERR_INVALID_ARGUMENT, "%s: null plan", who);
+    if (plan->mode == 1) return VX_OK;  // per-system launches of a fixed shape: nothing to choose
     hipStream_t s = (hipStream_t)stream;
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
@@ -308,10 +328,14 @@ extern "C" int vx_tdma_plan_graph_launch(vx_tdma_plan* plan, void* stream) {
             return fail(VX_ERR_LAUNCH, "%s: hipStreamCreate failed", who);
         if (hipStreamBeginCapture(plan->capture_stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
             return fail(VX_ERR_LAUNCH, "%s: hipStreamBeginCapture failed", who);
-        for (const Bucket& b : plan->buckets) launch_bucket(plan, b, b.bs, b.factor, plan->capture_stream);
+        const int rc = run_plan(plan, plan->capture_stream, who);
         hipGraph_t g = nullptr;
         if (hipStreamEndCapture(plan->capture_stream, &g) != hipSuccess || !g)
             return fail(VX_ERR_LAUNCH, "%s: capture failed", who);
+        if (rc) {
+            (void)hipGraphDestroy(g);
+            return rc;
+        }
         if (hipGraphInstantiate(&plan->exec, g, nullptr, nullptr, 0) != hipSuccess) {
             (void)hipGraphDestroy(g);
             return fail(VX_ERR_LAUNCH, "%s: hipGraphInstantiate failed", who);
