@@ -1686,16 +1686,21 @@ int launch_pre(const Grid& g, const Prm& p, const Fields& f, const double* K_iso
     const int64_t ncol = (int64_t)(g.X - 3) * (g.Y - 3);  // one wave per column, 4 per block
     const hipStream_t sd = fork_aux(s);
     if (dry_stream) *dry_stream = sd;
-    // Blocks per CU of the zero fill (0: one wave per column). One block per
-    // CU fills in the background at a modest store rate beside the wet-cell
-    // kernel instead of flooding the memory system ahead of it (measured on
-    // 1440x720x80 with the nontemporal stores: pre alone 4.6 ms with one wave
-    // per column, 4.26 ms with one block per CU, 4.45 with two; the one call
-    // 8.2 ms with one wave per column, 7.5 ms with one block per CU).
-#ifndef VX_DRY_BPC
-#define VX_DRY_BPC 1
+    // Blocks per CU of the zero fill (0: one wave per column). Beside the
+    // whole one call, one block per CU fills in the background at a modest
+    // store rate instead of flooding the memory system ahead of the wet-cell
+    // kernel (measured on 1440x720x80 with the nontemporal stores: 7.5 ms for
+    // the call, 8.2 ms with one wave per column). Pre alone is joined at the
+    // end of pre, so there the fill must not start late: one wave per column
+    // (4.6 ms; one block per CU gave 4.26 ms on one box and a 5.1 ms median on
+    // another, where the fill's blocks were dispatched behind the wet cells').
+#ifndef VX_DRY_BPC_PRE
+#define VX_DRY_BPC_PRE 0
 #endif
-    const unsigned bpc = VX_DRY_BPC;
+#ifndef VX_DRY_BPC_FUSED
+#define VX_DRY_BPC_FUSED 1
+#endif
+    const unsigned bpc = dry_stream ? VX_DRY_BPC_FUSED : VX_DRY_BPC_PRE;
     const unsigned dry_full = blocks_for(ncol, 4), ndry = bpc * (unsigned)device_cu_count();
     const unsigned dry_grid = ndry > 0 && ndry < dry_full ? ndry : dry_full;
     hipLaunchKernelGGL(iso_pre_dry_kernel, dim3(dry_grid), dim3(256), 0, sd, g, Ai_ez, Ai_nz, Ai_bx, Ai_by,
