@@ -28,6 +28,7 @@ struct VmixArgs {
     int64_t X, Y;
     int32_t nz, taup1;
     double dt;
+    bool vec_dv;  // nz even and both tendency arrays 16-byte aligned
 };
 
 constexpr int kVmixBlock = 256;  // 128 columns x 2 tracers
@@ -53,6 +54,7 @@ __global__ __launch_bounds__(kVmixBlock) void vertmix_kernel(VmixArgs p) {
     const double dt = p.dt;
     double* __restrict__ tr = p.tr[t] + p.taup1;         // level taup1 of cell c at tr[3 c]
     const double* __restrict__ kap = p.kappaH + base;
+    const bool vec = p.vec_dv;
 
     // forward elimination with the rows formed on the fly
     // (veros/core/thermodynamics.py:267-276, veros/core/operators.py:60-77);
@@ -106,18 +108,29 @@ __global__ __launch_bounds__(kVmixBlock) void vertmix_kernel(VmixArgs p) {
 #pragma unroll
                 for (int u = 0; u < kVmixChunk; ++u) vo[u] = tr[3 * (base + (q + u < nz ? q + u : nz - 1))];
             }
+            double dvv[kVmixChunk];
 #pragma unroll
             for (int u = kVmixChunk - 1; u >= 0; --u) {
                 const int k = q + u;
-                if (k < nz) {
-                    if (live && k >= k0) {
-                        const double x = dp[k] - cp[k] * xn;
-                        xn = x;
-                        tr[3 * (base + k)] = x;
-                        dv[k] = (x - vo[u]) / dt;
-                    } else {
-                        dv[k] = 0.0;
-                    }
+                dvv[u] = 0.0;
+                if (k < nz && live && k >= k0) {
+                    const double x = dp[k] - cp[k] * xn;
+                    xn = x;
+                    tr[3 * (base + k)] = x;
+                    dvv[u] = (x - vo[u]) / dt;
+                }
+            }
+            // the chunk's tendencies: 16-byte stores when the column start is
+            // 16-byte aligned (nz even)
+            if (vec) {
+#pragma unroll
+                for (int u = 0; u < kVmixChunk; u += 2) {
+                    if (q + u < nz) *reinterpret_cast<double2*>(dv + q + u) = make_double2(dvv[u], dvv[u + 1]);
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < kVmixChunk; ++u) {
+                    if (q + u < nz) dv[q + u] = dvv[u];
                 }
             }
         }
@@ -189,6 +202,7 @@ extern "C" int vx_vertmix_tempsalt_f64(void* stream, const vx_grid* grid, double
     p.nz = grid->nz;
     p.taup1 = taup1;
     p.dt = dt_tracer;
+    p.vec_dv = !(p.nz & 1) && aligned16(dtemp_vmix) && aligned16(dsalt_vmix);
     hipStream_t s = (hipStream_t)stream;
     const int nzb = ((p.nz + 15) / 16) * 16;
     switch (nzb) {
