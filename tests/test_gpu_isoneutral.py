@@ -267,3 +267,24 @@ def test_pre_and_temp_salt_fused_full_grid_equals_separate_calls():
     torch.cuda.synchronize()
     assert (vb.salt[..., vb.taup1] - 35.0).abs().max().item() <= 35.0 * 1e-13
     assert vb.dsalt_iso.abs().max().item() <= 35.0 * 1e-13 / st_b.settings.dt_tracer
+
+
+def test_pre_and_temp_salt_fused_on_a_side_stream():
+    """The one call enqueued on a non-default stream (its dry-cell fill forks
+    onto a helper stream and joins back into that stream) gives the same values
+    as on the default stream, with no host synchronisation in between."""
+    from veros_amd import isoneutral
+    st = _random_state(77, 40, 30, 24, cyclic=True)
+    ds_a, ds_b = _dev_state(st), _dev_state(st)
+    isoneutral.isoneutral_pre_and_diffusion_temp_salt(ds_a)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        isoneutral.isoneutral_pre_and_diffusion_temp_salt(ds_b)
+        # work queued behind the call on the same stream sees its results
+        snap = ds_b.variables.Ai_ez.clone()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    for k in PRE_KEYS + ("temp", "salt", "dtemp_iso", "dsalt_iso", "P_diss_iso"):
+        assert torch.equal(getattr(ds_a.variables, k), getattr(ds_b.variables, k)), k
+    assert torch.equal(snap, ds_a.variables.Ai_ez)
