@@ -288,3 +288,33 @@ def test_pre_and_temp_salt_fused_on_a_side_stream():
     for k in PRE_KEYS + ("temp", "salt", "dtemp_iso", "dsalt_iso", "P_diss_iso"):
         assert torch.equal(getattr(ds_a.variables, k), getattr(ds_b.variables, k)), k
     assert torch.equal(snap, ds_a.variables.Ai_ez)
+
+
+def test_pre_and_temp_salt_fused_graph_replays():
+    """The one call captured into a graph (its kernels, the helper stream's
+    fork/join and the device-built wet-cell plan, no memset) and replayed
+    several times, with eager work and synchronisations between replays,
+    matches the same number of eager calls after every replay."""
+    from veros_amd import isoneutral
+    st = _random_state(91, 36, 28, 20, cyclic=False)
+    ds_e, ds_g = _dev_state(st), _dev_state(st)
+    keys = PRE_KEYS + ("temp", "salt", "dtemp_iso", "dsalt_iso", "P_diss_iso")
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        isoneutral.isoneutral_pre_and_diffusion_temp_salt(ds_g)  # eager first call (allocates the workspace)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    isoneutral.isoneutral_pre_and_diffusion_temp_salt(ds_e)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        isoneutral.isoneutral_pre_and_diffusion_temp_salt(ds_g)
+    for rep in range(3):
+        junk = torch.randn(1 << 16, device="cuda").cumsum(0)  # eager work between replays
+        torch.cuda.synchronize()
+        graph.replay()
+        isoneutral.isoneutral_pre_and_diffusion_temp_salt(ds_e)
+        torch.cuda.synchronize()
+        for k in keys:
+            assert torch.equal(getattr(ds_e.variables, k), getattr(ds_g.variables, k)), (rep, k)
+        del junk
