@@ -253,15 +253,23 @@ def main():
     print(f"[bench] building {nx}x{ny}x{nz} workload", file=sys.stderr, flush=True)
     state, td = build_workload(nx, ny, nz)
     torch.cuda.synchronize()
-    if args.calibrate_copy:
-        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
     vs = state.variables
     stream = torch.cuda.current_stream()
+    # a plain 1 GiB device-to-device copy: the sustained HBM rate this box
+    # reaches (read + write bytes / time), reported next to the roofline, and a
+    # dispatch of known traffic for PMC scaling
+    src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(3 if args.calibrate_copy else 2):
+        dst.copy_(src)
+    c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    c0.record(stream)
+    for _ in range(5):
+        dst.copy_(src)
+    c1.record(stream)
+    torch.cuda.synchronize()
+    copy_gbps = 5 * 2 * src.numel() * 8 / (c0.elapsed_time(c1) * 1e-3) / 1e9
+    del src, dst
 
     def run_tdma(prec):
         utilities.solve_implicit_kbot(td["a"], td["b"], td["c"], td["d"], td["kbot"], precision=prec,
@@ -362,7 +370,8 @@ def main():
             "unit": "GB/s", "frac": round(gbs[dom] / HBM_PEAK_GBS, 4),
             "traffic": PRE_TS_TRAFFIC_BYTES.get((nx, ny, nz)),
             "algorithmic_bytes_per_launch": kern[dom][0], "algorithmic_bytes_per_cell": PRE_TS_BYTES_PER_CELL,
-            "avg_launch_ms": round(kern[dom][1], 4)}
+            "avg_launch_ms": round(kern[dom][1], 4),
+            "copy_GBps": round(copy_gbps, 1), "frac_of_copy": round(gbs[dom] / copy_gbps, 4)}
     out = {
         "metric": "TDMA Mcolumns/s + isoneutral Mcells/s; mixed vs FP64 GPU at <=1e-12 rel err",
         "value": round(N / (ms * 1e-3) / 1e6, 2),
