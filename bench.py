@@ -108,6 +108,15 @@ def build_workload(nx, ny, nz, seed=1002):
     vs["forc_salt_surface"] = 1e-6 * torch.randn((X, Y), generator=g, **f64)
     vs["dtemp_vmix"] = torch.zeros((X, Y, nz), **f64)
     vs["dsalt_vmix"] = torch.zeros((X, Y, nz), **f64)
+    # vertical friction inputs (veros_amd.workloads.add_friction_fields):
+    # u, v ~ 0.1 N(0, 1) m/s, kappaM ~ LogU(1e-5, 1e-2), the T and v-face areas
+    for k in ("u", "v"):
+        vs[k] = 0.1 * torch.randn((X, Y, nz, 3), generator=g, **f64)
+    vs["kappaM"] = torch.exp(lo + (hi - lo) * torch.rand((X, Y, nz), generator=g, **f64))
+    vs["area_t"] = vs["cost"][None, :] * vs["dyt"][None, :] * vs["dxt"][:, None]
+    vs["area_v"] = vs["cosu"][None, :] * vs["dyu"][None, :] * vs["dxt"][:, None]
+    for k in ("du_mix", "dv_mix", "K_diss_v"):
+        vs[k] = torch.zeros((X, Y, nz), **f64)
     from types import SimpleNamespace
     state = SimpleNamespace(variables=SimpleNamespace(**vs), settings=settings)
 
@@ -245,7 +254,7 @@ def main():
         sys.exit(2)
     import torch
 
-    from veros_amd import isoneutral, thermodynamics, utilities
+    from veros_amd import friction, isoneutral, thermodynamics, utilities
 
     torch.cuda.set_device(0)
     nx, ny, nz = args.nx, args.ny, args.nz
@@ -287,6 +296,9 @@ def main():
     def run_vertmix():
         thermodynamics.vertmix_tempsalt(state)
 
+    def run_friction():
+        friction.implicit_vert_friction(state)
+
     def run_diff_two_calls():
         isoneutral.isoneutral_diffusion(state, vs.temp, True)
         isoneutral.isoneutral_diffusion(state, vs.salt, False)
@@ -306,7 +318,7 @@ def main():
     for name, fn in (("tdma_fp64", lambda: run_tdma("fp64")), ("tdma_mixed", lambda: run_tdma("mixed")),
                      ("iso_pre", run_pre), ("iso_diffusion_TS", run_diff),
                      ("iso_diffusion_T_then_S", run_diff_two_calls), ("iso_pre_and_diffusion_TS", run_pre_diff),
-                     ("vertmix_TS", run_vertmix)):
+                     ("vertmix_TS", run_vertmix), ("friction_UV", run_friction)):
         for _ in range(args.warmup):
             fn()
         parts[name] = timed(fn, max(5, args.steps))
@@ -342,6 +354,10 @@ def main():
     # (2 * 8 over X*Y*nz), kbot and the two forcings per column (4 + 16)
     _, W_v = tdma_bytes(vs.kbot[2:-2, 2:-2], nz)
     B_vmix = 40 * W_v + 16 * (nx + 4) * (ny + 4) * nz + 20 * (nx + 4) * (ny + 4)
+    # implicit_vert_friction: per cell kappaM read (8), u and v at tau read and
+    # at taup1 written or kept (4 * 8), du_mix and dv_mix written (16),
+    # K_diss_v read and written (16), over the X*Y*nz array
+    B_fric = 72 * (nx + 4) * (ny + 4) * nz
     rates = {
         "tdma_mcolumns_per_s": {p: C / (parts[f"tdma_{p}"][0] * 1e-3) / 1e6 for p in ("fp64", "mixed")},
         "iso_mcells_per_s": N / (parts["iso_pre_and_diffusion_TS"][0] * 1e-3) / 1e6,
@@ -355,6 +371,7 @@ def main():
         "iso_diffusion_T_then_S": (B_diff_two, parts["iso_diffusion_T_then_S"][0]),
         "iso_pre_and_diffusion_TS": (B_pre_diff, parts["iso_pre_and_diffusion_TS"][0]),
         "vertmix_TS": (B_vmix, parts["vertmix_TS"][0]),
+        "friction_UV": (B_fric, parts["friction_UV"][0]),
     }
     gbs = {k: b / (t * 1e-3) / 1e9 for k, (b, t) in kern.items()}
     # roofline: pre + the temperature + salinity diffusion in one C-ABI call,

@@ -266,6 +266,24 @@ int vx_vertmix_tempsalt_f64(void* stream, const vx_grid* grid, double dt_tracer,
                             double* dtemp_vmix, double* dsalt_vmix, int32_t enable_cyclic_x);
 
 /*
+ * implicit_vert_friction (veros/core/friction.py:90-205): implicit vertical
+ * friction of u and v (X, Y, nz, 3) with viscosity kappaM (X, Y, nz) averaged
+ * onto the faces. Over the face columns [1, X-2) x [1, Y-2): the solve of
+ * level tau goes in place into u/v[..., taup1] on the levels k >= max(kbot,
+ * kbot_next) - 1 (a face with one dry side keeps tau's value there), and
+ * du_mix / dv_mix are set to (taup1 - tau) / dt_mom on every level; outside
+ * that range they keep their values. The dissipation of the solve,
+ * interpolated onto the T grid with dxu / dxt (u) and area_v / area_t (v)
+ * (veros/core/numerics.py:313-336), is added to K_diss_v. grid->kbot, dzt,
+ * dzw, dxt and dxu are read; the other grid arrays may be NULL. tau and taup1
+ * must differ.
+ */
+int vx_implicit_vert_friction_f64(void* stream, const vx_grid* grid, double dt_mom,
+                                  double* u, double* v, int32_t tau, int32_t taup1,
+                                  const double* kappaM, const double* area_t, const double* area_v,
+                                  double* du_mix, double* dv_mix, double* K_diss_v);
+
+/*
  * isoneutral_skew_diffusion for one tracer (veros/core/isoneutral/diffusion.py:172-207,
  * 210-283, 298-307, iso=False): the same flux stencil with K_iso = 0 and
  * K_skew = K_gm, the explicit update of tracer[..., taup1] and dtr_iso, no

@@ -134,6 +134,28 @@ def test_vertmix_argument_errors_without_gpu():
     assert "vx_vertmix_tempsalt_f64" in L.vx_last_error().decode()
 
 
+def test_friction_argument_errors_without_gpu():
+    L = _native.lib()
+    g = _native.VxGrid()
+    g.nx, g.ny, g.nz = 4, 4, 8
+
+    def call(grid=g, dt=3600.0, u=1, v=2, tau=0, taup1=1, kap=3, du=4, dv=5, kd=6):
+        return L.vx_implicit_vert_friction_f64(None, grid, dt, u, v, tau, taup1, kap, 7, 8, du, dv, kd)
+
+    assert call() == _native.VX_ERR_INVALID_ARGUMENT                # null kbot/dzt/dzw/dxt/dxu
+    g.kbot, g.dzt, g.dzw, g.dxt, g.dxu = 16, 16, 16, 16, 16
+    assert call(grid=None) == _native.VX_ERR_INVALID_ARGUMENT
+    assert call(tau=1) == _native.VX_ERR_INVALID_ARGUMENT            # tau == taup1
+    assert call(taup1=3) == _native.VX_ERR_INVALID_ARGUMENT
+    assert call(v=1) == _native.VX_ERR_INVALID_ARGUMENT              # u and v alias
+    assert call(kd=4) == _native.VX_ERR_INVALID_ARGUMENT             # K_diss_v aliases du_mix
+    assert call(kap=None) == _native.VX_ERR_INVALID_ARGUMENT
+    assert call(dt=-1.0) == _native.VX_ERR_INVALID_ARGUMENT
+    g.nz = _native.max_nz() + 1
+    assert call() == _native.VX_ERR_UNSUPPORTED
+    assert "vx_implicit_vert_friction_f64" in L.vx_last_error().decode()
+
+
 def test_batch_plan_argument_errors_without_gpu():
     import ctypes
     L = _native.lib()

@@ -83,6 +83,8 @@ _SIGNATURES = {
                                   + [c_void_p, c_void_p, c_void_p, c_void_p]),
     "vx_vertmix_tempsalt_f64": (ctypes.c_int, [c_void_p, ctypes.POINTER(VxGrid), c_f64, c_void_p, c_void_p, c_i32,
                                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_i32]),
+    "vx_implicit_vert_friction_f64": (ctypes.c_int, [c_void_p, ctypes.POINTER(VxGrid), c_f64, c_void_p, c_void_p,
+                                                     c_i32, c_i32] + [c_void_p] * 6),
     "vx_iso_adjoint_workspace_bytes": (c_i64, [ctypes.POINTER(VxGrid)]),
     "vx_iso_diffusion_adjoint_f64": (ctypes.c_int, [c_void_p, ctypes.POINTER(VxGrid),
                                                     ctypes.POINTER(VxIsoParams), c_void_p, c_i32]

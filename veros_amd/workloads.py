@@ -38,6 +38,7 @@ class MixSettings:
     ny: int
     nz: int
     dt_tracer: float = 3600.0
+    dt_mom: float = 3600.0
     K_iso_steep: float = 500.0
     iso_slopec: float = 1e-2
     iso_dslope: float = 5e-3
@@ -349,6 +350,33 @@ def add_vertmix_fields(rng, state, *, kappa_lo=1e-5, kappa_hi=1e-2, forc_scale=1
     vs.dtemp_vmix = rng.standard_normal((X, Y, nz)) if random_aux else np.zeros((X, Y, nz))
     vs.dsalt_vmix = rng.standard_normal((X, Y, nz)) if random_aux else np.zeros((X, Y, nz))
     return state
+
+
+def add_friction_fields(rng, state, *, kappa_lo=1e-5, kappa_hi=1e-2, u_scale=0.1, random_aux=False):
+    """Add the fields implicit_vert_friction reads and writes to an isoneutral
+    state: velocities ``u``, ``v ~ u_scale * N(0, 1)`` m/s at every time level,
+    ``kappaM ~ LogU(kappa_lo, kappa_hi)`` m^2/s per cell, the cell and
+    v-face areas ``area_t = dxt cost dyt``, ``area_v = dxt cosu dyu``
+    (veros/core/numerics.py:136-138), and the in/out fields ``du_mix``,
+    ``dv_mix``, ``K_diss_v`` (noise with ``random_aux``, so tests see which
+    cells are written)."""
+    vs = state.variables
+    X, Y, nz = vs.K_iso.shape
+    vs.u = u_scale * rng.standard_normal((X, Y, nz, 3))
+    vs.v = u_scale * rng.standard_normal((X, Y, nz, 3))
+    vs.kappaM = np.exp(rng.uniform(np.log(kappa_lo), np.log(kappa_hi), size=(X, Y, nz)))
+    vs.area_t = vs.cost[None, :] * vs.dyt[None, :] * vs.dxt[:, None]
+    vs.area_v = vs.cosu[None, :] * vs.dyu[None, :] * vs.dxt[:, None]
+    vs.du_mix = rng.standard_normal((X, Y, nz)) if random_aux else np.zeros((X, Y, nz))
+    vs.dv_mix = rng.standard_normal((X, Y, nz)) if random_aux else np.zeros((X, Y, nz))
+    vs.K_diss_v = np.abs(rng.standard_normal((X, Y, nz))) * 1e-9 if random_aux else np.zeros((X, Y, nz))
+    return state
+
+
+def friction_config2(seed=1002, nx=1440, ny=720, nz=80):
+    """The 1440x720x80 grid of the mixing step with the vertical-friction fields."""
+    st = iso_config2(seed, nx, ny, nz)
+    return add_friction_fields(np.random.default_rng(seed + 300), st)
 
 
 def vertmix_config2(seed=1002, nx=1440, ny=720, nz=80):
