@@ -108,3 +108,33 @@ def test_friction_full_grid_properties():
     torch.cuda.synchronize()
     assert (vs.u[1:-2, 1:-2, :, t1][wet] - 0.2).abs().max().item() <= 1e-14
     assert torch.equal(vs.K_diss_v, K0)
+
+
+def test_friction_graph_replays():
+    """The call captured into a graph (two kernels, no allocation, no memset)
+    and replayed several times, with eager work and synchronisations between
+    replays, matches the same number of eager calls after every replay
+    (K_diss_v accumulates, so every replay is checked against its own eager
+    counterpart)."""
+    from veros_amd import friction
+    st = _state(93, 30, 26, 40, cyclic=True, land=0.3)
+    ds_e, ds_g = _dev(st), _dev(st)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        friction.implicit_vert_friction(ds_g)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    friction.implicit_vert_friction(ds_e)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        friction.implicit_vert_friction(ds_g)
+    for rep in range(3):
+        junk = torch.randn(1 << 16, device="cuda").cumsum(0)  # eager work between replays
+        torch.cuda.synchronize()
+        graph.replay()
+        friction.implicit_vert_friction(ds_e)
+        torch.cuda.synchronize()
+        for k in KEYS:
+            assert torch.equal(getattr(ds_e.variables, k), getattr(ds_g.variables, k)), (rep, k)
+        del junk
