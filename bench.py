@@ -30,13 +30,10 @@ HBM_PEAK_GBS = 8000.0
 # profiles/ (read bytes = 2.0 x FETCH_SIZE, write bytes = 1.0 x WRITE_SIZE,
 # calibrated on a 1 GiB device copy)
 PRE_TS_TRAFFIC_BYTES = {(1440, 720, 80): 27_204_216_832}
-# algorithmic bytes per interior cell: the fused temperature + salinity step
-# with the energy branch reads K_iso, Ai_* and K_11/22/33 once (8 + 128 + 24),
-# P_diss read and written (16), and per tracer tr[tau], tr[taup1] read and
-# written, dtr_iso read and written, int_drhodX (8 + 16 + 16 + 8)
-TS_BYTES_PER_CELL = 8 + 128 + 24 + 16 + 2 * (8 + 16 + 16 + 8)
-# pre + the temperature + salinity step in one call: K_iso read (8), Ai_* and
-# K_11/22/33 written instead of read (128 + 24), the rest as above
+# the minimum bytes per interior cell of pre + the temperature + salinity step
+# in one call: K_iso read (8), Ai_* and K_11/22/33 written once and not read
+# back (128 + 24), P_diss read and written (16), and per tracer tr[tau] (8),
+# tr[taup1] read and written (16), dtr_iso read and written (16), int_drhodX (8)
 PRE_TS_BYTES_PER_CELL = 8 + 128 + 24 + 16 + 2 * (8 + 16 + 16 + 8)
 
 
@@ -55,6 +52,11 @@ def parse():
     ap.add_argument("--batched", action="store_true",
                     help="instead of the step: 64 independent uneven grids, sequential launches vs one batched "
                          "plan (untuned, autotuned, hipGraph), fp64 and mixed")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-grid batched leg")
+    ap.add_argument("--only", default=None,
+                    help="comma-separated parts to time (tdma_fp64, tdma_mixed, iso_pre, iso_diffusion_TS, "
+                         "iso_diffusion_T_then_S, iso_pre_and_diffusion_TS, vertmix_TS, friction_UV) and "
+                         "nothing else: one JSON line of their timings (for profiler runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy (a known-traffic dispatch for PMC scaling)")
     return ap.parse_args()
@@ -144,50 +146,116 @@ def tdma_bytes(kbot, nz):
     return 32 * W + 8 * C * nz + 4 * C, W
 
 
-def cpu_baseline(nx, ny, nz, budget_s):
-    """The oracle (NumPy float64; dgtsv for the TDMA) on a bounded sample of
-    the same workload: a 1440 x ny_s x 80 TDMA block and a 360 x ny_s x 80
-    isoneutral block, sized from a short calibration run."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+# windows of the 1440x720x80 state checked against the oracle: the west edge
+# with its cyclic halo, the interior, the north-east corner (halo-inclusive
+# index ranges; the oracle's cells [2:-2, 2:-2] of a window equal the whole
+# grid's)
+CHECK_WINDOWS = ((0, 100, 0, 84), (700, 800, 300, 384), (1344, 1444, 640, 724))
+ISO_NAMES = ("Ai_ez", "Ai_nz", "Ai_bx", "Ai_by", "K_11", "K_22", "K_33", "temp", "salt", "dtemp_iso",
+             "dsalt_iso", "P_diss_iso", "K_iso", "int_drhodT", "int_drhodS", "dxt", "dxu", "dyt", "dyu", "cost",
+             "cosu", "dzt", "dzw", "zt")
+
+
+def cpu_baseline(snap, budget_s):
+    """The oracle (NumPy float64; LAPACK dgtsv for the TDMA) timed on the host
+    cores on windows of the same 1440x720x80 state and on whole rows of the
+    same TDMA columns, and the per-output error of the GPU results against
+    it (max|x - x_ref| / max|x_ref| over the window cells [2:-2, 2:-2] / the
+    sampled columns)."""
     import numpy as np
 
     from oracle import isoneutral as OI
     from oracle import tdma as OT
-    from veros_amd import workloads as W
 
-    def time_iso(nys):
-        st = W.iso_config2(nx=360, ny=nys, nz=nz)
-        st.settings = W.config2_settings(360, nys, nz)
-        t0 = time.perf_counter()
-        pre = OI.isoneutral_diffusion_pre(st)
+    def oracle_step(w):
+        P0 = np.array(w.variables.P_diss_iso, copy=True)
+        pre = OI.isoneutral_diffusion_pre(w)
         for k, v in pre._asdict().items():
-            setattr(st.variables, k, v)
-        OI.isoneutral_diffusion(st, "temp", True)
-        OI.isoneutral_diffusion(st, "salt", False)
-        return time.perf_counter() - t0, 360 * nys * nz
+            setattr(w.variables, k, v)
+        OI.isoneutral_diffusion(w, "temp", True)
+        Pt = np.array(w.variables.P_diss_iso, copy=True)
+        OI.isoneutral_diffusion(w, "salt", False)
+        return Pt - P0, w.variables.P_diss_iso - Pt
 
-    def time_tdma(nys):
-        p = W.tdma_config2(nx=nx, ny=nys, nz=nz)
+    def rel(x, r):
+        s = np.abs(r).max()
+        return float(np.abs(x - r).max() / s) if s > 0 else float(np.abs(x - r).max())
+
+    errs = {}
+    t_iso = 0.0
+    n_iso = 0
+    sl = np.s_[2:-2, 2:-2]
+    for (before, after) in snap["iso_windows"]:
+        nxw, nyw = before.settings.nx, before.settings.ny
+        nz = before.settings.nz
         t0 = time.perf_counter()
-        OT.solve_implicit_kbot(p.a, p.b, p.c, p.d, p.kbot)
-        return time.perf_counter() - t0, nx * nys
-
-    t_iso, n_iso = time_iso(4)
-    t_td, n_td = time_tdma(8)
-    scale_iso = max(1, int((budget_s * 0.7) / max(t_iso, 1e-3) * 4))
-    scale_td = max(1, int((budget_s * 0.3) / max(t_td, 1e-3) * 8))
-    t_iso, n_iso = time_iso(min(scale_iso, 720))
-    t_td, n_td = time_tdma(min(scale_td, 720))
+        dPt, dPs = oracle_step(before)
+        t_iso += time.perf_counter() - t0
+        n_iso += nxw * nyw * nz
+        ref = before.variables
+        tau1 = ref.taup1
+        for k in ("Ai_ez", "Ai_nz", "Ai_bx", "Ai_by", "K_11", "K_22", "K_33", "temp", "salt", "dtemp_iso",
+                  "dsalt_iso"):
+            g, r = getattr(after, k), getattr(ref, k)
+            if k in ("temp", "salt"):
+                g, r = g[..., tau1], r[..., tau1]
+            errs[k] = max(errs.get(k, 0.0), rel(g[sl], r[sl]))
+        Pr = ref.P_diss_iso[sl]
+        errs["P_diss_iso"] = max(errs.get("P_diss_iso", 0.0), rel(after.P_diss_iso[sl], Pr))
+        scale = max(np.abs(Pr).max(), np.abs(dPt[sl]).max(), np.abs(dPs[sl]).max())
+        errs["P_diss_iso_vs_largest_tracer_term"] = max(errs.get("P_diss_iso_vs_largest_tracer_term", 0.0),
+                                                        float(np.abs(after.P_diss_iso[sl] - Pr).max() / scale))
     iso_rate = n_iso / t_iso / 1e6
+    # TDMA: whole rows of columns, as many as about a third of the budget allows
+    rows = snap["tdma_rows"]
+    t0 = time.perf_counter()
+    ref = OT.solve_implicit_kbot(rows["a"][:1], rows["b"][:1], rows["c"][:1], rows["d"][:1], rows["kbot"][:1])
+    t1 = time.perf_counter() - t0
+    nr = int(max(1, min(rows["a"].shape[0], (0.3 * budget_s) / max(t1, 1e-3))))
+    t0 = time.perf_counter()
+    ref = OT.solve_implicit_kbot(rows["a"][:nr], rows["b"][:nr], rows["c"][:nr], rows["d"][:nr],
+                                 rows["kbot"][:nr])
+    t_td = time.perf_counter() - t0
+    n_td = nr * rows["a"].shape[1]
     td_rate = n_td / t_td / 1e6
-    # step-equivalent: per interior cell, the TDMA costs (1/nz) columns
+    for prec, x in rows["x"].items():
+        errs[f"tdma_x_{prec}"] = rel(x[:nr], ref)
+    # more isoneutral cells for the timing, up to about two thirds of the budget
+    extra = snap.get("iso_timing_window")
+    if extra is not None and t_iso < 0.6 * budget_s:
+        t0 = time.perf_counter()
+        oracle_step(extra)
+        t_iso += time.perf_counter() - t0
+        n_iso += extra.settings.nx * extra.settings.ny * extra.settings.nz
+        iso_rate = n_iso / t_iso / 1e6
+    nx, ny, nz = snap["shape"]
     N = nx * ny * nz
     t_step = N / (iso_rate * 1e6) + (nx * ny) / (td_rate * 1e6)
-    return dict(value=N / t_step / 1e6, unit="Mcells/s", cores=1, kind="port",
-                sample=f"oracle (NumPy float64, LAPACK dgtsv) on a 360x{min(scale_iso, 720)}x{nz} isoneutral "
-                       f"block ({t_iso:.1f} s) and a {nx}x{min(scale_td, 720)}x{nz} TDMA block ({t_td:.1f} s)",
-                tdma_mcolumns_per_s=td_rate, iso_mcells_per_s=iso_rate,
-                host=dict(cpu_count=os.cpu_count(), affinity=len(os.sched_getaffinity(0)),
-                          cpu=platform.processor() or platform.machine(), numpy=np.__version__))
+    try:
+        import scipy
+        scipy_v = scipy.__version__
+    except ImportError:
+        scipy_v = None
+    base = dict(value=round(N / t_step / 1e6, 4), unit="Mcells/s", cores=1, kind="port",
+                sample=f"oracle (NumPy float64, LAPACK dgtsv) on {len(snap['iso_windows'])} windows of the "
+                       f"{nx}x{ny}x{nz} state plus one timing window ({n_iso} isoneutral cells, {t_iso:.1f} s) "
+                       f"and {nr} whole rows of the TDMA columns ({n_td} columns, {t_td:.1f} s)",
+                tdma_mcolumns_per_s=round(td_rate, 4), iso_mcells_per_s=round(iso_rate, 4),
+                host=dict(cpu_count=os.cpu_count(), affinity=len(os.sched_getaffinity(0)), cpu=cpu_model(),
+                          numpy=np.__version__, scipy=scipy_v),
+                jax_cpu="n/a (JAX is not installed on the host)")
+    return base, errs
 
 
 def run_batched(args):
@@ -243,22 +311,91 @@ def run_batched(args):
                       "columns": cols, "cells": cells, "algorithmic_bytes": nbytes, "results": res}))
 
 
+def timed_events(fn, reps, stream):
+    """(median, min) ms of ``reps`` calls, each bracketed by HIP events on
+    ``stream`` (the stream the entry points enqueue on)."""
+    import torch
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for s0, e0 in evs:
+        s0.record(stream)
+        fn()
+        e0.record(stream)
+    torch.cuda.synchronize()
+    ts = sorted(a.elapsed_time(b) for a, b in evs)
+    return ts[len(ts) // 2], ts[0]
+
+
+def batched_leg(args, stream):
+    """The 64-grid batched workload: 64 sequential single-grid launches against
+    one plan (one persistent launch per depth bucket), untuned, tuned and
+    replayed as a hipGraph, for both precision forms."""
+    import torch
+
+    from veros_amd import utilities
+    from veros_amd.batched import TdmaBatch, random_systems
+
+    systems = random_systems(64)
+    torch.cuda.synchronize()
+    cols = sum(s["a"].shape[0] * s["a"].shape[1] for s in systems)
+    cells = sum(s["a"].numel() for s in systems)
+    nbytes = sum(tdma_bytes(s["kbot"], s["a"].shape[2])[0] for s in systems)
+    reps = max(5, min(args.steps, 10))
+    res = {}
+    for prec in ("fp64", "mixed"):
+        def sequential():
+            for s0 in systems:
+                utilities.solve_implicit_kbot(s0["a"], s0["b"], s0["c"], s0["d"], s0["kbot"], precision=prec,
+                                              refine_steps=1, out=s0["x"])
+        plan = TdmaBatch(systems, prec)
+        for fn in (sequential, plan.run):
+            for _ in range(args.warmup):
+                fn()
+        r = {"sequential_64_launches_ms": timed_events(sequential, reps, stream)[0],
+             "plan_untuned_ms": timed_events(plan.run, reps, stream)[0]}
+        info = plan.tune()
+        r["plan_tuned_ms"] = timed_events(plan.run, reps, stream)[0]
+        plan.graph_launch()
+        r["plan_graph_ms"] = timed_events(plan.graph_launch, reps, stream)[0]
+        r["tuned_choices"] = info["buckets"]
+        best = min(r["plan_tuned_ms"], r["plan_graph_ms"])
+        r["speedup_vs_sequential"] = r["sequential_64_launches_ms"] / best
+        r["mcolumns_per_s"] = cols / (best * 1e-3) / 1e6
+        r["algorithmic_GBps"] = nbytes / (best * 1e-3) / 1e9
+        res[prec] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+        plan.close()
+        print(f"[bench] batched {prec}: {res[prec]['plan_graph_ms']} ms graph, "
+              f"{res[prec]['sequential_64_launches_ms']} ms sequential", file=sys.stderr, flush=True)
+    del systems
+    torch.cuda.empty_cache()
+    return {"workload": "64 independent grids, nx,ny ~ U{48..720}, nz ~ U{15..100}, kbot U{1..nz}, 30% land",
+            "columns": cols, "cells": cells, "algorithmic_bytes": nbytes, "results": res}
+
+
+def run_batched(args):
+    import torch
+    torch.cuda.set_device(0)
+    print(json.dumps(batched_leg(args, torch.cuda.current_stream())))
+
+
 def main():
     args = parse()
-    if args.batched:
-        return run_batched(args)
     if args.gpus != 1:
         print(json.dumps({"error": "one GPU only: --gpus above 1 is not supported (the TDMA columns would "
                                    "shard with no communication, the stencils by 2-D blocks with a 2-cell "
                                    "halo exchange of T and S per step; not built)"}))
         sys.exit(2)
+    if args.batched:
+        return run_batched(args)
     import torch
 
     from veros_amd import friction, isoneutral, thermodynamics, utilities
+    from veros_amd import workloads as W
+    from veros_amd.state import to_device_state
 
     torch.cuda.set_device(0)
     nx, ny, nz = args.nx, args.ny, args.nz
     N = nx * ny * nz
+    X, Y = nx + 4, ny + 4
     print(f"[bench] building {nx}x{ny}x{nz} workload", file=sys.stderr, flush=True)
     state, td = build_workload(nx, ny, nz)
     torch.cuda.synchronize()
@@ -271,18 +408,45 @@ def main():
     dst = torch.empty_like(src)
     for _ in range(3 if args.calibrate_copy else 2):
         dst.copy_(src)
-    c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    c0.record(stream)
-    for _ in range(5):
-        dst.copy_(src)
-    c1.record(stream)
-    torch.cuda.synchronize()
-    copy_gbps = 5 * 2 * src.numel() * 8 / (c0.elapsed_time(c1) * 1e-3) / 1e9
+    copy_ms = timed_events(lambda: dst.copy_(src), 5, stream)[0]
+    copy_gbps = 2 * src.numel() * 8 / (copy_ms * 1e-3) / 1e9
     del src, dst
 
-    def run_tdma(prec):
+    def run_tdma(prec, out=None):
         utilities.solve_implicit_kbot(td["a"], td["b"], td["c"], td["d"], td["kbot"], precision=prec,
-                                      refine_steps=args.refine, out=td["x"])
+                                      refine_steps=args.refine, out=td["x"] if out is None else out)
+
+    # ---- results checked against the oracle (cpu_baseline leg): inputs of
+    # windows of the state before the one call, its outputs after it, and
+    # whole rows of the TDMA columns solved by both forms
+    snap = None
+    if not args.no_cpu_baseline:
+        ww = [w for w in CHECK_WINDOWS if w[1] <= X and w[3] <= Y]
+        fm = W.whole_grid_masks(state)
+        before = [W.window_state(state, *w, names=ISO_NAMES, full_masks=fm) for w in ww]
+        # the timing window: 364 x 196 columns next to the first check window
+        tw = (100, 464, 0, min(Y, 200))
+        timing = W.window_state(state, *tw, names=ISO_NAMES, full_masks=fm) if tw[1] <= X else None
+        del fm
+        isoneutral.isoneutral_pre_and_diffusion_temp_salt(state)
+        torch.cuda.synchronize()
+        after = [W.window_state(state, *w, masks=False, names=ISO_NAMES).variables for w in ww]
+        rows = torch.arange(0, min(nx, 64), device="cuda")
+        xs = {}
+        for prec in ("fp64", "mixed"):
+            x = torch.empty_like(td["d"])
+            run_tdma(prec, out=x)
+            xs[prec] = x.index_select(0, rows).cpu().numpy()
+            del x
+        snap = dict(iso_windows=list(zip(before, after)), iso_timing_window=timing, shape=(nx, ny, nz),
+                    tdma_rows=dict({k: td[k].index_select(0, rows).cpu().numpy()
+                                    for k in ("a", "b", "c", "d", "kbot")}, x=xs))
+        # back to the workload's initial state for the timings
+        del state, td, vs
+        torch.cuda.empty_cache()
+        state, td = build_workload(nx, ny, nz)
+        vs = state.variables
+        torch.cuda.synchronize()
 
     def run_pre():
         isoneutral.isoneutral_diffusion_pre(state)
@@ -303,34 +467,35 @@ def main():
         isoneutral.isoneutral_diffusion(state, vs.temp, True)
         isoneutral.isoneutral_diffusion(state, vs.salt, False)
 
-    def timed(fn, reps):
-        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
-        for s, e in evs:
-            s.record(stream)
-            fn()
-            e.record(stream)
-        torch.cuda.synchronize()
-        ts = sorted(s.elapsed_time(e) for s, e in evs)
-        return ts[len(ts) // 2], ts[0]
-
     # per-part timings (median, min) in ms
     parts = {}
+    only = None if args.only is None else set(args.only.split(","))
     for name, fn in (("tdma_fp64", lambda: run_tdma("fp64")), ("tdma_mixed", lambda: run_tdma("mixed")),
                      ("iso_pre", run_pre), ("iso_diffusion_TS", run_diff),
                      ("iso_diffusion_T_then_S", run_diff_two_calls), ("iso_pre_and_diffusion_TS", run_pre_diff),
                      ("vertmix_TS", run_vertmix), ("friction_UV", run_friction)):
+        if only is not None and name not in only:
+            continue
         for _ in range(args.warmup):
             fn()
-        parts[name] = timed(fn, max(5, args.steps))
+        parts[name] = timed_events(fn, max(5, args.steps), stream)
         print(f"[bench] {name}: median {parts[name][0]:.3f} ms, min {parts[name][1]:.3f} ms",
               file=sys.stderr, flush=True)
 
+    if only is not None:
+        print(json.dumps({"parts_ms_median": {k: round(v[0], 4) for k, v in parts.items()},
+                          "config": {"nx": nx, "ny": ny, "nz": nz}}))
+        return
+
+    def step_of(prec):
+        def step():
+            run_tdma(prec)
+            run_pre_diff()
+        return step
+
+    # the timed region: the configured precision form's step, K steps
     prec = args.precision
-
-    def step():
-        run_tdma(prec)
-        run_pre_diff()
-
+    step = step_of(prec)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -342,53 +507,84 @@ def main():
     t1.record(stream)
     torch.cuda.synchronize()
     ms = t0.elapsed_time(t1) / args.steps
+    step_ms = {prec: ms}
+    other = "fp64" if prec == "mixed" else "mixed"
+    step_ms[other] = timed_events(step_of(other), max(5, args.steps), stream)[0]
+    print(f"[bench] step: {step_ms}", file=sys.stderr, flush=True)
 
     B_td, W_td = tdma_bytes(td["kbot"], nz)
     C = nx * ny
-    B_pre = 176 * N + 4 * (nx + 4) * (ny + 4)
-    B_diff_two = 2 * (224 * N + 4 * (nx + 4) * (ny + 4))
-    B_diff = TS_BYTES_PER_CELL * N + 4 * (nx + 4) * (ny + 4)
-    B_pre_diff = PRE_TS_BYTES_PER_CELL * N + 4 * (nx + 4) * (ny + 4)
+    cols_halo = X * Y
+    # SURVEY-style algorithmic bytes per interior cell: pre 176 (T, S, K_iso
+    # read; Ai_* and K_11/22/33 written), each tracer's isoneutral_diffusion
+    # 200 (tr[tau], tr[taup1], K_iso, Ai_*, K_*, dtr read; tr[taup1], dtr
+    # written) + 24 with the energy branch; 4 B of kbot per column and kernel
+    B8_pre = 176 * N + 4 * cols_halo
+    B8_tracer = (200 + 24) * N + 4 * cols_halo
+    B8_step = B8_pre + 2 * B8_tracer
+    # the one call's own minimum (the tensor written once and never re-read)
+    B_pre_diff = PRE_TS_BYTES_PER_CELL * N + 4 * cols_halo
     # vertmix_tempsalt: kappaH read and temp/salt[taup1] read and written on the
     # wet cells (8 + 2 * 16), the two tendencies written on every cell
     # (2 * 8 over X*Y*nz), kbot and the two forcings per column (4 + 16)
     _, W_v = tdma_bytes(vs.kbot[2:-2, 2:-2], nz)
-    B_vmix = 40 * W_v + 16 * (nx + 4) * (ny + 4) * nz + 20 * (nx + 4) * (ny + 4)
+    B_vmix = 40 * W_v + 16 * cols_halo * nz + 20 * cols_halo
     # implicit_vert_friction: per cell kappaM read (8), u and v at tau read and
     # at taup1 written or kept (4 * 8), du_mix and dv_mix written (16),
     # K_diss_v read and written (16), over the X*Y*nz array
-    B_fric = 72 * (nx + 4) * (ny + 4) * nz
-    rates = {
-        "tdma_mcolumns_per_s": {p: C / (parts[f"tdma_{p}"][0] * 1e-3) / 1e6 for p in ("fp64", "mixed")},
-        "iso_mcells_per_s": N / (parts["iso_pre_and_diffusion_TS"][0] * 1e-3) / 1e6,
-        "iso_mcells_per_s_pre_then_TS": N / ((parts["iso_pre"][0] + parts["iso_diffusion_TS"][0]) * 1e-3) / 1e6,
-    }
+    B_fric = 72 * cols_halo * nz
     kern = {
         "tdma_fp64": (B_td, parts["tdma_fp64"][0]),
         "tdma_mixed": (B_td, parts["tdma_mixed"][0]),
-        "iso_pre": (B_pre, parts["iso_pre"][0]),
-        "iso_diffusion_TS": (B_diff, parts["iso_diffusion_TS"][0]),
-        "iso_diffusion_T_then_S": (B_diff_two, parts["iso_diffusion_T_then_S"][0]),
-        "iso_pre_and_diffusion_TS": (B_pre_diff, parts["iso_pre_and_diffusion_TS"][0]),
+        "iso_pre": (B8_pre, parts["iso_pre"][0]),
+        "iso_diffusion_TS": (2 * B8_tracer, parts["iso_diffusion_TS"][0]),
+        "iso_diffusion_T_then_S": (2 * B8_tracer, parts["iso_diffusion_T_then_S"][0]),
+        "iso_pre_and_diffusion_TS": (B8_step, parts["iso_pre_and_diffusion_TS"][0]),
         "vertmix_TS": (B_vmix, parts["vertmix_TS"][0]),
         "friction_UV": (B_fric, parts["friction_UV"][0]),
     }
     gbs = {k: b / (t * 1e-3) / 1e9 for k, (b, t) in kern.items()}
     # roofline: pre + the temperature + salinity diffusion in one C-ABI call,
-    # the largest launch of the step (the wet-cell plan's three scan kernels,
-    # the dry-column zero fill on a helper stream beside the wet-cell slopes +
-    # face fluxes, the top-level K_33 zero, then the divergence + K33 solve +
-    # updates over the wet cells)
+    # the largest launch of the step
     dom = "iso_pre_and_diffusion_TS"
+    t_dom = parts[dom][0]
+    traffic = PRE_TS_TRAFFIC_BYTES.get((nx, ny, nz))
     roof = {"bound": "hbm",
             "kernel": "vx_iso_pre_diffusion_ts_f64 = wet_scan_{count,chunks,columns}_kernel + "
                       "(iso_pre_dry_kernel || iso_pre_kernel<true> + k33_top_zero_kernel + iso_step_wet_kernel<2>)",
-            "achieved": round(gbs[dom], 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(gbs[dom] / HBM_PEAK_GBS, 4),
-            "traffic": PRE_TS_TRAFFIC_BYTES.get((nx, ny, nz)),
-            "algorithmic_bytes_per_launch": kern[dom][0], "algorithmic_bytes_per_cell": PRE_TS_BYTES_PER_CELL,
-            "avg_launch_ms": round(kern[dom][1], 4),
+            "achieved": round(gbs[dom], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs[dom] / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over this call on this workload "
+                              "(read = 2.0 x FETCH_SIZE, write = 1.0 x WRITE_SIZE, factors from a 1 GiB copy); "
+                              "profiles/mixing_step_1440x720x80.md",
+            "algorithmic_bytes_per_launch": B8_step,
+            "algorithmic_bytes_per_cell": 176 + 2 * 224,
+            "avg_launch_ms": round(t_dom, 4),
+            "alternative_accountings": {
+                "fused_minimum": {"bytes_per_cell": PRE_TS_BYTES_PER_CELL, "bytes": B_pre_diff,
+                                  "achieved": round(B_pre_diff / (t_dom * 1e-3) / 1e9, 1),
+                                  "frac": round(B_pre_diff / (t_dom * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+                "measured_traffic": None if traffic is None else {
+                    "bytes": traffic, "achieved": round(traffic / (t_dom * 1e-3) / 1e9, 1),
+                    "frac": round(traffic / (t_dom * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}},
             "copy_GBps": round(copy_gbps, 1), "frac_of_copy": round(gbs[dom] / copy_gbps, 4)}
+
+    # the 360x160x60 ACC-like channel (FP64), isoneutral one call
+    st1 = to_device_state(W.iso_config1())
+    for _ in range(args.warmup):
+        isoneutral.isoneutral_pre_and_diffusion_temp_salt(st1)
+    t1_ms = timed_events(lambda: isoneutral.isoneutral_pre_and_diffusion_temp_salt(st1), max(5, args.steps),
+                         stream)[0]
+    N1 = 360 * 160 * 60
+    del st1
+
+    batched = None
+    if not args.no_batched:
+        del state, td, vs
+        torch.cuda.empty_cache()
+        batched = batched_leg(args, stream)
+
     out = {
         "metric": "TDMA Mcolumns/s + isoneutral Mcells/s; mixed vs FP64 GPU at <=1e-12 rel err",
         "value": round(N / (ms * 1e-3) / 1e6, 2),
@@ -400,17 +596,27 @@ def main():
         "config": {"workload": f"mixed-precision TDMA + isoneutral step, {nx}x{ny}x{nz} quarter-degree grid "
                                f"(35% land, kbot U{{5..80}})", "nx": nx, "ny": ny, "nz": nz,
                    "precision": prec, "refine_steps": args.refine, "tdma_wet_cells": W_td},
-        "tdma_mcolumns_per_s": {k: round(v, 1) for k, v in rates["tdma_mcolumns_per_s"].items()},
-        "iso_mcells_per_s": round(rates["iso_mcells_per_s"], 1),
-        "iso_mcells_per_s_pre_then_TS": round(rates["iso_mcells_per_s_pre_then_TS"], 1),
+        "step_ms": {k: round(v, 4) for k, v in step_ms.items()},
+        "step_mcells_per_s": {k: round(N / (v * 1e-3) / 1e6, 1) for k, v in step_ms.items()},
+        "tdma_mcolumns_per_s": {p: round(C / (parts[f"tdma_{p}"][0] * 1e-3) / 1e6, 1) for p in ("fp64", "mixed")},
+        "tdma_roofline_frac": {p: round(gbs[f"tdma_{p}"] / HBM_PEAK_GBS, 4) for p in ("fp64", "mixed")},
         "mixed_over_fp64_tdma": round(parts["tdma_fp64"][0] / parts["tdma_mixed"][0], 3),
+        "iso_mcells_per_s": round(N / (t_dom * 1e-3) / 1e6, 1),
+        "iso_mcells_per_s_pre_then_TS": round(N / ((parts["iso_pre"][0] + parts["iso_diffusion_TS"][0]) * 1e-3)
+                                              / 1e6, 1),
+        "config1_iso_fp64": {"workload": "360x160x60 ACC-like channel, one call pre + T + S", "ms": round(t1_ms, 4),
+                             "mcells_per_s": round(N1 / (t1_ms * 1e-3) / 1e6, 1)},
         "kernel_ms_median": {k: round(v[0], 4) for k, v in parts.items()},
         "kernel_algorithmic_GBps": {k: round(v, 1) for k, v in gbs.items()},
         "roofline": roof,
     }
-    if not args.no_cpu_baseline:
-        print("[bench] CPU baseline (oracle on host cores)", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(nx, ny, nz, args.cpu_budget)
+    if batched is not None:
+        out["batched"] = batched
+    if snap is not None:
+        print("[bench] CPU baseline and per-output errors (oracle on host cores)", file=sys.stderr, flush=True)
+        base, errs = cpu_baseline(snap, args.cpu_budget)
+        out["cpu_baseline"] = base
+        out["max_rel_err_vs_fp64_oracle"] = {k: float(f"{v:.3e}") for k, v in errs.items()}
     print(json.dumps(out))
 
 

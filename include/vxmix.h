@@ -135,7 +135,10 @@ int vx_tdma_plan_create(const vx_tdma_system* systems, int32_t nsys, int32_t mod
 int vx_tdma_plan_run(vx_tdma_plan* plan, void* stream);
 /* Autotune: time (block size, blocks per CU) candidates per bucket on the
  * plan's data, keep the fastest; synchronises the stream. Choices are cached
- * process-wide for later plans of the same bucket, mode and size. */
+ * process-wide for later plans of the same bucket, mode and size. Tuning
+ * solves every system many times and writes x; a plan in which some x
+ * overlaps an input (in-place solve) is rejected with
+ * VX_ERR_INVALID_ARGUMENT, since its right-hand side would be overwritten. */
 int vx_tdma_plan_tune(vx_tdma_plan* plan, void* stream);
 /* First call captures vx_tdma_plan_run into a hipGraph; every call launches
  * that graph on `stream`. */

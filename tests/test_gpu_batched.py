@@ -88,3 +88,83 @@ def test_plan_rejects_bad_systems():
         TdmaBatch(sys, "bf16")
     with pytest.raises(TypeError):
         TdmaBatch([sys[0], bad], "mixed")
+
+
+# ---------------------------------------------------------------------------
+# The 64-grid batched workload at its real sizes: nx, ny ~ U{48..720},
+# nz ~ U{15..100}, kbot ~ U{1..nz} with 30% land (about 9.4 M columns and
+# 540 M cells). Every system is checked on 48 sampled columns per system
+# (dry columns included) against the oracle's dgtsv, and equals its own
+# single-grid solve value for value.
+# ---------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def batch64():
+    from veros_amd.batched import random_systems
+    sys64 = random_systems(64)
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(64)
+    picks = []
+    for s in sys64:
+        nx, ny, _ = s["a"].shape
+        ii = torch.from_numpy(rng.integers(0, nx, size=48)).cuda()
+        jj = torch.from_numpy(rng.integers(0, ny, size=48)).cuda()
+        cols = {k: s[k][ii, jj][None].cpu().numpy() for k in ("a", "b", "c", "d")}
+        kb = s["kbot"][ii, jj][None].cpu().numpy()
+        picks.append((ii, jj, O.solve_implicit_kbot(cols["a"], cols["b"], cols["c"], cols["d"], kb)))
+    yield sys64, picks
+    del sys64
+    torch.cuda.empty_cache()
+
+
+def _check_batch64(sys64, picks, precision, what):
+    torch.cuda.synchronize()
+    for q, (s, (ii, jj, ref)) in enumerate(zip(sys64, picks)):
+        got = s["x"][ii, jj][None].cpu().numpy()
+        assert rel_err(got, ref) <= 1e-12, (what, q, s["a"].shape)
+    for q in (0, 17, 63):                 # value for value against the single-grid solve
+        assert torch.equal(sys64[q]["x"], _single(sys64[q], precision)), (what, q)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch64_run_tune_and_graph(batch64, precision):
+    from veros_amd.batched import TdmaBatch
+    sys64, picks = batch64
+    cols = sum(s["a"].shape[0] * s["a"].shape[1] for s in sys64)
+    assert cols > 5_000_000
+    plan = TdmaBatch(sys64, precision)
+    assert plan.info()["total_columns"] == cols
+    for s in sys64:
+        s["x"].fill_(float("nan"))
+    plan.run()
+    _check_batch64(sys64, picks, precision, "run")
+    plan.tune()
+    for s in sys64:
+        s["x"].fill_(float("nan"))
+    plan.run()
+    _check_batch64(sys64, picks, precision, "tuned run")
+    for rep in range(2):
+        for s in sys64:
+            s["x"].fill_(float("nan"))
+        torch.cuda.synchronize()
+        plan.graph_launch()
+        _check_batch64(sys64, picks, precision, f"graph replay {rep}")
+    plan.close()
+
+
+def test_tune_rejects_in_place_plan():
+    """A plan whose x aliases d solves correctly, but tuning (many repeated
+    solves) would overwrite d, so tune refuses it."""
+    from veros_amd.batched import TdmaBatch
+    sys = _systems(seed=13)[:3]
+    ref = [_single(s, "fp64") for s in sys]
+    for s in sys:
+        s["x"] = s["d"]
+    plan = TdmaBatch(sys, "fp64")
+    plan.run()
+    torch.cuda.synchronize()
+    for s, r in zip(sys, ref):
+        assert torch.equal(s["x"], r)
+    with pytest.raises(ValueError):
+        plan.tune()
+    plan.close()

@@ -318,3 +318,41 @@ def test_pre_and_temp_salt_fused_graph_replays():
         for k in keys:
             assert torch.equal(getattr(ds_e.variables, k), getattr(ds_g.variables, k)), (rep, k)
         del junk
+
+
+def test_graph_replay_after_workspace_growth():
+    """A graph captured against the cached workspace still replays correctly
+    after a larger grid has made the cache allocate a bigger buffer: the old
+    buffer is never freed, so the replay cannot write into memory torch has
+    since handed to another tensor."""
+    from veros_amd import _arrays, isoneutral
+    for buf in _arrays._workspaces.values():   # start from an empty cache
+        _arrays._retired.append(buf)
+    _arrays._workspaces.clear()
+    st = _random_state(95, 22, 18, 12, cyclic=True)
+    ds_e, ds_g = _dev_state(st), _dev_state(st)
+    keys = PRE_KEYS + ("temp", "salt", "dtemp_iso", "dsalt_iso", "P_diss_iso")
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        isoneutral.isoneutral_pre_and_diffusion_temp_salt(ds_g)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    isoneutral.isoneutral_pre_and_diffusion_temp_salt(ds_e)
+    small = next(iter(_arrays._workspaces.values()))
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        isoneutral.isoneutral_pre_and_diffusion_temp_salt(ds_g)
+    big = _dev_state(_random_state(96, 60, 50, 30, cyclic=True))
+    isoneutral.isoneutral_pre_and_diffusion_temp_salt(big)       # grows the cached workspace
+    torch.cuda.synchronize()
+    assert next(iter(_arrays._workspaces.values())).numel() > small.numel()
+    sentinel = torch.full((small.numel() // 8 + 1,), 7.0, dtype=torch.float64, device="cuda")
+    for rep in range(2):
+        torch.cuda.synchronize()
+        graph.replay()
+        isoneutral.isoneutral_pre_and_diffusion_temp_salt(ds_e)
+        torch.cuda.synchronize()
+        for k in keys:
+            assert torch.equal(getattr(ds_e.variables, k), getattr(ds_g.variables, k)), (rep, k)
+    assert bool((sentinel == 7.0).all())

@@ -173,3 +173,32 @@ def test_step_is_linear_in_tracer():
     fw = O.isoneutral_diffusion_kernel(state, 2.0 * u - 3.0 * v, True)
     np.testing.assert_allclose(fw.temp, 2.0 * fu.temp - 3.0 * fv.temp, atol=1e-11)
     np.testing.assert_allclose(fw.dtemp_iso, 2.0 * fu.dtemp_iso - 3.0 * fv.dtemp_iso, atol=1e-11)
+
+
+def test_window_state_reproduces_whole_grid():
+    """The oracle on a window of a state (veros_amd.workloads.window_state)
+    gives the whole grid's values on the window's cells [2:-2, 2:-2], value for
+    value: pre, then temperature and salinity diffusion with the energy
+    branch. This is what lets large grids be checked on windows."""
+    from veros_amd import workloads as W
+    st = W.iso_state(np.random.default_rng(3), 30, 24, 10, dx_deg=1.0, dy_deg=1.0, y_origin=-40.0,
+                     dz_top=10.0, dz_bottom=40.0, land_frac=0.2, stratified=False, random_k_iso=True,
+                     random_aux=True, tau=2, taup1=1)
+    full = W.state_to_numpy_copy(st)
+    pre = O.isoneutral_diffusion_pre(full)
+    for k, v in pre._asdict().items():
+        setattr(full.variables, k, v)
+    O.isoneutral_diffusion(full, "temp", True)
+    O.isoneutral_diffusion(full, "salt", False)
+    for i0, i1, j0, j1 in ((0, 14, 0, 12), (10, 25, 5, 20), (20, 34, 14, 28)):
+        w = W.window_state(st, i0, i1, j0, j1)
+        pw = O.isoneutral_diffusion_pre(w)
+        for k in pre._fields:
+            assert np.array_equal(getattr(pre, k)[i0:i1, j0:j1][2:-2, 2:-2], getattr(pw, k)[2:-2, 2:-2]), k
+        for k, v in pw._asdict().items():
+            setattr(w.variables, k, v)
+        O.isoneutral_diffusion(w, "temp", True)
+        O.isoneutral_diffusion(w, "salt", False)
+        for k in ("temp", "salt", "dtemp_iso", "dsalt_iso", "P_diss_iso"):
+            a = getattr(full.variables, k)[i0:i1, j0:j1][2:-2, 2:-2]
+            assert np.array_equal(a, getattr(w.variables, k)[2:-2, 2:-2]), k

@@ -48,14 +48,21 @@ def require_contiguous(name, t):
 
 
 _workspaces: dict = {}
+# Every workspace buffer ever handed out stays referenced until the process
+# ends: a HIP graph captured against a buffer keeps its address, so freeing it
+# when a larger grid needs a larger buffer would let a later replay of that
+# graph write into memory torch has handed to another tensor.
+_retired: list = []
 
 
 def workspace(nbytes: int, dev: torch.device) -> torch.Tensor:
-    """A cached device buffer of at least ``nbytes`` (grown, never shrunk),
-    allocated on first use so the hot calls allocate nothing afterwards.
-    Calls that share it must be ordered on one stream."""
+    """A cached device buffer of at least ``nbytes`` (grown, never shrunk or
+    freed), allocated on first use so the hot calls allocate nothing
+    afterwards. Calls that share it must be ordered on one stream."""
     buf = _workspaces.get(dev)
     if buf is None or buf.numel() < nbytes:
+        if buf is not None:
+            _retired.append(buf)
         buf = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=dev)
         _workspaces[dev] = buf
     return buf

@@ -126,6 +126,8 @@ struct vx_tdma_plan {
     hipGraphExec_t exec = nullptr;
     // mode 1: the systems, each solved by the single-grid mixed kernel
     std::vector<vx_tdma_system> mixed;
+    // every non-empty system (tune checks that none solves in place)
+    std::vector<vx_tdma_system> all;
 };
 
 namespace vx {
@@ -162,7 +164,7 @@ extern "C" int vx_tdma_plan_create(const vx_tdma_system* systems, int32_t nsys, 
     // runs the single-grid mixed kernel (vx_tdma_mixed, one refinement step)
     // once per system, on the caller's stream or captured into one hipGraph.
     std::vector<std::vector<SysDesc>> per(kNumBuckets);
-    std::vector<vx_tdma_system> mixed;
+    std::vector<vx_tdma_system> mixed, all;
     for (int q = 0; q < nsys; ++q) {
         const vx_tdma_system& y = systems[q];
         if (y.nz < 1 || y.nx < 0 || y.ny < 0)
@@ -185,10 +187,12 @@ extern "C" int vx_tdma_plan_create(const vx_tdma_system* systems, int32_t nsys, 
                      : 0;
         per[(y.nz + 15) / 16 - 1].push_back(sd);
         if (mode == 1) mixed.push_back(y);
+        all.push_back(y);
     }
     auto* P = new vx_tdma_plan();
     P->mode = mode;
     P->mixed = std::move(mixed);
+    P->all = std::move(all);
     int dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
@@ -237,6 +241,28 @@ extern "C" int vx_tdma_plan_create(const vx_tdma_system* systems, int32_t nsys, 
 
 namespace vx {
 namespace {
+// Byte range [lo, hi) spanned by one (nx, ny, nz) float64 array of a system.
+std::pair<uintptr_t, uintptr_t> span_of(const double* p, const vx_tdma_system& y) {
+    const uintptr_t lo = reinterpret_cast<uintptr_t>(p);
+    const int64_t last = (y.nx - 1) * y.sx + (y.ny - 1) * y.sy + y.nz;
+    return {lo, lo + (uintptr_t)last * sizeof(double)};
+}
+
+// True when some system's x overlaps one of its (or another system's) inputs.
+bool solves_in_place(const std::vector<vx_tdma_system>& sys) {
+    for (const vx_tdma_system& y : sys) {
+        const auto xs = span_of(y.x, y);
+        for (const vx_tdma_system& z : sys) {
+            for (const double* in : {z.a, z.b, z.c, z.d, z.b_edge, z.d_edge}) {
+                if (!in) continue;
+                const auto r = span_of(in, z);
+                if (xs.first < r.second && r.first < xs.second) return true;
+            }
+        }
+    }
+    return false;
+}
+
 int run_plan(vx_tdma_plan* plan, hipStream_t s, const char* who) {
     if (plan->mode == 1) {
         for (const vx_tdma_system& y : plan->mixed) {
@@ -262,6 +288,11 @@ extern "C" int vx_tdma_plan_tune(vx_tdma_plan* plan, void* stream) {
     using namespace vx;
     const char* who = "vx_tdma_plan_tune";
     if (!plan) return fail(VX_ERR_INVALID_ARGUMENT, "%s: null plan", who);
+    // tuning solves every bucket many times over: an in-place system (x
+    // overlapping an input) would solve its own previous result
+    if (solves_in_place(plan->all))
+        return fail(VX_ERR_INVALID_ARGUMENT, "%s: a system's x overlaps an input (in-place solve); tuning would "
+                    "overwrite that input", who);
     if (plan->mode == 1) return VX_OK;  // per-system launches of a fixed shape: nothing to choose
     hipStream_t s = (hipStream_t)stream;
     hipEvent_t e0, e1;

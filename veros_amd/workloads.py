@@ -395,3 +395,63 @@ def state_to_numpy_copy(state):
     vs = state.variables
     new = SimpleNamespace(**{k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in vars(vs).items()})
     return SimpleNamespace(variables=new, settings=state.settings.copy())
+
+
+# ----------------------------------------------------------------------------
+# sub-windows of a state (for checking a large grid on a block of columns)
+# ----------------------------------------------------------------------------
+
+X_ARRAYS = ("dxt", "dxu", "xt", "xu")
+Y_ARRAYS = ("dyt", "dyu", "cost", "cosu", "yt", "yu")
+
+
+def _host(v):
+    if hasattr(v, "detach"):
+        return v.detach().cpu().numpy()
+    return np.asarray(v)
+
+
+def whole_grid_masks(state):
+    """maskT/U/V/W of the whole grid (NumPy), as window_state builds them."""
+    st = state.settings
+    return masks_from_kbot(_host(state.variables.kbot), st.nz, cyclic=bool(getattr(st, "enable_cyclic_x", True)))
+
+
+def window_state(state, i0, i1, j0, j1, *, masks=True, names=None, full_masks=None):
+    """A NumPy copy of the columns ``[i0, i1) x [j0, j1)`` of ``state`` (indices
+    of the halo-inclusive arrays; the window's outer two rows and columns act
+    as its halos). Arrays with leading ``(X, Y)`` axes are sliced on both,
+    the 1-D x metrics on ``[i0, i1)``, the y metrics on ``[j0, j1)``; other
+    arrays and scalars are copied. With ``masks``, maskT/U/V/W are built from
+    the whole grid's ``kbot`` (cyclic in x as ``settings.enable_cyclic_x``;
+    or taken from ``full_masks``) and then sliced, so they equal the whole
+    grid's masks on the window.
+
+    On the window's cells ``[2:-2, 2:-2]`` the isoneutral and vertical-mixing
+    operators read only cells of the window, so their results there equal the
+    whole grid's."""
+    vs = state.variables
+    X, Y = tuple(vs.kbot.shape)
+    out = {}
+    for k, v in vars(vs).items():
+        if names is not None and k not in names and k not in ("tau", "taup1", "taum1", "kbot"):
+            continue
+        if not hasattr(v, "shape") or np.ndim(v) == 0:
+            out[k] = v
+            continue
+        shp = tuple(v.shape)
+        if len(shp) >= 2 and shp[:2] == (X, Y):
+            out[k] = _host(v[i0:i1, j0:j1]).copy()
+        elif len(shp) == 1 and shp[0] == X and k in X_ARRAYS:
+            out[k] = _host(v[i0:i1]).copy()
+        elif len(shp) == 1 and shp[0] == Y and k in Y_ARRAYS:
+            out[k] = _host(v[j0:j1]).copy()
+        else:
+            out[k] = _host(v).copy()
+    st = state.settings
+    if masks:
+        full = full_masks if full_masks is not None else whole_grid_masks(state)
+        for k, m in full.items():
+            out[k] = m[i0:i1, j0:j1].copy()
+    settings = st.copy(nx=i1 - i0 - 4, ny=j1 - j0 - 4) if hasattr(st, "copy") else st
+    return SimpleNamespace(variables=SimpleNamespace(**out), settings=settings)
