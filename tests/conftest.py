@@ -63,3 +63,15 @@ def rel_err(x, ref, mask=None):
     if scale == 0.0:
         return float(diff)
     return float(diff / scale)
+
+
+def column_state_from_golden(rec):
+    """Rebuild the state of a vertmix_* / friction_* fixture (inputs in_*,
+    settings set_*) with veros_amd.workloads.MixSettings."""
+    from veros_amd.workloads import MixSettings
+
+    vs = SimpleNamespace(**{k[3:]: np.array(v) for k, v in rec.items() if k.startswith("in_")})
+    vs.tau = int(rec["set_tau"])
+    vs.taup1 = int(rec["set_taup1"])
+    kw = {k[4:]: v.item() for k, v in rec.items() if k.startswith("set_") and k not in ("set_tau", "set_taup1")}
+    return SimpleNamespace(variables=vs, settings=MixSettings(**kw))

@@ -13,8 +13,11 @@ Functions covered (reference file:line):
 * create_water_masks           veros/core/utilities.py:38-48
 * isoneutral_diffusion_pre     veros/core/isoneutral/isoneutral.py:18-229
 * isoneutral_diffusion_kernel  veros/core/isoneutral/diffusion.py:210-283 (iso and skew)
+* vertmix_tempsalt             veros/core/thermodynamics.py:248-300 (with enforce_boundaries)
+* implicit_vert_friction       veros/core/friction.py:90-205 (with ugrid_to_tgrid /
+                               vgrid_to_tgrid, veros/core/numerics.py:313-336)
 
-Usage: python tests/golden/make_golden.py
+Usage: python tests/golden/make_golden.py [column]   (no argument: every fixture)
 """
 from __future__ import annotations
 
@@ -50,8 +53,9 @@ def _import_reference():
     os.environ["VEROS_FLOAT_TYPE"] = "float64"
     from veros.core import operators, utilities  # noqa: F401
     from veros.core.isoneutral import isoneutral, diffusion
+    from veros.core import thermodynamics, friction
     return SimpleNamespace(operators=operators, utilities=utilities, isoneutral=isoneutral,
-                           diffusion=diffusion)
+                           diffusion=diffusion, thermodynamics=thermodynamics, friction=friction)
 
 
 def _ref_state(state):
@@ -60,7 +64,8 @@ def _ref_state(state):
     st = state.settings
     v = SimpleNamespace(**{k: (np.array(x, copy=True) if isinstance(x, np.ndarray) else x)
                            for k, x in vars(vs).items()})
-    s = SimpleNamespace(nz=st.nz, dt_tracer=st.dt_tracer, K_iso_steep=st.K_iso_steep,
+    s = SimpleNamespace(nz=st.nz, dt_tracer=st.dt_tracer, dt_mom=st.dt_mom, enable_cyclic_x=st.enable_cyclic_x,
+                        K_iso_steep=st.K_iso_steep,
                         iso_slopec=st.iso_slopec, iso_dslope=st.iso_dslope,
                         enable_conserve_energy=st.enable_conserve_energy, grav=st.grav,
                         rho_0=st.rho_0, eq_of_state_type=1)
@@ -177,13 +182,90 @@ def iso_cases(ref):
     return out
 
 
+COLUMN_SPECS = {
+    # cyclic x, tau != 0, noisy in/out fields (every written range visible)
+    "12x10x8_cyclic": dict(seed=31, nx=12, ny=10, nz=8, cyclic=True, tau=2, taup1=0, dz=(10.0, 60.0)),
+    # closed x, single-cell and full-depth columns, an all-land row
+    "9x7x5_closed": dict(seed=32, nx=9, ny=7, nz=5, cyclic=False, tau=1, taup1=2, dz=(10.0, 30.0)),
+    # deeper columns, strong mixing (kappa up to 1 m^2/s), cyclic
+    "16x12x20_strong": dict(seed=33, nx=16, ny=12, nz=20, cyclic=True, tau=0, taup1=1, dz=(5.0, 200.0),
+                            kappa_hi=1.0),
+}
+
+
+def column_state(name):
+    """A small state with the isoneutral, vertical-mixing and friction fields
+    (this package's seeded generators) for the column-solve fixtures."""
+    from veros_amd import workloads as W
+    sp = COLUMN_SPECS[name]
+    rng = np.random.default_rng(sp["seed"])
+    settings = W.MixSettings(nx=sp["nx"], ny=sp["ny"], nz=sp["nz"], enable_cyclic_x=sp["cyclic"],
+                             dt_mom=1800.0)
+    st = W.iso_state(rng, sp["nx"], sp["ny"], sp["nz"], dx_deg=1.0, dy_deg=1.0, y_origin=-30.0,
+                     dz_top=sp["dz"][0], dz_bottom=sp["dz"][1], land_frac=0.25, cyclic=sp["cyclic"],
+                     settings=settings, tau=sp["tau"], taup1=sp["taup1"], random_aux=True, jitter=0.05)
+    vs = st.variables
+    nz = sp["nz"]
+    if name == "9x7x5_closed":
+        vs.kbot[2, 2:-2] = nz          # single wet cell columns
+        vs.kbot[3, 2:-2] = 1           # full-depth columns
+        vs.kbot[5, 2:-2] = 0           # an all-land row
+        for k, m in W.masks_from_kbot(vs.kbot, nz, cyclic=False).items():
+            setattr(vs, k, m)
+    kh = sp.get("kappa_hi", 1e-2)
+    W.add_vertmix_fields(rng, st, kappa_hi=kh, random_aux=True)
+    W.add_friction_fields(rng, st, kappa_hi=kh, random_aux=True)
+    return st
+
+
+VERTMIX_KEYS = ("temp", "salt", "kappaH", "forc_temp_surface", "forc_salt_surface", "dtemp_vmix", "dsalt_vmix",
+                "kbot", "dzt", "dzw")
+FRICTION_KEYS = ("u", "v", "kappaM", "du_mix", "dv_mix", "K_diss_v", "kbot", "maskU", "maskV", "dxt", "dxu",
+                 "area_t", "area_v", "dzt", "dzw")
+
+
+def column_cases(ref):
+    out = {}
+    for name in COLUMN_SPECS:
+        st = column_state(name)
+        vs, settings = st.variables, st.settings
+        base = {"set_" + k: np.asarray(getattr(settings, k)) for k in ("nx", "ny", "nz", "dt_tracer", "dt_mom",
+                                                                        "enable_cyclic_x")}
+        base["set_tau"] = np.asarray(vs.tau)
+        base["set_taup1"] = np.asarray(vs.taup1)
+        rs = _ref_state(st)
+        res = ref.thermodynamics.vertmix_tempsalt(rs)
+        rec = dict(base, **{"in_" + k: np.asarray(getattr(vs, k)) for k in VERTMIX_KEYS})
+        rec.update({"out_" + k: np.asarray(v) for k, v in res._asdict().items()})
+        out[f"vertmix_{name}"] = rec
+        rs = _ref_state(st)
+        res = ref.friction.implicit_vert_friction(rs)
+        rec = dict(base, **{"in_" + k: np.asarray(getattr(vs, k)) for k in FRICTION_KEYS})
+        rec.update({"out_" + k: np.asarray(v) for k, v in res._asdict().items()})
+        out[f"friction_{name}"] = rec
+    return out
+
+
 def main():
     sys.path.insert(0, str(REPO))
     ref = _import_reference()
+    only = set(sys.argv[1:])
+    if only:
+        # regenerate only the named groups (tdma, iso, column)
+        if "column" in only:
+            for name, rec in column_cases(ref).items():
+                np.savez_compressed(HERE / f"{name}.npz", **rec)
+                print("wrote", f"{name}.npz")
+        if only - {"column"}:
+            raise SystemExit(f"unknown groups {sorted(only - {'column'})}")
+        return
     for name, rec in tdma_cases(ref).items():
         np.savez_compressed(HERE / f"tdma_{name}.npz", **rec)
         print("wrote", f"tdma_{name}.npz")
     for name, rec in iso_cases(ref).items():
+        np.savez_compressed(HERE / f"{name}.npz", **rec)
+        print("wrote", f"{name}.npz")
+    for name, rec in column_cases(ref).items():
         np.savez_compressed(HERE / f"{name}.npz", **rec)
         print("wrote", f"{name}.npz")
 

@@ -13,7 +13,9 @@ import numpy as np
 import pytest
 import torch
 
-from conftest import rel_err
+from types import SimpleNamespace
+
+from conftest import column_state_from_golden, golden_files, load_golden, rel_err
 from oracle import friction as F
 from veros_amd import workloads as W
 
@@ -61,6 +63,18 @@ def test_friction_random_against_oracle(shape, cyclic, land, levels):
     _check(out, ref, st)
     for k in KEYS:
         assert getattr(out, k) is getattr(ds.variables, k)  # in place
+
+
+@pytest.mark.parametrize("path", golden_files("friction_"), ids=lambda p: p.stem)
+def test_friction_golden(path):
+    """Against the reference's own outputs (veros.core.friction.implicit_vert_friction,
+    written by tests/golden/make_golden.py), with the bar of _check."""
+    rec = load_golden(path)
+    st = column_state_from_golden(rec)
+    ref = SimpleNamespace(**{k[4:]: v for k, v in rec.items() if k.startswith("out_")})
+    from veros_amd import friction
+    out = friction.implicit_vert_friction(_dev(st))
+    _check(out, ref, st)
 
 
 def test_friction_channel_360x160x60_against_oracle():

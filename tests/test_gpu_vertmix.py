@@ -13,7 +13,9 @@ import numpy as np
 import pytest
 import torch
 
-from conftest import rel_err
+from types import SimpleNamespace
+
+from conftest import column_state_from_golden, golden_files, load_golden, rel_err
 from oracle import vertmix as V
 from veros_amd import workloads as W
 
@@ -61,6 +63,18 @@ def test_vertmix_random_against_oracle(shape, cyclic, land):
     _check(out, ref, st)
     for k in KEYS:
         assert getattr(out, k) is getattr(ds.variables, k)  # in place
+
+
+@pytest.mark.parametrize("path", golden_files("vertmix_"), ids=lambda p: p.stem)
+def test_vertmix_golden(path):
+    """Against the reference's own outputs (veros.core.thermodynamics.vertmix_tempsalt,
+    written by tests/golden/make_golden.py), with the bar of _check."""
+    rec = load_golden(path)
+    st = column_state_from_golden(rec)
+    ref = SimpleNamespace(**{k[4:]: v for k, v in rec.items() if k.startswith("out_")})
+    from veros_amd import thermodynamics
+    out = thermodynamics.vertmix_tempsalt(_dev(st))
+    _check(out, ref, st)
 
 
 def test_vertmix_channel_360x160x60_against_oracle():
