@@ -25,16 +25,40 @@ def _solve(p_or_arrays, kbot, precision="fp64", refine=1, b_edge=None, d_edge=No
     return x.cpu().numpy()
 
 
+def _max_cond(g):
+    """Largest 2-norm condition number of the fixture's wet systems."""
+    a, b, c, kb = g["a"], g.get("b_edge", g["b"]), g["c"], g["kbot"]
+    nz = a.shape[-1]
+    worst = 1.0
+    for i, j in zip(*np.nonzero((kb >= 1) & (kb <= nz))):
+        k0 = kb[i, j] - 1
+        diag = g["b"][i, j, k0:].copy()
+        if "b_edge" in g:
+            diag[0] = b[i, j, k0]
+        m = np.diag(diag) + np.diag(a[i, j, k0 + 1:], -1) + np.diag(c[i, j, k0:-1], 1)
+        worst = max(worst, np.linalg.cond(m))
+    return worst
+
+
 @pytest.mark.parametrize("path", golden_files("tdma_"), ids=lambda p: p.stem)
 @pytest.mark.parametrize("mode", ["fp64", "mixed1", "mixed2"])
 def test_golden(path, mode):
     """Every reference fixture, including the standard-normal coefficients of
     the reference's own tridiagonal test (dgtsv pivots there; these systems
-    need no pivot, so a Thomas sweep matches it to rounding)."""
+    need no pivot, so an elimination without pivoting matches it to rounding).
+
+    The mixed form's error after R refinement steps is about
+    (cond * u32)^(R+1) of the solution (u32 = 2^-24, each float32 solve
+    leaves a relative error of about cond * u32 and the float64 residual
+    step multiplies it by the same factor). On the diffusion fixtures
+    cond < 10, so every mode meets 1e-12; the standard-normal systems reach
+    cond = 126, where one step guarantees only about 6e-11 and two steps
+    1e-15. The bar is therefore max(1e-12, 8 * (cond * u32)^(R+1))."""
     g = load_golden(path)
     prec, refine = ("fp64", 0) if mode == "fp64" else ("mixed", int(mode[-1]))
     x = _solve((g["a"], g["b"], g["c"], g["d"]), g["kbot"], prec, refine, g.get("b_edge"), g.get("d_edge"))
-    assert rel_err(x, g["x_implicit"]) <= TOL
+    tol = TOL if prec == "fp64" else max(TOL, 8.0 * (_max_cond(g) * 2.0 ** -24) ** (refine + 1))
+    assert rel_err(x, g["x_implicit"]) <= tol
     assert np.all(x[~g["water"]] == 0)
 
 

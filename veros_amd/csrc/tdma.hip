@@ -12,9 +12,37 @@
 // indexed). Coefficients are read in 16-byte pairs where alignment allows;
 // each lane streams whole cache lines of its own column, so the L1 does the
 // transposition and HBM sees full-line traffic.
-#include "tdma_core.hpp"
+#include "tdma_seg.hpp"
+
+#include <cstdlib>
 
 namespace vx {
+
+template <int S, bool VEC>
+__global__ __launch_bounds__(kSegBlock) void tdma_seg_f64_kernel(TdmaArgs p) {
+    seg_f64<S, VEC>(p, seg_lane_of(p, S));
+}
+
+template <int S, bool VEC, int REFINE>
+__global__ __launch_bounds__(kSegBlock) void tdma_seg_mixed_kernel(TdmaArgs p) {
+    seg_mixed<S, VEC, REFINE>(p, seg_lane_of(p, S));
+}
+
+// LDS-staged segmented kernels (16-byte aligned operands only). Dynamic LDS:
+// seg_lds_wave_bytes(S) per wave.
+template <int S, bool VEC>
+__global__ __launch_bounds__(kSegLdsBlock) void tdma_segl_f64_kernel(TdmaArgs p) {
+    extern __shared__ double seg_lds[];
+    double* lds = seg_lds + (threadIdx.x / 64) * (seg_lds_wave_bytes(S) / 8);
+    seg_f64_lds<S>(p, seg_lane_of(p, S), lds);
+}
+
+template <int S, bool VEC, int REFINE>
+__global__ __launch_bounds__(kSegLdsBlock) void tdma_segl_mixed_kernel(TdmaArgs p) {
+    extern __shared__ double seg_lds[];
+    double* lds = seg_lds + (threadIdx.x / 64) * (seg_lds_wave_bytes(S) / 8);
+    seg_mixed_lds<S, REFINE>(p, seg_lane_of(p, S), lds);
+}
 
 template <int NZB, bool VEC>
 __global__ __launch_bounds__(kTdmaBlock) void tdma_f64_kernel(TdmaArgs p) {
@@ -154,6 +182,113 @@ int launch_bucket(const TdmaArgs& p, hipStream_t s, const char* who) {
     return check_launch(who);
 }
 
+// Segmented kernels (kSegP lanes per column), instantiated per levels-per-lane S.
+template <template <int, bool> class K>
+int launch_seg(const TdmaArgs& p, hipStream_t s, const char* who) {
+    const int S = seg_levels(p.nz);
+    const bool vec = vec_ok(p);
+    const int64_t threads = p.ncol * kSegP;
+    const dim3 grid((unsigned)((threads + kSegBlock - 1) / kSegBlock));
+    const dim3 block(kSegBlock);
+#define VX_SEG_CASE(N)                                                            \
+    case N:                                                                       \
+        if (vec) hipLaunchKernelGGL((K<N, true>::fn), grid, block, 0, s, p);    \
+        else hipLaunchKernelGGL((K<N, false>::fn), grid, block, 0, s, p);       \
+        break;
+    switch (S) {
+        VX_SEG_CASE(2)
+        VX_SEG_CASE(4)
+        VX_SEG_CASE(6)
+        VX_SEG_CASE(8)
+        VX_SEG_CASE(10)
+        VX_SEG_CASE(12)
+        VX_SEG_CASE(14)
+        VX_SEG_CASE(16)
+        default:
+            return fail(VX_ERR_UNSUPPORTED, "%s: nz=%d", who, p.nz);
+    }
+#undef VX_SEG_CASE
+    return check_launch(who);
+}
+
+// LDS-staged segmented kernels: only with 16-byte aligned operands.
+template <template <int, bool> class K>
+int launch_segl(const TdmaArgs& p, hipStream_t s, const char* who) {
+    const int S = seg_levels(p.nz);
+    const int64_t threads = p.ncol * kSegP;
+    const dim3 grid((unsigned)((threads + kSegLdsBlock - 1) / kSegLdsBlock));
+    const dim3 block(kSegLdsBlock);
+    const size_t lds = (size_t)seg_lds_wave_bytes(S) * (kSegLdsBlock / 64);
+#define VX_SEGL_CASE(N)                                                     \
+    case N:                                                                 \
+        hipLaunchKernelGGL((K<N, true>::fn), grid, block, lds, s, p);       \
+        break;
+    switch (S) {
+        VX_SEGL_CASE(2)
+        VX_SEGL_CASE(4)
+        VX_SEGL_CASE(6)
+        VX_SEGL_CASE(8)
+        VX_SEGL_CASE(10)
+        VX_SEGL_CASE(12)
+        VX_SEGL_CASE(14)
+        VX_SEGL_CASE(16)
+        default:
+            return fail(VX_ERR_UNSUPPORTED, "%s: nz=%d", who, p.nz);
+    }
+#undef VX_SEGL_CASE
+    return check_launch(who);
+}
+
+template <int N, bool V>
+struct SeglF64K {
+    static constexpr auto fn = tdma_segl_f64_kernel<N, V>;
+};
+template <int N, bool V>
+struct SeglMixK0 {
+    static constexpr auto fn = tdma_segl_mixed_kernel<N, V, 0>;
+};
+template <int N, bool V>
+struct SeglMixK1 {
+    static constexpr auto fn = tdma_segl_mixed_kernel<N, V, 1>;
+};
+template <int N, bool V>
+struct SeglMixK2 {
+    static constexpr auto fn = tdma_segl_mixed_kernel<N, V, 2>;
+};
+
+template <int N, bool V>
+struct SegF64K {
+    static constexpr auto fn = tdma_seg_f64_kernel<N, V>;
+};
+template <int N, bool V>
+struct SegMixK0 {
+    static constexpr auto fn = tdma_seg_mixed_kernel<N, V, 0>;
+};
+template <int N, bool V>
+struct SegMixK1 {
+    static constexpr auto fn = tdma_seg_mixed_kernel<N, V, 1>;
+};
+template <int N, bool V>
+struct SegMixK2 {
+    static constexpr auto fn = tdma_seg_mixed_kernel<N, V, 2>;
+};
+
+// Kernel family of the single-grid solves (VX_TDMA_KERNEL, read once):
+// "lds" (default; segmented with LDS-staged coalesced copies when the operands
+// are 16-byte aligned, else "segment"), "segment" (segmented, register loads),
+// "column" (lane per column).
+int tdma_kernel_family() {
+    static const int fam = [] {
+        const char* e = std::getenv("VX_TDMA_KERNEL");
+        if (e != nullptr && std::string(e) == "column") return 0;
+        if (e != nullptr && std::string(e) == "segment") return 1;
+        return 2;
+    }();
+    return fam;
+}
+bool use_column_kernels() { return tdma_kernel_family() == 0; }
+bool use_lds_kernels(const TdmaArgs& p) { return tdma_kernel_family() == 2 && vec_ok(p); }
+
 template <int N, bool V>
 struct F64K {
     static constexpr auto fn = tdma_f64_kernel<N, V>;
@@ -203,6 +338,8 @@ extern "C" int vx_tdma_f64(void* stream, const double* a, const double* b, const
     TdmaArgs p = make_args(a, b, c, d, b_edge, d_edge, kbot, kbot_sx, kbot_sy, x, nx, ny, nz, sx, sy);
     int rc = validate_tdma(p, nx, ny, nz, "vx_tdma_f64");
     if (rc != VX_OK || p.ncol == 0) return rc;
+    if (use_lds_kernels(p)) return launch_segl<SeglF64K>(p, (hipStream_t)stream, "vx_tdma_f64");
+    if (!use_column_kernels()) return launch_seg<SegF64K>(p, (hipStream_t)stream, "vx_tdma_f64");
     return launch_bucket<F64K>(p, (hipStream_t)stream, "vx_tdma_f64");
 }
 
@@ -216,6 +353,20 @@ extern "C" int vx_tdma_mixed(void* stream, const double* a, const double* b, con
     TdmaArgs p = make_args(a, b, c, d, b_edge, d_edge, kbot, kbot_sx, kbot_sy, x, nx, ny, nz, sx, sy);
     int rc = validate_tdma(p, nx, ny, nz, "vx_tdma_mixed");
     if (rc != VX_OK || p.ncol == 0) return rc;
+    if (use_lds_kernels(p)) {
+        switch (refine_steps) {
+            case 0: return launch_segl<SeglMixK0>(p, (hipStream_t)stream, "vx_tdma_mixed");
+            case 1: return launch_segl<SeglMixK1>(p, (hipStream_t)stream, "vx_tdma_mixed");
+            default: return launch_segl<SeglMixK2>(p, (hipStream_t)stream, "vx_tdma_mixed");
+        }
+    }
+    if (!use_column_kernels()) {
+        switch (refine_steps) {
+            case 0: return launch_seg<SegMixK0>(p, (hipStream_t)stream, "vx_tdma_mixed");
+            case 1: return launch_seg<SegMixK1>(p, (hipStream_t)stream, "vx_tdma_mixed");
+            default: return launch_seg<SegMixK2>(p, (hipStream_t)stream, "vx_tdma_mixed");
+        }
+    }
     switch (refine_steps) {
         case 0: return launch_bucket<MixK0>(p, (hipStream_t)stream, "vx_tdma_mixed");
         case 1: return launch_bucket<MixK1>(p, (hipStream_t)stream, "vx_tdma_mixed");

@@ -1,0 +1,534 @@
+// Segmented (partitioned) tridiagonal solve: kSegP lanes per column.
+//
+// The column's levels [0, P*S) are cut into P segments of S levels; lane s of
+// the column's lane group holds levels [s*S, s*S + S) in registers. Rows
+// outside the wet suffix [k0, nz) are identity rows (a = c = d = 0, b = 1), as
+// in solve_tridiagonal_jax (veros/core/operators.py:133-136), so a single
+// segmented system covers the whole column and its dry part solves to exact
+// zeros.
+//
+// Per segment the last level is the interface unknown L_s. The S-1 interior
+// rows are eliminated in the lane (forward sweep carrying the left spike from
+// L_{s-1}, back substitution), which expresses every interior x as
+//   x_t = alpha_t + beta_t * L_{s-1} + gamma_t * L_s.
+// Substituting that into the interface rows gives a P x P tridiagonal system
+// in L_0..L_{P-1} (its rows need alpha/beta/gamma of the next lane's first
+// level, fetched by lane shuffles); every lane of the group solves it
+// redundantly with Thomas and then reconstructs its own levels.
+//
+// Two ways in: the register variant loads the lane's S levels of a, b, c, d
+// itself (any alignment); the LDS variant (16-byte aligned operands) copies
+// the wave's 8 columns into LDS with coalesced global_load_lds_dwordx4 and
+// stores x back coalesced through LDS. Either way all loads of a wave are in
+// flight at once: one memory round trip per column.
+#pragma once
+#include "tdma_core.hpp"
+
+namespace vx {
+
+constexpr int kSegP = 8;              // lanes per column
+constexpr int kSegBlock = 256;        // register variant: 32 columns per block
+constexpr int kSegLdsBlock = 128;     // LDS variant: 2 waves, 16 columns per block
+constexpr int kSegCols = 64 / kSegP;  // columns per wave
+
+// levels per lane for a depth nz (even, >= 2)
+__host__ __device__ constexpr int seg_levels(int nz) {
+    const int s = (nz + kSegP - 1) / kSegP;
+    return s < 2 ? 2 : s + (s & 1);
+}
+
+// LDS bytes of one wave in the LDS variant: a, b, c, d images of 8 columns
+__host__ __device__ constexpr int seg_lds_wave_bytes(int S) { return 4 * kSegCols * kSegP * S * 8; }
+
+struct SegLane {
+    int64_t base;  // column offset in a, b, c, d, x
+    int s;         // segment index in [0, kSegP)
+    int lo;        // first level of the segment
+    int k0;        // first wet level (nz when the column is dry)
+    bool valid;    // the column exists
+};
+
+__device__ __forceinline__ SegLane seg_lane_of(const TdmaArgs& p, int S) {
+    SegLane L;
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t col = gt / kSegP;
+    L.s = (int)(gt % kSegP);
+    L.lo = L.s * S;
+    L.valid = col < p.ncol;
+    L.base = 0;
+    L.k0 = p.nz;
+    if (L.valid) {
+        const int64_t i = col / p.ny;
+        const int64_t j = col - i * p.ny;
+        L.base = i * p.sx + j * p.sy;
+        const int kb = p.kbot[i * p.kx + j * p.ky];
+        L.k0 = (kb >= 1 && kb <= p.nz) ? kb - 1 : p.nz;
+    }
+    return L;
+}
+
+// 1/x from v_rcp_f64 refined by two Newton steps (about 1 ulp; finite
+// non-zero x only).
+__device__ __forceinline__ double seg_rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+__device__ __forceinline__ float seg_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// The masked system on the lane's levels: identity rows off the wet suffix,
+// a = 0 (and b, d from b_edge, d_edge when given) at the edge row k0, c = 0
+// at the top row nz-1. Values at rows off the wet suffix may be garbage on
+// entry; they are replaced by selects.
+template <int S>
+__device__ __forceinline__ void seg_mask_rows(const TdmaArgs& p, const SegLane& L, double be, double de,
+                                              double (&va)[S], double (&vb)[S], double (&vc)[S], double (&vd)[S]) {
+    const int nz = p.nz;
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+        const int k = L.lo + t;
+        const bool wet = k >= L.k0 && k < nz;
+        va[t] = (wet && k != L.k0) ? va[t] : 0.0;
+        vb[t] = wet ? vb[t] : 1.0;
+        vc[t] = (wet && k != nz - 1) ? vc[t] : 0.0;
+        vd[t] = wet ? vd[t] : 0.0;
+    }
+    // edge-row replacements (uniform branches: the pointers are kernel
+    // arguments); a dry column (k0 == nz) has no edge row
+    const int ke = L.k0 < nz ? L.k0 - L.lo : -1;
+    if (p.b_edge != nullptr) {
+#pragma unroll
+        for (int t = 0; t < S; ++t) vb[t] = (t == ke) ? be : vb[t];
+    }
+    if (p.d_edge != nullptr) {
+#pragma unroll
+        for (int t = 0; t < S; ++t) vd[t] = (t == ke) ? de : vd[t];
+    }
+}
+
+// b_edge / d_edge at the edge row, loaded by the lane whose segment holds it.
+__device__ __forceinline__ void seg_edge_values(const TdmaArgs& p, const SegLane& L, int S, double& be,
+                                                double& de) {
+    be = 0.0;
+    de = 0.0;
+    const bool edge_here = L.valid && L.k0 < p.nz && L.k0 >= L.lo && L.k0 < L.lo + S;
+    if (edge_here && p.b_edge) be = p.b_edge[L.base + L.k0];
+    if (edge_here && p.d_edge) de = p.d_edge[L.base + L.k0];
+}
+
+// ---------------------------------------------------------------------------
+// float64 segmented solve, in place: on return z holds the lane's levels of
+// x; va and vc are overwritten by the sweep's factors.
+// ---------------------------------------------------------------------------
+template <int S>
+__device__ __forceinline__ void seg_solve_f64(const SegLane& L, double (&va)[S], const double (&vb)[S],
+                                              double (&vc)[S], double (&z)[S]) {
+    // a (spike w, then beta), c (c', then gamma) and z (dp, then alpha) are
+    // transformed in place; every entry is read before it is overwritten, and
+    // the interface level S-1 is left untouched.
+    const double aL = va[S - 1], bL = vb[S - 1], cL = vc[S - 1], dL = z[S - 1];
+    {
+        const double r = seg_rcp(vb[0]);
+        vc[0] *= r;
+        va[0] *= r;
+        z[0] *= r;
+    }
+#pragma unroll
+    for (int t = 1; t < S - 1; ++t) {
+        const double at = va[t];
+        const double r = seg_rcp(vb[t] - at * vc[t - 1]);
+        vc[t] *= r;
+        va[t] = -at * va[t - 1] * r;
+        z[t] = (z[t] - at * z[t - 1]) * r;
+    }
+    va[S - 2] = -va[S - 2];
+    vc[S - 2] = -vc[S - 2];
+#pragma unroll
+    for (int t = S - 3; t >= 0; --t) {
+        const double cp = vc[t];
+        z[t] = z[t] - cp * z[t + 1];
+        va[t] = -va[t] - cp * va[t + 1];
+        vc[t] = -cp * vc[t + 1];
+    }
+    const bool last = L.s == kSegP - 1;
+    double n_al = __shfl_down(z[0], 1, kSegP);
+    double n_be = __shfl_down(va[0], 1, kSegP);
+    double n_ga = __shfl_down(vc[0], 1, kSegP);
+    if (last) {
+        n_al = 0.0;
+        n_be = 0.0;
+        n_ga = 0.0;
+    }
+    const double RA = aL * va[S - 2];
+    const double RB = bL + aL * vc[S - 2] + cL * n_be;
+    const double RC = cL * n_ga;
+    const double RD = dL - aL * z[S - 2] - cL * n_al;
+    double cps[kSegP], dps[kSegP];
+    double cpr = 0.0, dpr = 0.0;
+#pragma unroll
+    for (int j = 0; j < kSegP; ++j) {
+        const double Aj = __shfl(RA, j, kSegP);
+        const double Bj = __shfl(RB, j, kSegP);
+        const double Cj = __shfl(RC, j, kSegP);
+        const double Dj = __shfl(RD, j, kSegP);
+        const double r = seg_rcp(Bj - Aj * cpr);
+        cpr = Cj * r;
+        dpr = (Dj - Aj * dpr) * r;
+        cps[j] = cpr;
+        dps[j] = dpr;
+    }
+    double Ls = 0.0, Lm = 0.0, Ln = 0.0;
+#pragma unroll
+    for (int j = kSegP - 1; j >= 0; --j) {
+        Ln = dps[j] - cps[j] * Ln;
+        if (j == L.s) Ls = Ln;
+        if (j == L.s - 1) Lm = Ln;
+    }
+#pragma unroll
+    for (int t = 0; t < S - 1; ++t) z[t] = z[t] + va[t] * Lm + vc[t] * Ls;
+    z[S - 1] = Ls;
+}
+
+// ---------------------------------------------------------------------------
+// float32 factorisation of the lane's segment and of the group's interface
+// system, formed once and reused by every float32 solve of the mixed form.
+// ---------------------------------------------------------------------------
+template <int S>
+struct SegF32 {
+    float af[S];       // a of every level
+    float rd[S];       // 1 / den of the interior rows 0..S-2
+    float cp[S];       // c' of the interior rows
+    float be[S];       // beta of the interior rows
+    float ga[S];       // gamma of the interior rows
+    float aL, cL;      // interface row a, c
+    float RA[kSegP];   // the interface system's sub-diagonal (rows 0..P-1)
+    float rdr[kSegP];  // its 1 / den
+    float cpr[kSegP];  // its c'
+};
+
+template <int S>
+__device__ __forceinline__ void seg_factor_f32(const SegLane& L, const double (&va)[S], const double (&vb)[S],
+                                               const double (&vc)[S], SegF32<S>& F) {
+#pragma unroll
+    for (int t = 0; t < S; ++t) F.af[t] = (float)va[t];
+    float w[S];
+    {
+        const float r = seg_rcp((float)vb[0]);
+        F.rd[0] = r;
+        F.cp[0] = (float)vc[0] * r;
+        w[0] = F.af[0] * r;
+    }
+#pragma unroll
+    for (int t = 1; t < S - 1; ++t) {
+        const float r = seg_rcp((float)vb[t] - F.af[t] * F.cp[t - 1]);
+        F.rd[t] = r;
+        F.cp[t] = (float)vc[t] * r;
+        w[t] = -F.af[t] * w[t - 1] * r;
+    }
+    F.be[S - 2] = -w[S - 2];
+    F.ga[S - 2] = -F.cp[S - 2];
+#pragma unroll
+    for (int t = S - 3; t >= 0; --t) {
+        F.be[t] = -w[t] - F.cp[t] * F.be[t + 1];
+        F.ga[t] = -F.cp[t] * F.ga[t + 1];
+    }
+    F.aL = F.af[S - 1];
+    F.cL = (float)vc[S - 1];
+    const bool last = L.s == kSegP - 1;
+    float n_be = __shfl_down(F.be[0], 1, kSegP);
+    float n_ga = __shfl_down(F.ga[0], 1, kSegP);
+    if (last) {
+        n_be = 0.f;
+        n_ga = 0.f;
+    }
+    const float RA = F.aL * F.be[S - 2];
+    const float RB = (float)vb[S - 1] + F.aL * F.ga[S - 2] + F.cL * n_be;
+    const float RC = F.cL * n_ga;
+    float cpr = 0.f;
+#pragma unroll
+    for (int j = 0; j < kSegP; ++j) {
+        const float Aj = __shfl(RA, j, kSegP);
+        const float Bj = __shfl(RB, j, kSegP);
+        const float Cj = __shfl(RC, j, kSegP);
+        const float r = seg_rcp(Bj - Aj * cpr);
+        cpr = Cj * r;
+        F.RA[j] = Aj;
+        F.rdr[j] = r;
+        F.cpr[j] = cpr;
+    }
+}
+
+// z: rhs in, solution out (float32, the lane's levels).
+template <int S>
+__device__ __forceinline__ void seg_solve_f32(const SegLane& L, const SegF32<S>& F, float (&z)[S]) {
+    z[0] *= F.rd[0];
+#pragma unroll
+    for (int t = 1; t < S - 1; ++t) z[t] = (z[t] - F.af[t] * z[t - 1]) * F.rd[t];
+#pragma unroll
+    for (int t = S - 3; t >= 0; --t) z[t] = z[t] - F.cp[t] * z[t + 1];
+    float n_al = __shfl_down(z[0], 1, kSegP);
+    if (L.s == kSegP - 1) n_al = 0.f;
+    const float RD = z[S - 1] - F.aL * z[S - 2] - F.cL * n_al;
+    float dps[kSegP];
+    float dpr = 0.f;
+#pragma unroll
+    for (int j = 0; j < kSegP; ++j) {
+        const float Dj = __shfl(RD, j, kSegP);
+        dpr = (Dj - F.RA[j] * dpr) * F.rdr[j];
+        dps[j] = dpr;
+    }
+    float Ls = 0.f, Lm = 0.f, Ln = 0.f;
+#pragma unroll
+    for (int j = kSegP - 1; j >= 0; --j) {
+        Ln = dps[j] - F.cpr[j] * Ln;
+        if (j == L.s) Ls = Ln;
+        if (j == L.s - 1) Lm = Ln;
+    }
+#pragma unroll
+    for (int t = 0; t < S - 1; ++t) z[t] = z[t] + F.be[t] * Lm + F.ga[t] * Ls;
+    z[S - 1] = Ls;
+}
+
+// Mixed precision on the masked rows: float32 solve, then REFINE steps of the
+// float64 residual r = d - A x solved again in float32 (same factors) and
+// added in float64. On return x holds the lane's levels.
+template <int S, int REFINE>
+__device__ __forceinline__ void seg_mixed_core(const SegLane& L, const double (&va)[S], const double (&vb)[S],
+                                               const double (&vc)[S], const double (&vd)[S], double (&x)[S]) {
+    SegF32<S> F;
+    seg_factor_f32<S>(L, va, vb, vc, F);
+    float z[S];
+#pragma unroll
+    for (int t = 0; t < S; ++t) z[t] = (float)vd[t];
+    seg_solve_f32<S>(L, F, z);
+#pragma unroll
+    for (int t = 0; t < S; ++t) x[t] = (double)z[t];
+#pragma unroll 1
+    for (int step = 0; step < REFINE; ++step) {
+        double xm = __shfl_up(x[S - 1], 1, kSegP);  // level lo-1, from the lane below
+        double xp = __shfl_down(x[0], 1, kSegP);    // level lo+S, from the lane above
+        if (L.s == 0) xm = 0.0;
+        if (L.s == kSegP - 1) xp = 0.0;
+#pragma unroll
+        for (int t = 0; t < S; ++t) {
+            const double lo_x = t == 0 ? xm : x[t > 0 ? t - 1 : 0];
+            const double hi_x = t == S - 1 ? xp : x[t < S - 1 ? t + 1 : S - 1];
+            z[t] = (float)(vd[t] - va[t] * lo_x - vb[t] * x[t] - vc[t] * hi_x);
+        }
+        seg_solve_f32<S>(L, F, z);
+#pragma unroll
+        for (int t = 0; t < S; ++t) x[t] += (double)z[t];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Register variant: each lane loads its own levels (8-byte or, when VEC,
+// 16-byte loads; VEC needs lo, nz and the column base even and aligned
+// operands).
+// ---------------------------------------------------------------------------
+template <int S, bool VEC>
+__device__ __forceinline__ void seg_load(const double* __restrict__ col, int lo, int nz, int k0, double (&v)[S]) {
+    if constexpr (VEC) {
+#pragma unroll
+        for (int t = 0; t < S; t += 2) {
+            const int k = lo + t;
+            if (k < nz && k + 1 >= k0) {
+                const double2 w = *reinterpret_cast<const double2*>(col + k);
+                v[t] = w.x;
+                v[t + 1] = w.y;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < S; ++t) {
+            const int k = lo + t;
+            if (k < nz && k >= k0) v[t] = col[k];
+        }
+    }
+}
+
+template <int S, bool VEC>
+__device__ __forceinline__ void seg_store(double* __restrict__ col, int lo, int nz, const double (&v)[S]) {
+    if constexpr (VEC) {
+#pragma unroll
+        for (int t = 0; t < S; t += 2) {
+            if (lo + t < nz) *reinterpret_cast<double2*>(col + lo + t) = make_double2(v[t], v[t + 1]);
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < S; ++t) {
+            if (lo + t < nz) col[lo + t] = v[t];
+        }
+    }
+}
+
+template <int S, bool VEC>
+__device__ __forceinline__ void seg_rows(const TdmaArgs& p, const SegLane& L, double (&va)[S], double (&vb)[S],
+                                         double (&vc)[S], double (&vd)[S]) {
+    const int nz = p.nz;
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+        va[t] = 0.0;
+        vb[t] = 1.0;
+        vc[t] = 0.0;
+        vd[t] = 0.0;
+    }
+    if (L.valid && L.k0 < nz && L.lo + S > L.k0) {
+        seg_load<S, VEC>(p.a + L.base, L.lo, nz, L.k0, va);
+        seg_load<S, VEC>(p.b + L.base, L.lo, nz, L.k0, vb);
+        seg_load<S, VEC>(p.c + L.base, L.lo, nz, L.k0, vc);
+        seg_load<S, VEC>(p.d + L.base, L.lo, nz, L.k0, vd);
+    }
+    double be, de;
+    seg_edge_values(p, L, S, be, de);
+    seg_mask_rows<S>(p, L, be, de, va, vb, vc, vd);
+}
+
+template <int S, bool VEC>
+__device__ __forceinline__ void seg_f64(const TdmaArgs& p, const SegLane& L) {
+    double va[S], vb[S], vc[S], vd[S];
+    seg_rows<S, VEC>(p, L, va, vb, vc, vd);
+    seg_solve_f64<S>(L, va, vb, vc, vd);
+    if (L.valid) seg_store<S, VEC>(p.x + L.base, L.lo, p.nz, vd);
+}
+
+template <int S, bool VEC, int REFINE>
+__device__ __forceinline__ void seg_mixed(const TdmaArgs& p, const SegLane& L) {
+    double va[S], vb[S], vc[S], vd[S];
+    seg_rows<S, VEC>(p, L, va, vb, vc, vd);
+    double x[S];
+    seg_mixed_core<S, REFINE>(L, va, vb, vc, vd, x);
+    if (L.valid) seg_store<S, VEC>(p.x + L.base, L.lo, p.nz, x);
+}
+
+// ---------------------------------------------------------------------------
+// LDS variant. A wave owns the 8 columns of its lane groups; the columns' a,
+// b, c, d are copied by the wave straight from global memory into its LDS
+// region with global_load_lds_dwordx4 (no VGPRs for staging): lane l of the
+// copy's instruction i moves the 16-byte pair p = 64 i + l of the wave's LDS
+// image [column][P*S], so consecutive lanes read consecutive 16 bytes of a
+// column and every instruction is a coalesced run. Only pairs that touch the
+// wet suffix are copied. Each lane then reads its segment's levels from LDS,
+// and the solution goes back the same way (segments written to LDS, pairs
+// stored coalesced). LDS per wave: seg_lds_wave_bytes(S) (20 KiB at nz = 80).
+// ---------------------------------------------------------------------------
+typedef __attribute__((address_space(3))) void* lds_void_ptr;
+
+__device__ __forceinline__ void seg_wait_vmem() {
+    // s_waitcnt vmcnt(0): the wave's LDS copies have landed
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+}
+
+// The pairs a lane copies for its wave; offsets and predicates are formed
+// once and shared by the four arrays. The column's base and first wet level
+// come from its lane group by shuffles.
+template <int S>
+struct SegCopyPlan {
+    static constexpr int PS = kSegP * S;
+    static constexpr int NPAIR = kSegCols * PS / 2;
+    static constexpr int NI = (NPAIR + 63) / 64;
+    int64_t off[NI];
+    bool pred[NI];
+};
+
+template <int S>
+__device__ __forceinline__ void seg_copy_plan(const SegLane& L, int lane, int nz, bool wet_only,
+                                              SegCopyPlan<S>& cp) {
+    using P = SegCopyPlan<S>;
+#pragma unroll
+    for (int i = 0; i < P::NI; ++i) {
+        const int pidx = i * 64 + lane;
+        const int c = pidx / (P::PS / 2);
+        const int k = 2 * (pidx - c * (P::PS / 2));
+        const int src = (c < kSegCols ? c : 0) * kSegP;  // first lane of column c's group
+        const int64_t b = __shfl(L.base, src);
+        const int k0 = __shfl(L.k0, src);
+        const bool v = __shfl((int)L.valid, src) != 0;
+        cp.off[i] = b + k;
+        cp.pred[i] = pidx < P::NPAIR && v && k < nz && (!wet_only || k + 1 >= k0);
+    }
+}
+
+template <int S>
+__device__ __forceinline__ void seg_lds_copy(const double* __restrict__ arr, double* lds_arr,
+                                             const SegCopyPlan<S>& cp) {
+#pragma unroll
+    for (int i = 0; i < SegCopyPlan<S>::NI; ++i) {
+        if (cp.pred[i])
+            __builtin_amdgcn_global_load_lds((const void*)(arr + cp.off[i]), (lds_void_ptr)(lds_arr + 128 * i), 16,
+                                             0, 0);
+    }
+}
+
+template <int S>
+__device__ __forceinline__ void seg_lds_read(const double* lds_arr, const SegLane& L, double (&v)[S]) {
+    const double* src = lds_arr + ((threadIdx.x & 63) / kSegP) * (kSegP * S) + L.lo;
+#pragma unroll
+    for (int t = 0; t < S; t += 2) {
+        const double2 w = *reinterpret_cast<const double2*>(src + t);
+        v[t] = w.x;
+        v[t + 1] = w.y;
+    }
+}
+
+template <int S>
+__device__ __forceinline__ void seg_rows_lds(const TdmaArgs& p, const SegLane& L, double* lds, double (&va)[S],
+                                             double (&vb)[S], double (&vc)[S], double (&vd)[S]) {
+    constexpr int IMG = kSegCols * kSegP * S;
+    SegCopyPlan<S> cp;
+    seg_copy_plan<S>(L, threadIdx.x & 63, p.nz, true, cp);
+    seg_lds_copy<S>(p.a, lds + 0 * IMG, cp);
+    seg_lds_copy<S>(p.b, lds + 1 * IMG, cp);
+    seg_lds_copy<S>(p.c, lds + 2 * IMG, cp);
+    seg_lds_copy<S>(p.d, lds + 3 * IMG, cp);
+    double be, de;
+    seg_edge_values(p, L, S, be, de);
+    seg_wait_vmem();
+    seg_lds_read<S>(lds + 0 * IMG, L, va);
+    seg_lds_read<S>(lds + 1 * IMG, L, vb);
+    seg_lds_read<S>(lds + 2 * IMG, L, vc);
+    seg_lds_read<S>(lds + 3 * IMG, L, vd);
+    seg_mask_rows<S>(p, L, be, de, va, vb, vc, vd);
+}
+
+// Store the lane's levels of x through LDS (the a image is free by now; a
+// wave's LDS operations complete in order): segments written, then coalesced
+// 16-byte pairs stored by consecutive lanes.
+template <int S>
+__device__ __forceinline__ void seg_store_lds(const TdmaArgs& p, const SegLane& L, double* lds, const double (&x)[S]) {
+    constexpr int PS = kSegP * S;
+    double* img = lds + ((threadIdx.x & 63) / kSegP) * PS + L.lo;
+#pragma unroll
+    for (int t = 0; t < S; t += 2) *reinterpret_cast<double2*>(img + t) = make_double2(x[t], x[t + 1]);
+    SegCopyPlan<S> cp;
+    seg_copy_plan<S>(L, threadIdx.x & 63, p.nz, false, cp);
+#pragma unroll
+    for (int i = 0; i < SegCopyPlan<S>::NI; ++i) {
+        if (cp.pred[i]) {
+            const int pidx = i * 64 + (threadIdx.x & 63);
+            const double2 w = *reinterpret_cast<const double2*>(lds + 2 * pidx);
+            *reinterpret_cast<double2*>(p.x + cp.off[i]) = w;
+        }
+    }
+}
+
+template <int S>
+__device__ __forceinline__ void seg_f64_lds(const TdmaArgs& p, const SegLane& L, double* lds) {
+    double va[S], vb[S], vc[S], vd[S];
+    seg_rows_lds<S>(p, L, lds, va, vb, vc, vd);
+    seg_solve_f64<S>(L, va, vb, vc, vd);
+    seg_store_lds<S>(p, L, lds, vd);
+}
+
+template <int S, int REFINE>
+__device__ __forceinline__ void seg_mixed_lds(const TdmaArgs& p, const SegLane& L, double* lds) {
+    double va[S], vb[S], vc[S], vd[S];
+    seg_rows_lds<S>(p, L, lds, va, vb, vc, vd);
+    double x[S];
+    seg_mixed_core<S, REFINE>(L, va, vb, vc, vd, x);
+    seg_store_lds<S>(p, L, lds, x);
+}
+
+}  // namespace vx
