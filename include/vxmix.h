@@ -252,6 +252,28 @@ int vx_iso_pre_diffusion_ts_f64(void* stream, const vx_grid* grid, const vx_iso_
                                 void* workspace);
 
 /*
+ * vx_iso_pre_diffusion_ts_f64 without a workspace argument. No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). The face fluxes and the wet-cell plan live in a fixed device
+ * array of the library, 192 MiB per GPU, held from the library's load on
+ * that GPU until the process ends; grids whose
+ * vx_iso_pre_diffusion_ts_workspace_bytes exceed it are rejected with
+ * VX_ERR_UNSUPPORTED. The call allocates nothing (no hipMalloc, no helper
+ * stream: every launch is on `stream`). Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU (other streams, other host
+ * threads) race and must be serialised by the caller. Results equal
+ * vx_iso_pre_diffusion_ts_f64's.
+ */
+int vx_iso_pre_diffusion_ts_f64_noalloc(void* stream, const vx_grid* grid, const vx_iso_params* prm,
+                                        double* temp, double* salt, int32_t tau, int32_t taup1,
+                                        const double* K_iso,
+                                        double* Ai_ez, double* Ai_nz, double* Ai_bx, double* Ai_by,
+                                        double* K_11, double* K_22, double* K_33,
+                                        double* dtemp_iso, double* dsalt_iso,
+                                        const double* int_drhodT, const double* int_drhodS, double* P_diss);
+
+/*
  * vertmix_tempsalt (veros/core/thermodynamics.py:248-300): implicit vertical
  * mixing of temperature and salinity with diffusivity kappaH (X, Y, nz) and
  * the surface forcing forc_temp_surface / forc_salt_surface (X, Y) on the top

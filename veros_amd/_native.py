@@ -78,6 +78,9 @@ _SIGNATURES = {
                                                    c_void_p, c_void_p, c_i32, c_i32, c_void_p] + [c_void_p] * 7
                                     + [c_void_p] * 6),
     "vx_iso_pre_diffusion_ts_workspace_bytes": (c_i64, [ctypes.POINTER(VxGrid)]),
+    "vx_iso_pre_diffusion_ts_f64_noalloc": (ctypes.c_int, [c_void_p, ctypes.POINTER(VxGrid),
+                                                           ctypes.POINTER(VxIsoParams), c_void_p, c_void_p, c_i32,
+                                                           c_i32, c_void_p] + [c_void_p] * 7 + [c_void_p] * 5),
     "vx_iso_skew_diffusion_f64": (ctypes.c_int, [c_void_p, ctypes.POINTER(VxGrid), ctypes.POINTER(VxIsoParams),
                                                  c_void_p, c_i32, c_i32, c_void_p] + [c_void_p] * 6
                                   + [c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -1681,10 +1681,11 @@ bool tensor_ok(const double* Ai_ez, const double* Ai_nz, const double* Ai_bx, co
 template <bool FUSED>
 int launch_pre(const Grid& g, const Prm& p, const Fields& f, const double* K_iso, double* Ai_ez, double* Ai_nz,
                double* Ai_bx, double* Ai_by, double* K_11, double* K_22, double* K_33, PreFlux fo, hipStream_t s,
-               const char* who, hipStream_t* dry_stream = nullptr) {
+               const char* who, hipStream_t* dry_stream = nullptr, bool use_aux = true) {
     int rc = VX_OK;
     const int64_t ncol = (int64_t)(g.X - 3) * (g.Y - 3);  // one wave per column, 4 per block
-    const hipStream_t sd = fork_aux(s);
+    // without the helper stream (use_aux false) the zero fill runs on `s`
+    const hipStream_t sd = use_aux ? fork_aux(s) : s;
     if (dry_stream) *dry_stream = sd;
     // Blocks per CU of the zero fill (0: one wave per column). Beside the
     // whole one call, one block per CU fills in the background at a modest
@@ -1882,14 +1883,16 @@ extern "C" int vx_iso_diffusion_adjoint_f64(void* stream, const vx_grid* grid, c
     return check_launch(who);
 }
 
-extern "C" int vx_iso_pre_diffusion_ts_f64(void* stream, const vx_grid* grid, const vx_iso_params* prm, double* temp,
-                                           double* salt, int32_t tau, int32_t taup1, const double* K_iso,
-                                           double* Ai_ez, double* Ai_nz, double* Ai_bx, double* Ai_by, double* K_11,
-                                           double* K_22, double* K_33, double* dtemp_iso, double* dsalt_iso,
-                                           const double* int_drhodT, const double* int_drhodS, double* P_diss,
-                                           void* workspace) {
-    using namespace vx;
-    const char* who = "vx_iso_pre_diffusion_ts_f64";
+namespace vx {
+namespace {
+
+// The one call (pre, then the temperature and salinity steps) with the given
+// workspace; `use_aux` false keeps every launch on `s` (no helper stream).
+int pre_diffusion_ts(hipStream_t s, const vx_grid* grid, const vx_iso_params* prm, double* temp, double* salt,
+                     int32_t tau, int32_t taup1, const double* K_iso, double* Ai_ez, double* Ai_nz, double* Ai_bx,
+                     double* Ai_by, double* K_11, double* K_22, double* K_33, double* dtemp_iso, double* dsalt_iso,
+                     const double* int_drhodT, const double* int_drhodS, double* P_diss, void* workspace,
+                     bool use_aux, const char* who) {
     Grid g;
     int rc = check_common(grid, prm, tau, taup1, g, who);
     if (rc) return rc;
@@ -1916,7 +1919,6 @@ extern "C" int vx_iso_pre_diffusion_ts_f64(void* stream, const vx_grid* grid, co
     a.Aez = Ai_ez, a.Anz = Ai_nz, a.Abx = Ai_bx, a.Aby = Ai_by, a.K11 = K_11, a.K22 = K_22, a.K33 = K_33;
     a.P = P_diss;
     const Prm p = make_prm(prm);
-    hipStream_t s = (hipStream_t)stream;
     const PreFlux fo{{a.t[0].fe, a.t[1].fe}, {a.t[0].fn, a.t[1].fn}, {a.t[0].ft, a.t[1].ft}};
     // the wet-cell plan of the step (reads kbot only)
     const WetPlan wp = wire_wet_plan(g, workspace, 2);
@@ -1926,10 +1928,85 @@ extern "C" int vx_iso_pre_diffusion_ts_f64(void* stream, const vx_grid* grid, co
     // zero fill overlaps the whole call and is joined at its end.
     hipStream_t sd = s;
     rc = launch_pre<true>(g, p, Fields{temp, salt, tau}, K_iso, Ai_ez, Ai_nz, Ai_bx, Ai_by, K_11, K_22, K_33, fo, s,
-                          who, &sd);
+                          who, &sd, use_aux);
     if (!rc) rc = launch_wet_step<2, true, true>(g, p, a, wp, s, who);
     const int rj = join_aux(s, sd, who);
     return rc ? rc : rj;
+}
+
+}  // namespace
+}  // namespace vx
+
+extern "C" int vx_iso_pre_diffusion_ts_f64(void* stream, const vx_grid* grid, const vx_iso_params* prm, double* temp,
+                                           double* salt, int32_t tau, int32_t taup1, const double* K_iso,
+                                           double* Ai_ez, double* Ai_nz, double* Ai_bx, double* Ai_by, double* K_11,
+                                           double* K_22, double* K_33, double* dtemp_iso, double* dsalt_iso,
+                                           const double* int_drhodT, const double* int_drhodS, double* P_diss,
+                                           void* workspace) {
+    return vx::pre_diffusion_ts((hipStream_t)stream, grid, prm, temp, salt, tau, taup1, K_iso, Ai_ez, Ai_nz, Ai_bx,
+                                Ai_by, K_11, K_22, K_33, dtemp_iso, dsalt_iso, int_drhodT, int_drhodS, P_diss,
+                                workspace, true, "vx_iso_pre_diffusion_ts_f64");
+}
+
+// ---------------------------------------------------------------------------
+// The one call without a caller workspace: the face fluxes and the wet-cell
+// plan live in `scratch`, a device array of the library (one per GPU, held
+// until the process ends). Its device address is taken once per device with
+// hipGetSymbolAddress; host code never passes the array's own name to a
+// kernel. The entry point allocates nothing: no hipMalloc, no helper stream.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int64_t kNoAllocScratchBytes = int64_t(192) << 20;  // 192 MiB
+}
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace vx {
+namespace {
+// Device address of `scratch` on the current device, looked up on the first
+// call on that device (null and an error message if the lookup fails).
+void* scratch_on_current_device(const char* who) {
+    static std::mutex m;
+    static std::map<int, void*> per_device;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        fail(VX_ERR_LAUNCH, "%s: hipGetDevice failed", who);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(m);
+    auto it = per_device.find(dev);
+    if (it != per_device.end()) return it->second;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr) {
+        fail(VX_ERR_LAUNCH, "%s: hipGetSymbolAddress(scratch) failed: %s", who, hipGetErrorString(e));
+        return nullptr;
+    }
+    per_device[dev] = ptr;
+    return ptr;
+}
+}  // namespace
+}  // namespace vx
+
+extern "C" int vx_iso_pre_diffusion_ts_f64_noalloc(void* stream, const vx_grid* grid, const vx_iso_params* prm,
+                                                   double* temp, double* salt, int32_t tau, int32_t taup1,
+                                                   const double* K_iso, double* Ai_ez, double* Ai_nz, double* Ai_bx,
+                                                   double* Ai_by, double* K_11, double* K_22, double* K_33,
+                                                   double* dtemp_iso, double* dsalt_iso, const double* int_drhodT,
+                                                   const double* int_drhodS, double* P_diss) {
+    using namespace vx;
+    const char* who = "vx_iso_pre_diffusion_ts_f64_noalloc";
+    const int64_t need = vx_iso_pre_diffusion_ts_workspace_bytes(grid);
+    if (need < 0) return fail(VX_ERR_INVALID_ARGUMENT, "%s: invalid grid", who);
+    if (need > kNoAllocScratchBytes)
+        return fail(VX_ERR_UNSUPPORTED,
+                    "%s: the grid needs %lld bytes of scratch, above the library's fixed %lld (use "
+                    "vx_iso_pre_diffusion_ts_f64 with a workspace)",
+                    who, (long long)need, (long long)kNoAllocScratchBytes);
+    void* ws = scratch_on_current_device(who);
+    if (!ws) return VX_ERR_LAUNCH;
+    return pre_diffusion_ts((hipStream_t)stream, grid, prm, temp, salt, tau, taup1, K_iso, Ai_ez, Ai_nz, Ai_bx,
+                            Ai_by, K_11, K_22, K_33, dtemp_iso, dsalt_iso, int_drhodT, int_drhodS, P_diss, ws, false,
+                            who);
 }
 
 extern "C" int vx_iso_pre_f64(void* stream, const vx_grid* grid, const vx_iso_params* prm, const double* temp,
