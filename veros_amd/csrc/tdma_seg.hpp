@@ -12,9 +12,9 @@
 // L_{s-1}, back substitution), which expresses every interior x as
 //   x_t = alpha_t + beta_t * L_{s-1} + gamma_t * L_s.
 // Substituting that into the interface rows gives a P x P tridiagonal system
-// in L_0..L_{P-1} (its rows need alpha/beta/gamma of the next lane's first
-// level, fetched by lane shuffles); every lane of the group solves it
-// redundantly with Thomas and then reconstructs its own levels.
+// in L_0..L_{P-1}, one row per lane (its rows need alpha/beta/gamma of the
+// next lane's first level); the group solves it by parallel cyclic reduction
+// with DPP row shifts, and each lane then reconstructs its own levels.
 //
 // Two ways in: the register variant loads the lane's S levels of a, b, c, d
 // itself (any alignment); the LDS variant (16-byte aligned operands) copies
@@ -77,6 +77,110 @@ __device__ __forceinline__ double seg_rcp(double x) {
     return fma(r, e, r);
 }
 __device__ __forceinline__ float seg_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Moves between the lanes of a group. A group's 8 lanes lie inside one
+// 16-lane DPP row, so row shifts reach every lane of the group; values that
+// come from the neighbouring group (or from outside the row, read as 0) are
+// replaced by the callers.
+constexpr int kDppShl = 0x100;  // row_shl:n  lane l reads lane l+n
+constexpr int kDppShr = 0x110;  // row_shr:n  lane l reads lane l-n
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// The interface system, one row per lane of the group (A x_{s-1} + B x_s +
+// C x_{s+1} = D), solved by parallel cyclic reduction over strides 1, 2, 4:
+// each step folds the rows s-d and s+d into row s, so that row s couples
+// only to s-2d and s+2d; rows outside the group act as identity rows (the
+// rows s < d already have A = 0, those with s + d >= P have C = 0). After
+// the last stride every row is diagonal. Each step's multipliers depend on
+// the matrix only; PcrFactors keeps them for further right-hand sides.
+template <class T>
+struct PcrFactors {
+    T al[3], ga[3];  // multipliers of the rows below and above, per stride
+    T rb;            // 1 / the final diagonal
+};
+
+template <int D, class T>
+__device__ __forceinline__ void pcr_step(int s, T& A, T& B, T& C, T& R, T& al, T& ga) {
+    T Am = dpp_move<kDppShr + D>(A), Bm = dpp_move<kDppShr + D>(B), Cm = dpp_move<kDppShr + D>(C);
+    T Rm = dpp_move<kDppShr + D>(R);
+    T Ap = dpp_move<kDppShl + D>(A), Bp = dpp_move<kDppShl + D>(B), Cp = dpp_move<kDppShl + D>(C);
+    T Rp = dpp_move<kDppShl + D>(R);
+    if (s < D) {
+        Am = T(0);
+        Bm = T(1);
+        Cm = T(0);
+        Rm = T(0);
+    }
+    if (s + D >= kSegP) {
+        Ap = T(0);
+        Bp = T(1);
+        Cp = T(0);
+        Rp = T(0);
+    }
+    al = -A * seg_rcp(Bm);
+    ga = -C * seg_rcp(Bp);
+    A = al * Am;
+    C = ga * Cp;
+    B = B + al * Cm + ga * Ap;
+    R = R + al * Rm + ga * Rp;
+}
+
+// Solves the group's interface system; returns this lane's unknown and
+// fills F (when non-null) for pcr_resolve.
+template <class T>
+__device__ __forceinline__ T pcr_solve(int s, T A, T B, T C, T R, PcrFactors<T>* F) {
+    T al, ga;
+    pcr_step<1>(s, A, B, C, R, al, ga);
+    if (F) F->al[0] = al, F->ga[0] = ga;
+    pcr_step<2>(s, A, B, C, R, al, ga);
+    if (F) F->al[1] = al, F->ga[1] = ga;
+    pcr_step<4>(s, A, B, C, R, al, ga);
+    if (F) F->al[2] = al, F->ga[2] = ga;
+    const T rb = seg_rcp(B);
+    if (F) F->rb = rb;
+    return R * rb;
+}
+
+template <int D, class T>
+__device__ __forceinline__ T pcr_rhs_step(int s, T R, T al, T ga) {
+    T Rm = dpp_move<kDppShr + D>(R);
+    T Rp = dpp_move<kDppShl + D>(R);
+    if (s < D) Rm = T(0);
+    if (s + D >= kSegP) Rp = T(0);
+    return R + al * Rm + ga * Rp;
+}
+
+// Another right-hand side with the factors of pcr_solve.
+template <class T>
+__device__ __forceinline__ T pcr_resolve(int s, T R, const PcrFactors<T>& F) {
+    R = pcr_rhs_step<1>(s, R, F.al[0], F.ga[0]);
+    R = pcr_rhs_step<2>(s, R, F.al[1], F.ga[1]);
+    R = pcr_rhs_step<4>(s, R, F.al[2], F.ga[2]);
+    return R * F.rb;
+}
+
+// x of the lane above / below in the group (0 at the group's ends).
+template <class T>
+__device__ __forceinline__ T grp_next(int s, T v) {
+    const T w = dpp_move<kDppShl + 1>(v);
+    return s == kSegP - 1 ? T(0) : w;
+}
+template <class T>
+__device__ __forceinline__ T grp_prev(int s, T v) {
+    const T w = dpp_move<kDppShr + 1>(v);
+    return s == 0 ? T(0) : w;
+}
 
 // The masked system on the lane's levels: identity rows off the wet suffix,
 // a = 0 (and b, d from b_edge, d_edge when given) at the edge row k0, c = 0
@@ -152,40 +256,15 @@ __device__ __forceinline__ void seg_solve_f64(const SegLane& L, double (&va)[S],
         va[t] = -va[t] - cp * va[t + 1];
         vc[t] = -cp * vc[t + 1];
     }
-    const bool last = L.s == kSegP - 1;
-    double n_al = __shfl_down(z[0], 1, kSegP);
-    double n_be = __shfl_down(va[0], 1, kSegP);
-    double n_ga = __shfl_down(vc[0], 1, kSegP);
-    if (last) {
-        n_al = 0.0;
-        n_be = 0.0;
-        n_ga = 0.0;
-    }
+    const double n_al = grp_next(L.s, z[0]);
+    const double n_be = grp_next(L.s, va[0]);
+    const double n_ga = grp_next(L.s, vc[0]);
     const double RA = aL * va[S - 2];
     const double RB = bL + aL * vc[S - 2] + cL * n_be;
     const double RC = cL * n_ga;
     const double RD = dL - aL * z[S - 2] - cL * n_al;
-    double cps[kSegP], dps[kSegP];
-    double cpr = 0.0, dpr = 0.0;
-#pragma unroll
-    for (int j = 0; j < kSegP; ++j) {
-        const double Aj = __shfl(RA, j, kSegP);
-        const double Bj = __shfl(RB, j, kSegP);
-        const double Cj = __shfl(RC, j, kSegP);
-        const double Dj = __shfl(RD, j, kSegP);
-        const double r = seg_rcp(Bj - Aj * cpr);
-        cpr = Cj * r;
-        dpr = (Dj - Aj * dpr) * r;
-        cps[j] = cpr;
-        dps[j] = dpr;
-    }
-    double Ls = 0.0, Lm = 0.0, Ln = 0.0;
-#pragma unroll
-    for (int j = kSegP - 1; j >= 0; --j) {
-        Ln = dps[j] - cps[j] * Ln;
-        if (j == L.s) Ls = Ln;
-        if (j == L.s - 1) Lm = Ln;
-    }
+    const double Ls = pcr_solve<double>(L.s, RA, RB, RC, RD, nullptr);
+    const double Lm = grp_prev(L.s, Ls);
 #pragma unroll
     for (int t = 0; t < S - 1; ++t) z[t] = z[t] + va[t] * Lm + vc[t] * Ls;
     z[S - 1] = Ls;
@@ -203,9 +282,7 @@ struct SegF32 {
     float be[S];       // beta of the interior rows
     float ga[S];       // gamma of the interior rows
     float aL, cL;      // interface row a, c
-    float RA[kSegP];   // the interface system's sub-diagonal (rows 0..P-1)
-    float rdr[kSegP];  // its 1 / den
-    float cpr[kSegP];  // its c'
+    PcrFactors<float> pcr;  // the group's interface system
 };
 
 template <int S>
@@ -236,28 +313,12 @@ __device__ __forceinline__ void seg_factor_f32(const SegLane& L, const double (&
     }
     F.aL = F.af[S - 1];
     F.cL = (float)vc[S - 1];
-    const bool last = L.s == kSegP - 1;
-    float n_be = __shfl_down(F.be[0], 1, kSegP);
-    float n_ga = __shfl_down(F.ga[0], 1, kSegP);
-    if (last) {
-        n_be = 0.f;
-        n_ga = 0.f;
-    }
+    const float n_be = grp_next(L.s, F.be[0]);
+    const float n_ga = grp_next(L.s, F.ga[0]);
     const float RA = F.aL * F.be[S - 2];
     const float RB = (float)vb[S - 1] + F.aL * F.ga[S - 2] + F.cL * n_be;
     const float RC = F.cL * n_ga;
-    float cpr = 0.f;
-#pragma unroll
-    for (int j = 0; j < kSegP; ++j) {
-        const float Aj = __shfl(RA, j, kSegP);
-        const float Bj = __shfl(RB, j, kSegP);
-        const float Cj = __shfl(RC, j, kSegP);
-        const float r = seg_rcp(Bj - Aj * cpr);
-        cpr = Cj * r;
-        F.RA[j] = Aj;
-        F.rdr[j] = r;
-        F.cpr[j] = cpr;
-    }
+    (void)pcr_solve<float>(L.s, RA, RB, RC, 0.f, &F.pcr);
 }
 
 // z: rhs in, solution out (float32, the lane's levels).
@@ -268,24 +329,10 @@ __device__ __forceinline__ void seg_solve_f32(const SegLane& L, const SegF32<S>&
     for (int t = 1; t < S - 1; ++t) z[t] = (z[t] - F.af[t] * z[t - 1]) * F.rd[t];
 #pragma unroll
     for (int t = S - 3; t >= 0; --t) z[t] = z[t] - F.cp[t] * z[t + 1];
-    float n_al = __shfl_down(z[0], 1, kSegP);
-    if (L.s == kSegP - 1) n_al = 0.f;
+    const float n_al = grp_next(L.s, z[0]);
     const float RD = z[S - 1] - F.aL * z[S - 2] - F.cL * n_al;
-    float dps[kSegP];
-    float dpr = 0.f;
-#pragma unroll
-    for (int j = 0; j < kSegP; ++j) {
-        const float Dj = __shfl(RD, j, kSegP);
-        dpr = (Dj - F.RA[j] * dpr) * F.rdr[j];
-        dps[j] = dpr;
-    }
-    float Ls = 0.f, Lm = 0.f, Ln = 0.f;
-#pragma unroll
-    for (int j = kSegP - 1; j >= 0; --j) {
-        Ln = dps[j] - F.cpr[j] * Ln;
-        if (j == L.s) Ls = Ln;
-        if (j == L.s - 1) Lm = Ln;
-    }
+    const float Ls = pcr_resolve<float>(L.s, RD, F.pcr);
+    const float Lm = grp_prev(L.s, Ls);
 #pragma unroll
     for (int t = 0; t < S - 1; ++t) z[t] = z[t] + F.be[t] * Lm + F.ga[t] * Ls;
     z[S - 1] = Ls;
@@ -307,10 +354,8 @@ __device__ __forceinline__ void seg_mixed_core(const SegLane& L, const double (&
     for (int t = 0; t < S; ++t) x[t] = (double)z[t];
 #pragma unroll 1
     for (int step = 0; step < REFINE; ++step) {
-        double xm = __shfl_up(x[S - 1], 1, kSegP);  // level lo-1, from the lane below
-        double xp = __shfl_down(x[0], 1, kSegP);    // level lo+S, from the lane above
-        if (L.s == 0) xm = 0.0;
-        if (L.s == kSegP - 1) xp = 0.0;
+        const double xm = grp_prev(L.s, x[S - 1]);  // level lo-1, from the lane below
+        const double xp = grp_next(L.s, x[0]);      // level lo+S, from the lane above
 #pragma unroll
         for (int t = 0; t < S; ++t) {
             const double lo_x = t == 0 ? xm : x[t > 0 ? t - 1 : 0];
