@@ -126,10 +126,9 @@ typedef struct vx_tdma_system {
 
 typedef struct vx_tdma_plan vx_tdma_plan;
 
-/* mode 0: float64 Thomas (vx_tdma_f64), one persistent launch per nz bucket.
- * mode 1: float32 sweeps with one float64 refinement step (vx_tdma_mixed),
- * one launch per system (captured together by vx_tdma_plan_graph_launch).
- * Validation errors leave *out NULL. */
+/* mode 0: the float64 segmented solve of vx_tdma_f64; mode 1: the mixed
+ * solve of vx_tdma_mixed with one float64 refinement step. Either way one
+ * persistent launch per nz bucket. Validation errors leave *out NULL. */
 int vx_tdma_plan_create(const vx_tdma_system* systems, int32_t nsys, int32_t mode, vx_tdma_plan** out);
 /* Solve every system: one launch per non-empty nz bucket, on `stream`. */
 int vx_tdma_plan_run(vx_tdma_plan* plan, void* stream);
@@ -152,7 +151,14 @@ void vx_tdma_plan_destroy(vx_tdma_plan* plan);
 /* ------------------------------------------------------------------------
  * Isoneutral stencils. Arrays are full veros state arrays including the
  * 2-cell halos: X = nx + 4, Y = ny + 4, each C-contiguous.
- *   temp/salt/tracer/int_drhodX : (X, Y, nz, 3)  time level innermost
+ *   temp/salt/tracer/int_drhodX : (X, Y, nz, 3), time level innermost
+ *       (grid->time_layout = VX_LEVELS_INNER, the reference's layout,
+ *       veros/variables.py:338), or the three levels stored one after another
+ *       (VX_LEVELS_OUTER: level l of cell (i, j, k) at l*X*Y*nz +
+ *       (i*Y + j)*nz + k; the same arrays seen as the (X, Y, nz, 3) view with
+ *       strides (Y*nz, nz, 1, X*Y*nz)). One layout for every time-level
+ *       array of a call; the velocities u, v of the vertical friction follow
+ *       it too.
  *   K_iso, K_11, K_22, K_33, dtr_iso, P_diss : (X, Y, nz)
  *   Ai_ez, Ai_nz, Ai_bx, Ai_by   : (X, Y, nz, 2, 2)
  *   kbot : (X, Y) int32;  dxt, dxu : (X);  dyt, dyu, cost, cosu : (Y);
@@ -162,12 +168,15 @@ void vx_tdma_plan_destroy(vx_tdma_plan* plan);
  * requires the halo columns of kbot to be consistent (cyclic copy or 0).
  * ------------------------------------------------------------------------ */
 
+#define VX_LEVELS_INNER 0
+#define VX_LEVELS_OUTER 1
+
 typedef struct vx_grid {
     const int32_t* kbot;
     const double *dxt, *dxu, *dyt, *dyu, *cost, *cosu, *dzt, *dzw;
     int64_t nx, ny;
     int32_t nz;
-    int32_t _pad;
+    int32_t time_layout;  /* VX_LEVELS_INNER (0) or VX_LEVELS_OUTER (1) */
 } vx_grid;
 
 typedef struct vx_iso_params {

@@ -45,9 +45,9 @@ def _state(seed, nx, ny, nz, cyclic=True, land=0.3, tau=0, taup1=1):
     return W.add_friction_fields(rng, st, random_aux=True)
 
 
-def _dev(st):
+def _dev(st, levels="inner"):
     from veros_amd.state import to_device_state
-    return to_device_state(st)
+    return to_device_state(st, levels=levels)
 
 
 @pytest.mark.parametrize("shape,cyclic,land,levels", [((9, 7, 6), True, 0.3, (0, 1)), ((14, 11, 33), False, 0.3, (1, 2)),
@@ -66,14 +66,15 @@ def test_friction_random_against_oracle(shape, cyclic, land, levels):
 
 
 @pytest.mark.parametrize("path", golden_files("friction_"), ids=lambda p: p.stem)
-def test_friction_golden(path):
+@pytest.mark.parametrize("levels", ["inner", "outer"])
+def test_friction_golden(path, levels):
     """Against the reference's own outputs (veros.core.friction.implicit_vert_friction,
     written by tests/golden/make_golden.py), with the bar of _check."""
     rec = load_golden(path)
     st = column_state_from_golden(rec)
     ref = SimpleNamespace(**{k[4:]: v for k, v in rec.items() if k.startswith("out_")})
     from veros_amd import friction
-    out = friction.implicit_vert_friction(_dev(st))
+    out = friction.implicit_vert_friction(_dev(st, levels))
     _check(out, ref, st)
 
 

@@ -14,9 +14,9 @@ TOL = 1e-12
 PRE_KEYS = ("Ai_ez", "Ai_nz", "Ai_bx", "Ai_by", "K_11", "K_22", "K_33")
 
 
-def _dev_state(st):
+def _dev_state(st, levels="inner"):
     from veros_amd.state import to_device_state
-    return to_device_state(st)
+    return to_device_state(st, levels=levels)
 
 
 def _np(t):
@@ -24,10 +24,11 @@ def _np(t):
 
 
 @pytest.mark.parametrize("path", golden_files("iso_"), ids=lambda p: p.stem)
-def test_pre_golden(path):
+@pytest.mark.parametrize("levels", ["inner", "outer"])
+def test_pre_golden(path, levels):
     from veros_amd import isoneutral
     g = load_golden(path)
-    ds = _dev_state(state_from_golden(g))
+    ds = _dev_state(state_from_golden(g), levels)
     out = isoneutral.isoneutral_diffusion_pre(ds)
     for k in PRE_KEYS:
         assert rel_err(_np(getattr(out, k)), g["pre_" + k]) <= TOL, k
@@ -36,13 +37,14 @@ def test_pre_golden(path):
 @pytest.mark.parametrize("path", golden_files("iso_"), ids=lambda p: p.stem)
 @pytest.mark.parametrize("trn", ["temp", "salt"])
 @pytest.mark.parametrize("iso", [True, False], ids=["iso", "skew"])
-def test_diffusion_golden(path, trn, iso):
+@pytest.mark.parametrize("levels", ["inner", "outer"])
+def test_diffusion_golden(path, trn, iso, levels):
     from veros_amd import isoneutral
     g = load_golden(path)
     st = state_from_golden(g)
     for k in PRE_KEYS:
         setattr(st.variables, k, g["pre_" + k])
-    ds = _dev_state(st)
+    ds = _dev_state(st, levels)
     istemp = trn == "temp"
     out = isoneutral.isoneutral_diffusion_kernel(ds, getattr(ds.variables, trn), istemp, iso=iso)
     tag = ("iso_" if iso else "skew_") + trn
@@ -53,7 +55,8 @@ def test_diffusion_golden(path, trn, iso):
 
 
 @pytest.mark.parametrize("path", golden_files("iso_"), ids=lambda p: p.stem)
-def test_pre_and_temp_salt_fused_golden(path):
+@pytest.mark.parametrize("levels", ["inner", "outer"])
+def test_pre_and_temp_salt_fused_golden(path, levels):
     """The one call (pre, then temperature, then salinity) against the
     reference's recorded outputs: pre's tensor; the temperature and salinity
     steps, each recorded from the same inputs with that tensor (the salinity
@@ -61,7 +64,7 @@ def test_pre_and_temp_salt_fused_golden(path):
     P_diss_iso as the input plus both recorded increments."""
     from veros_amd import isoneutral
     g = load_golden(path)
-    ds = _dev_state(state_from_golden(g))
+    ds = _dev_state(state_from_golden(g), levels)
     isoneutral.isoneutral_pre_and_diffusion_temp_salt(ds)
     vs = ds.variables
     for k in PRE_KEYS:

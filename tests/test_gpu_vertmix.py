@@ -41,9 +41,9 @@ def _state(seed, nx, ny, nz, cyclic=True, land=0.3, kmin=1):
     return W.add_vertmix_fields(rng, st, random_aux=True)
 
 
-def _dev(st):
+def _dev(st, levels="inner"):
     from veros_amd.state import to_device_state
-    return to_device_state(st)
+    return to_device_state(st, levels=levels)
 
 
 def _np(t):
@@ -66,14 +66,15 @@ def test_vertmix_random_against_oracle(shape, cyclic, land):
 
 
 @pytest.mark.parametrize("path", golden_files("vertmix_"), ids=lambda p: p.stem)
-def test_vertmix_golden(path):
+@pytest.mark.parametrize("levels", ["inner", "outer"])
+def test_vertmix_golden(path, levels):
     """Against the reference's own outputs (veros.core.thermodynamics.vertmix_tempsalt,
     written by tests/golden/make_golden.py), with the bar of _check."""
     rec = load_golden(path)
     st = column_state_from_golden(rec)
     ref = SimpleNamespace(**{k[4:]: v for k, v in rec.items() if k.startswith("out_")})
     from veros_amd import thermodynamics
-    out = thermodynamics.vertmix_tempsalt(_dev(st))
+    out = thermodynamics.vertmix_tempsalt(_dev(st, levels))
     _check(out, ref, st)
 
 

@@ -15,6 +15,8 @@ VX_OK = 0
 VX_ERR_INVALID_ARGUMENT = 1
 VX_ERR_UNSUPPORTED = 2
 VX_ERR_LAUNCH = 3
+VX_LEVELS_INNER = 0
+VX_LEVELS_OUTER = 1
 
 c_void_p = ctypes.c_void_p
 c_i32 = ctypes.c_int32
@@ -27,7 +29,7 @@ class VxGrid(ctypes.Structure):
         ("kbot", c_void_p),
         ("dxt", c_void_p), ("dxu", c_void_p), ("dyt", c_void_p), ("dyu", c_void_p),
         ("cost", c_void_p), ("cosu", c_void_p), ("dzt", c_void_p), ("dzw", c_void_p),
-        ("nx", c_i64), ("ny", c_i64), ("nz", c_i32), ("_pad", c_i32),
+        ("nx", c_i64), ("ny", c_i64), ("nz", c_i32), ("time_layout", c_i32),
     ]
 
 

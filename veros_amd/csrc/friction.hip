@@ -32,6 +32,9 @@ struct FricArgs {
     int64_t X, Y;
     int32_t nz, tau, taup1;
     double dt;
+    // time levels of u, v: level l of cell c at l * ls + c * cs (cs = 3,
+    // ls = 1: levels innermost; cs = 1, ls = X*Y*nz: levels outermost)
+    int64_t cs, ls;
 };
 
 constexpr int kFricBlock = 256;  // 128 face columns x 2 components
@@ -74,8 +77,8 @@ __global__ __launch_bounds__(kFricBlock) void friction_solve_kernel(FricArgs p) 
     const int64_t base = col * nz;
     const double* __restrict__ ka = p.kappaM + base;
     const double* __restrict__ kn = p.kappaM + nb * nz;
-    double* __restrict__ ut = p.vel[t] + p.tau;     // level tau of cell c at ut[3 c]
-    double* __restrict__ u1 = p.vel[t] + p.taup1;   // level taup1
+    double* __restrict__ ut = p.vel[t] + p.tau * p.ls;    // level tau of cell c at ut[cs c]
+    double* __restrict__ u1 = p.vel[t] + p.taup1 * p.ls;  // level taup1
     double* __restrict__ dm = p.dmix[t] + base;
 
     // forward elimination with the rows formed on the fly
@@ -92,7 +95,7 @@ __global__ __launch_bounds__(kFricBlock) void friction_solve_kernel(FricArgs p) 
                 const int kk = q + u < nz ? q + u : nz - 1;
                 va[u] = ka[kk];
                 vn[u] = kn[kk];
-                vt[u] = ut[3 * (base + kk)];
+                vt[u] = ut[p.cs * (base + kk)];
             }
         }
 #pragma unroll
@@ -125,7 +128,7 @@ __global__ __launch_bounds__(kFricBlock) void friction_solve_kernel(FricArgs p) 
             double vo[kFricChunk], vw[kFricChunk];
 #pragma unroll
             for (int u = 0; u < kFricChunk; ++u) {
-                const int64_t c3 = 3 * (base + (q + u < nz ? q + u : nz - 1));
+                const int64_t c3 = p.cs * (base + (q + u < nz ? q + u : nz - 1));
                 vo[u] = ut[c3];
                 vw[u] = q + u < k0 ? u1[c3] : 0.0;
             }
@@ -137,7 +140,7 @@ __global__ __launch_bounds__(kFricBlock) void friction_solve_kernel(FricArgs p) 
                     if (k >= k0) {
                         x = dp[k] - cp[k] * xn;
                         xn = x;
-                        u1[3 * (base + k)] = x;
+                        u1[p.cs * (base + k)] = x;
                     }
                     dm[k] = (x - vo[u]) / dt;
                 }
@@ -158,10 +161,11 @@ __device__ inline double face_diss(const FricArgs& p, int t, int64_t i, int64_t 
     const double m = (f.both && k >= f.k0) ? 1.0 : 0.0;  // mask[k] * mask[k+1]
     const double fxa = 0.5 * (p.kappaM[col * p.nz + k] + p.kappaM[nb * p.nz + k]);
     const double* v = p.vel[t];
-    const int64_t c0 = 3 * (col * p.nz + k), c1 = c0 + 3;
+    const int64_t c0 = p.cs * (col * p.nz + k), c1 = c0 + p.cs;
+    const int64_t l0 = p.tau * p.ls, l1 = p.taup1 * p.ls;
     const double dzw = p.dzw[k];
-    const double flux = fxa * (v[c1 + p.taup1] - v[c0 + p.taup1]) / dzw * m * m;
-    return (v[c1 + p.tau] - v[c0 + p.tau]) * flux / dzw;
+    const double flux = fxa * (v[c1 + l1] - v[c0 + l1]) / dzw * m * m;
+    return (v[c1 + l0] - v[c0 + l0]) * flux / dzw;
 }
 
 // K_diss_v += ugrid_to_tgrid(diss_u), then += vgrid_to_tgrid(diss_v)
@@ -244,6 +248,11 @@ extern "C" int vx_implicit_vert_friction_f64(void* stream, const vx_grid* grid, 
     p.tau = tau;
     p.taup1 = taup1;
     p.dt = dt_mom;
+    if (grid->time_layout != VX_LEVELS_INNER && grid->time_layout != VX_LEVELS_OUTER)
+        return fail(VX_ERR_INVALID_ARGUMENT, "%s: time_layout must be VX_LEVELS_INNER or VX_LEVELS_OUTER", who);
+    const bool outer = grid->time_layout == VX_LEVELS_OUTER;
+    p.cs = outer ? 1 : 3;
+    p.ls = outer ? p.X * p.Y * p.nz : 1;
     hipStream_t s = (hipStream_t)stream;
     const int nzb = ((p.nz + 15) / 16) * 16;
     switch (nzb) {

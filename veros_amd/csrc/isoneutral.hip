@@ -35,6 +35,10 @@ struct Grid {
     const int32_t* kbot;
     const double *dxt, *dxu, *dyt, *dyu, *cost, *cosu, *dzt, *dzw;
     int32_t X, Y, nz;
+    // time-level arrays (temp, salt, tracer, int_drhodX): level l of cell c at
+    // l * ls + c * cs; cs = 3, ls = 1 for the reference's (X, Y, nz, 3)
+    // layout, cs = 1, ls = X*Y*nz for levels stored one after another
+    uint32_t cs, ls;
 };
 
 struct Prm {
@@ -48,9 +52,9 @@ __device__ __forceinline__ double mT(int kb, int k) { return (kb > 0 && kb - 1 <
 // Level k of a column with bottom index kb is a wet T cell (maskT = 1).
 __device__ __forceinline__ bool wet(int kb, int k) { return kb > 0 && kb - 1 <= k; }
 
-// Tracer value at time level `lev` of an (X, Y, nz, 3) array.
+// Tracer value at time level `lev` of a time-level array.
 __device__ __forceinline__ double tr3(const double* __restrict__ f, const Grid& g, int i, int j, int k, int lev) {
-    return f[idx3(g, i, j, k) * 3 + lev];
+    return f[(uint32_t)idx3(g, i, j, k) * g.cs + (uint32_t)lev * g.ls];
 }
 
 // Masked face gradients (veros/core/isoneutral/isoneutral.py:46-95).
@@ -348,15 +352,15 @@ __device__ __forceinline__ void pre_wet_cell(const Grid& g, const Prm& p, const 
     const uint32_t c = (uint32_t)i * si + (uint32_t)j * sj + (uint32_t)k;
     const uint32_t ce = c + si, cw = c - si, cn = c + sj, cs = c - sj;
     const bool hp = k < nz - 1, hm = k >= 1;
-    const double* __restrict__ T = f.T + f.tau;
-    const double* __restrict__ S = f.S + f.tau;
+    const double* __restrict__ T = f.T + (uint32_t)f.tau * g.ls;
+    const double* __restrict__ S = f.S + (uint32_t)f.tau * g.ls;
     const int kb_c = g.kbot[i * g.Y + j], kb_e = g.kbot[(i + 1) * g.Y + j], kb_w = g.kbot[(i - 1) * g.Y + j];
     const int kb_n = g.kbot[i * g.Y + j + 1], kb_s = g.kbot[i * g.Y + j - 1];
 
     // tracer values: c(enter), e(ast), w(est), n(orth), s(outh); p = k+1, m = k-1
     auto L2 = [&](uint32_t o, bool ok, double& t, double& sv) {
-        t = ok ? T[3u * o] : 0.0;
-        sv = ok ? S[3u * o] : 0.0;
+        t = ok ? T[g.cs * o] : 0.0;
+        sv = ok ? S[g.cs * o] : 0.0;
     };
     double Tc, Sc, Tcp, Scp, Tcm, Scm, Te, Se, Tep, Sep, Tem, Sem, Tw, Sw, Twp, Swp;
     double Tn, Sn, Tnp, Snp, Tnm, Snm, Ts, Ss, Tsp, Ssp;
@@ -782,26 +786,26 @@ __global__ __launch_bounds__(256) void iso_flux_kernel(Grid g, StepArgs a, int n
         double Tw[NTR], Twu[NTR], Ts[NTR], Tsu[NTR];
 #pragma unroll
         for (int r = 0; r < NTR; ++r) {
-            const double* __restrict__ T = a.t[r].tr + a.tau;
-            Tc[r] = T[3u * c];
-            Tcu[r] = T[3u * (c + up)];
-            Tcd[r] = T[3u * (c - dn)];
+            const double* __restrict__ T = a.t[r].tr + (uint32_t)a.tau * g.ls;
+            Tc[r] = T[g.cs * c];
+            Tcu[r] = T[g.cs * (c + up)];
+            Tcd[r] = T[g.cs * (c - dn)];
             Te[r] = Teu[r] = Ted[r] = Tn[r] = Tnu[r] = Tnd[r] = Tw[r] = Twu[r] = Ts[r] = Tsu[r] = 0.0;
             if (need_e || top) {
-                Te[r] = T[3u * (c + si)];
-                Teu[r] = T[3u * (c + si + up)];
+                Te[r] = T[g.cs * (c + si)];
+                Teu[r] = T[g.cs * (c + si + up)];
             }
-            if (need_e) Ted[r] = T[3u * (c + si - dn)];
+            if (need_e) Ted[r] = T[g.cs * (c + si - dn)];
             if (need_n || top) {
-                Tn[r] = T[3u * (c + sj)];
-                Tnu[r] = T[3u * (c + sj + up)];
+                Tn[r] = T[g.cs * (c + sj)];
+                Tnu[r] = T[g.cs * (c + sj + up)];
             }
-            if (need_n) Tnd[r] = T[3u * (c + sj - dn)];
+            if (need_n) Tnd[r] = T[g.cs * (c + sj - dn)];
             if (top) {
-                Tw[r] = T[3u * (c - si)];
-                Twu[r] = T[3u * (c - si + 1)];
-                Ts[r] = T[3u * (c - sj)];
-                Tsu[r] = T[3u * (c - sj + 1)];
+                Tw[r] = T[g.cs * (c - si)];
+                Twu[r] = T[g.cs * (c - si + 1)];
+                Ts[r] = T[g.cs * (c - sj)];
+                Tsu[r] = T[g.cs * (c - sj + 1)];
             }
         }
         // flux_east (:22-47), flux_north (:49-77), flux_top (:79-111)
@@ -1078,20 +1082,20 @@ __global__ __launch_bounds__(kWetThreads) __attribute__((amdgpu_waves_per_eu(VX_
                 fn_c[t] = fn_at(g, ta, i, j, c);
                 fn_s[t] = south_ok ? fn_at(g, ta, i, j - 1, c - st.sj) : 0.0;
                 if (interior) {
-                    trp[t] = ta.tr[3u * c + a.taup1];
+                    trp[t] = ta.tr[g.cs * c + (uint32_t)a.taup1 * g.ls];
                     dtr0[t] = ta.dtr[c];
                     ft_c[t] = ta.ft[c];
                     // the top face of the dry cell below the lowest wet cell
                     if (!WETFLUX && k == kb - 1 && k >= 1) ft_b[t] = ta.ft[c - 1];
                 }
                 if (energy) {
-                    const double* __restrict__ I = ta.I + a.tau;
-                    Ic[t] = I[3u * c];
-                    Ie[t] = I[3u * (c + st.si)];
-                    Iw[t] = I[3u * (c - st.si)];
-                    In[t] = I[3u * (c + st.sj)];
-                    Is[t] = I[3u * (c - st.sj)];
-                    if (interior && k < nz - 1) Iu[t] = I[3u * (c + 1)];
+                    const double* __restrict__ I = ta.I + (uint32_t)a.tau * g.ls;
+                    Ic[t] = I[g.cs * c];
+                    Ie[t] = I[g.cs * (c + st.si)];
+                    Iw[t] = I[g.cs * (c - st.si)];
+                    In[t] = I[g.cs * (c + st.sj)];
+                    Is[t] = I[g.cs * (c - st.sj)];
+                    if (interior && k < nz - 1) Iu[t] = I[g.cs * (c + 1)];
                 }
             }
         }
@@ -1195,7 +1199,7 @@ __global__ __launch_bounds__(kWetThreads) __attribute__((amdgpu_waves_per_eu(VX_
                 const double* sTt = sT + t * 4 * T;
                 if (interior) {
                     const double newk = sTt[2 * T + lo];
-                    a.t[t].tr[3u * c + a.taup1] = newk;
+                    a.t[t].tr[g.cs * c + (uint32_t)a.taup1 * g.ls] = newk;
                     a.t[t].dtr[c] = (dtr0[t] + dtrk[t]) + (newk - rhs[t]) / dt;
                 }
                 if (energy) {
@@ -1349,7 +1353,7 @@ __global__ __launch_bounds__(256) void iso_adj_scatter_kernel(Grid g, AdjArgs a)
     const int nz = g.nz;
     const uint32_t si = (uint32_t)g.Y * (uint32_t)nz, sj = (uint32_t)nz;
     const uint32_t c = (uint32_t)i * si + (uint32_t)j * sj + (uint32_t)k;
-    const double* __restrict__ tr = a.tr + a.tau;
+    const double* __restrict__ tr = a.tr + (uint32_t)a.tau * g.ls;
     const bool hm = k >= 1;
     const uint32_t up = (k + 1 < nz) ? 1u : 0u, dn = hm ? 1u : 0u;
     const double* K = a.Kiso;
@@ -1361,8 +1365,8 @@ __global__ __launch_bounds__(256) void iso_adj_scatter_kernel(Grid g, AdjArgs a)
             const double diffloc = hm ? 0.25 * (K[c] + K[c - 1] + K[ce] + K[ce - 1]) : 0.5 * (K[c] + K[ce]);
             const double* A = a.Aez + 4u * c;  // (ip, kr) at 2 ip + kr
             const double q4 = 1.0 / (4.0 * g.dzt[k]);
-            const double t0 = tr[3u * c], t0u = tr[3u * (c + up)], t0d = tr[3u * (c - dn)];
-            const double t1 = tr[3u * ce], t1u = tr[3u * (ce + up)], t1d = tr[3u * (ce - dn)];
+            const double t0 = tr[g.cs * c], t0u = tr[g.cs * (c + up)], t0d = tr[g.cs * (c - dn)];
+            const double t1 = tr[g.cs * ce], t1u = tr[g.cs * (ce + up)], t1d = tr[g.cs * (ce - dn)];
             // diffloc terms
             const double w00 = mu * diffloc * A[0] * q4, w01 = mu * diffloc * A[1] * q4;
             const double w10 = mu * diffloc * A[2] * q4, w11 = mu * diffloc * A[3] * q4;
@@ -1394,8 +1398,8 @@ __global__ __launch_bounds__(256) void iso_adj_scatter_kernel(Grid g, AdjArgs a)
             const double diffloc = hm ? 0.25 * (K[c] + K[c - 1] + K[cn] + K[cn - 1]) : 0.5 * (K[c] + K[cn]);
             const double* A = a.Anz + 4u * c;
             const double q4 = 1.0 / (4.0 * g.dzt[k]);
-            const double t0 = tr[3u * c], t0u = tr[3u * (c + up)], t0d = tr[3u * (c - dn)];
-            const double t1 = tr[3u * cn], t1u = tr[3u * (cn + up)], t1d = tr[3u * (cn - dn)];
+            const double t0 = tr[g.cs * c], t0u = tr[g.cs * (c + up)], t0d = tr[g.cs * (c - dn)];
+            const double t1 = tr[g.cs * cn], t1u = tr[g.cs * (cn + up)], t1d = tr[g.cs * (cn - dn)];
             const double w00 = mu * diffloc * A[0] * q4, w01 = mu * diffloc * A[1] * q4;
             const double w10 = mu * diffloc * A[2] * q4, w11 = mu * diffloc * A[3] * q4;
             adj_add(a.g_tau + c, w00 - w01);
@@ -1433,8 +1437,8 @@ __global__ __launch_bounds__(256) void iso_adj_scatter_kernel(Grid g, AdjArgs a)
                 const double bx0 = B[kr] * fx, bx1 = B[2 + kr] * fx;
                 const double cy0 = C[kr] * g.cosu[j - 1] * fy, cy1 = C[2 + kr] * g.cosu[j] * fy;
                 const uint32_t ck = c + kr;
-                S += bx0 * (tr[3u * ck] - tr[3u * (cw + kr)]) + bx1 * (tr[3u * (ce + kr)] - tr[3u * ck]) +
-                     cy0 * (tr[3u * ck] - tr[3u * (cs + kr)]) + cy1 * (tr[3u * (cn + kr)] - tr[3u * ck]);
+                S += bx0 * (tr[g.cs * ck] - tr[g.cs * (cw + kr)]) + bx1 * (tr[g.cs * (ce + kr)] - tr[g.cs * ck]) +
+                     cy0 * (tr[g.cs * ck] - tr[g.cs * (cs + kr)]) + cy1 * (tr[g.cs * (cn + kr)] - tr[g.cs * ck]);
                 const double m = mu * K2;
                 adj_add(a.g_tau + ck, m * (bx0 - bx1 + cy0 - cy1));
                 adj_add(a.g_tau + cw + kr, -m * bx0);
@@ -1473,6 +1477,14 @@ int make_grid(const vx_grid* in, Grid& g, const char* who) {
     g.dzw = in->dzw;
     g.X = (int32_t)(in->nx + 4);
     g.Y = (int32_t)(in->ny + 4);
+    if (in->time_layout != VX_LEVELS_INNER && in->time_layout != VX_LEVELS_OUTER)
+        return fail(VX_ERR_INVALID_ARGUMENT, "%s: time_layout must be VX_LEVELS_INNER or VX_LEVELS_OUTER (got %d)",
+                    who, in->time_layout);
+    if ((in->nx + 4) * (in->ny + 4) * (int64_t)in->nz * 3 >= ((int64_t)1 << 32))
+        return fail(VX_ERR_UNSUPPORTED, "%s: grid too large for 32-bit offsets", who);
+    const bool outer = in->time_layout == VX_LEVELS_OUTER;
+    g.cs = outer ? 1u : 3u;
+    g.ls = outer ? (uint32_t)((in->nx + 4) * (in->ny + 4) * (int64_t)in->nz) : 1u;
     g.nz = in->nz;
     return VX_OK;
 }

@@ -29,6 +29,9 @@ struct VmixArgs {
     int32_t nz, taup1;
     double dt;
     bool vec_dv;  // nz even and both tendency arrays 16-byte aligned
+    // time levels of temp/salt: level l of cell c at l * ls + c * cs (cs = 3,
+    // ls = 1: levels innermost; cs = 1, ls = X*Y*nz: levels outermost)
+    int64_t cs, ls;
 };
 
 constexpr int kVmixBlock = 256;  // 128 columns x 2 tracers
@@ -52,7 +55,7 @@ __global__ __launch_bounds__(kVmixBlock) void vertmix_kernel(VmixArgs p) {
     const int kb = interior ? p.kbot[col] : 0;
     const int k0 = (kb >= 1 && kb <= nz) ? kb - 1 : nz;  // first wet level (nz: none)
     const double dt = p.dt;
-    double* __restrict__ tr = p.tr[t] + p.taup1;         // level taup1 of cell c at tr[3 c]
+    double* __restrict__ tr = p.tr[t] + p.taup1 * p.ls;  // level taup1 of cell c at tr[cs c]
     const double* __restrict__ kap = p.kappaH + base;
     const bool vec = p.vec_dv;
 
@@ -72,7 +75,7 @@ __global__ __launch_bounds__(kVmixBlock) void vertmix_kernel(VmixArgs p) {
             for (int u = 0; u < kVmixChunk; ++u) {
                 const int kk = q + u < nz ? q + u : nz - 1;
                 vk[u] = kap[kk];
-                vt[u] = tr[3 * (base + kk)];
+                vt[u] = tr[p.cs * (base + kk)];
             }
         }
 #pragma unroll
@@ -106,7 +109,7 @@ __global__ __launch_bounds__(kVmixBlock) void vertmix_kernel(VmixArgs p) {
             const bool live = q + kVmixChunk > k0;
             if (live) {
 #pragma unroll
-                for (int u = 0; u < kVmixChunk; ++u) vo[u] = tr[3 * (base + (q + u < nz ? q + u : nz - 1))];
+                for (int u = 0; u < kVmixChunk; ++u) vo[u] = tr[p.cs * (base + (q + u < nz ? q + u : nz - 1))];
             }
             double dvv[kVmixChunk];
 #pragma unroll
@@ -116,7 +119,7 @@ __global__ __launch_bounds__(kVmixBlock) void vertmix_kernel(VmixArgs p) {
                 if (k < nz && live && k >= k0) {
                     const double x = dp[k] - cp[k] * xn;
                     xn = x;
-                    tr[3 * (base + k)] = x;
+                    tr[p.cs * (base + k)] = x;
                     dvv[u] = (x - vo[u]) / dt;
                 }
             }
@@ -150,8 +153,8 @@ __global__ __launch_bounds__(256) void vertmix_cyclic_kernel(VmixArgs p) {
     const int64_t off = r - (int64_t)h * plane;
     const int64_t dst = h < 2 ? h : p.X - 4 + h;
     const int64_t src = h < 2 ? p.X - 4 + h : 2 + (h - 2);
-    double* __restrict__ tr = p.tr[t] + p.taup1;
-    tr[3 * (dst * plane + off)] = tr[3 * (src * plane + off)];
+    double* __restrict__ tr = p.tr[t] + p.taup1 * p.ls;
+    tr[p.cs * (dst * plane + off)] = tr[p.cs * (src * plane + off)];
 }
 
 template <int NZB>
@@ -203,6 +206,11 @@ extern "C" int vx_vertmix_tempsalt_f64(void* stream, const vx_grid* grid, double
     p.taup1 = taup1;
     p.dt = dt_tracer;
     p.vec_dv = !(p.nz & 1) && aligned16(dtemp_vmix) && aligned16(dsalt_vmix);
+    if (grid->time_layout != VX_LEVELS_INNER && grid->time_layout != VX_LEVELS_OUTER)
+        return fail(VX_ERR_INVALID_ARGUMENT, "%s: time_layout must be VX_LEVELS_INNER or VX_LEVELS_OUTER", who);
+    const bool outer = grid->time_layout == VX_LEVELS_OUTER;
+    p.cs = outer ? 1 : 3;
+    p.ls = outer ? p.X * p.Y * p.nz : 1;
     hipStream_t s = (hipStream_t)stream;
     const int nzb = ((p.nz + 15) / 16) * 16;
     switch (nzb) {

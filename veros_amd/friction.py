@@ -19,7 +19,8 @@ import torch
 
 from . import _native
 from ._arrays import ptr, stream_handle
-from .isoneutral import KernelOutput, _check_array
+from .isoneutral import KernelOutput, _check_array, _check_levels
+from .layout import common_layout
 
 
 def implicit_vert_friction(state):
@@ -30,8 +31,8 @@ def implicit_vert_friction(state):
     dzw = _check_array("dzw", vs.dzw, (nz,))
     dxt = _check_array("dxt", vs.dxt, (X,))
     dxu = _check_array("dxu", vs.dxu, (X,))
-    u = _check_array("u", vs.u, (X, Y, nz, 3))
-    v = _check_array("v", vs.v, (X, Y, nz, 3))
+    u = _check_levels("u", vs.u, (X, Y, nz, 3))
+    v = _check_levels("v", vs.v, (X, Y, nz, 3))
     K = _check_array("kappaM", vs.kappaM, (X, Y, nz))
     at = _check_array("area_t", vs.area_t, (X, Y))
     av = _check_array("area_v", vs.area_v, (X, Y))
@@ -41,6 +42,7 @@ def implicit_vert_friction(state):
     g = _native.VxGrid()
     g.kbot, g.dzt, g.dzw, g.dxt, g.dxu = ptr(kbot), ptr(dzt), ptr(dzw), ptr(dxt), ptr(dxu)
     g.nx, g.ny, g.nz = X - 4, Y - 4, nz
+    g.time_layout = common_layout(u=u, v=v)
     rc = _native.lib().vx_implicit_vert_friction_f64(
         stream_handle(), ctypes.byref(g), float(st.dt_mom), ptr(u), ptr(v), int(vs.tau), int(vs.taup1), ptr(K),
         ptr(at), ptr(av), ptr(du), ptr(dv), ptr(kd))

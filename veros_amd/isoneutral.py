@@ -36,6 +36,7 @@ import torch
 
 from . import _native
 from ._arrays import ptr, require_contiguous, stream_handle, workspace
+from .layout import LEVELS_INNER, common_layout, levels_layout
 
 # veros/core/density/linear_eq.py:10-15
 LINEAR_EOS_RHO0 = 1024.0
@@ -67,8 +68,22 @@ def _check_array(name, t, shape, dtype=torch.float64):
     return t
 
 
-def grid_struct(vs, keep):
-    """vx_grid for the state's grid; ``keep`` collects the tensors it points into."""
+def _check_levels(name, t, shape):
+    """A time-level array: (X, Y, nz, 3) float64 on the GPU with the levels
+    innermost or outermost (veros_amd.layout)."""
+    if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
+        raise TypeError(f"{name} must be a torch tensor on the GPU")
+    if t.dtype != torch.float64:
+        raise TypeError(f"{name} must be {torch.float64} (got {t.dtype})")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"unexpected shape for {name} (expected {tuple(shape)}, got {tuple(t.shape)})")
+    levels_layout(t)
+    return t
+
+
+def grid_struct(vs, keep, layout=LEVELS_INNER):
+    """vx_grid for the state's grid (time-level arrays in ``layout``); ``keep``
+    collects the tensors it points into."""
     kbot = vs.kbot
     X, Y = kbot.shape
     nz = vs.K_iso.shape[2]
@@ -83,6 +98,7 @@ def grid_struct(vs, keep):
         keep.append(t)
     keep.append(kb)
     g.nx, g.ny, g.nz = X - 4, Y - 4, nz
+    g.time_layout = int(layout)
     return g
 
 
@@ -103,10 +119,10 @@ def isoneutral_diffusion_pre(state):
     vs, st = state.variables, state.settings
     X, Y, nz = vs.K_iso.shape
     keep = []
-    g = grid_struct(vs, keep)
+    T = _check_levels("temp", vs.temp, (X, Y, nz, 3))
+    S = _check_levels("salt", vs.salt, (X, Y, nz, 3))
+    g = grid_struct(vs, keep, common_layout(temp=T, salt=S))
     p = params_struct(st)
-    T = _check_array("temp", vs.temp, (X, Y, nz, 3))
-    S = _check_array("salt", vs.salt, (X, Y, nz, 3))
     K = _check_array("K_iso", vs.K_iso, (X, Y, nz))
     outs = {}
     for name in ("Ai_ez", "Ai_nz", "Ai_bx", "Ai_by"):
@@ -127,9 +143,8 @@ def isoneutral_diffusion_kernel(state, tr, istemp, iso=True):
     vs, st = state.variables, state.settings
     X, Y, nz = vs.K_iso.shape if iso else vs.K_gm.shape
     keep = []
-    g = grid_struct(vs, keep)
     p = params_struct(st)
-    tr = _check_array("tr", tr, (X, Y, nz, 3))
+    tr = _check_levels("tr", tr, (X, Y, nz, 3))
     dname = "dtemp_iso" if istemp else "dsalt_iso"
     dtr = _check_array(dname, getattr(vs, dname), (X, Y, nz))
     Kname = "K_iso" if iso else "K_gm"
@@ -141,11 +156,12 @@ def isoneutral_diffusion_kernel(state, tr, istemp, iso=True):
     energy = bool(st.enable_conserve_energy)
     pname = "P_diss_iso" if iso else "P_diss_skew"
     if energy:
-        I = _check_array("int_drhodT" if istemp else "int_drhodS",
-                         vs.int_drhodT if istemp else vs.int_drhodS, (X, Y, nz, 3))
+        I = _check_levels("int_drhodT" if istemp else "int_drhodS",
+                          vs.int_drhodT if istemp else vs.int_drhodS, (X, Y, nz, 3))
         P = _check_array(pname, getattr(vs, pname), (X, Y, nz))
     else:
         I = P = None
+    g = grid_struct(vs, keep, common_layout(tr=tr, int_drhodX=I))
     L = _native.lib()
     ws = workspace(L.vx_iso_workspace_bytes(g, 1), tr.device)
     fn = L.vx_iso_diffusion_f64 if iso else L.vx_iso_skew_diffusion_f64
@@ -177,21 +193,21 @@ def isoneutral_diffusion_temp_salt(state):
     vs, st = state.variables, state.settings
     X, Y, nz = vs.K_iso.shape
     keep = []
-    g = grid_struct(vs, keep)
     p = params_struct(st)
-    T = _check_array("temp", vs.temp, (X, Y, nz, 3))
-    S = _check_array("salt", vs.salt, (X, Y, nz, 3))
+    T = _check_levels("temp", vs.temp, (X, Y, nz, 3))
+    S = _check_levels("salt", vs.salt, (X, Y, nz, 3))
     arrays = [_check_array("K_iso", vs.K_iso, (X, Y, nz))]
     arrays += [_check_array(n, getattr(vs, n), (X, Y, nz, 2, 2)) for n in ("Ai_ez", "Ai_nz", "Ai_bx", "Ai_by")]
     arrays += [_check_array(n, getattr(vs, n), (X, Y, nz)) for n in ("K_11", "K_22", "K_33")]
     dT = _check_array("dtemp_iso", vs.dtemp_iso, (X, Y, nz))
     dS = _check_array("dsalt_iso", vs.dsalt_iso, (X, Y, nz))
     if st.enable_conserve_energy:
-        IT = _check_array("int_drhodT", vs.int_drhodT, (X, Y, nz, 3))
-        IS = _check_array("int_drhodS", vs.int_drhodS, (X, Y, nz, 3))
+        IT = _check_levels("int_drhodT", vs.int_drhodT, (X, Y, nz, 3))
+        IS = _check_levels("int_drhodS", vs.int_drhodS, (X, Y, nz, 3))
         P = _check_array("P_diss_iso", vs.P_diss_iso, (X, Y, nz))
     else:
         IT = IS = P = None
+    g = grid_struct(vs, keep, common_layout(temp=T, salt=S, int_drhodT=IT, int_drhodS=IS))
     L = _native.lib()
     ws = workspace(L.vx_iso_workspace_bytes(g, 2), T.device)
     rc = L.vx_iso_diffusion_ts_f64(stream_handle(), g, p, ptr(T), ptr(S), int(vs.tau), int(vs.taup1),
@@ -214,21 +230,21 @@ def isoneutral_pre_and_diffusion_temp_salt(state):
     vs, st = state.variables, state.settings
     X, Y, nz = vs.K_iso.shape
     keep = []
-    g = grid_struct(vs, keep)
     p = params_struct(st)
-    T = _check_array("temp", vs.temp, (X, Y, nz, 3))
-    S = _check_array("salt", vs.salt, (X, Y, nz, 3))
+    T = _check_levels("temp", vs.temp, (X, Y, nz, 3))
+    S = _check_levels("salt", vs.salt, (X, Y, nz, 3))
     K = _check_array("K_iso", vs.K_iso, (X, Y, nz))
     tens = [_check_array(n, getattr(vs, n), (X, Y, nz, 2, 2)) for n in ("Ai_ez", "Ai_nz", "Ai_bx", "Ai_by")]
     tens += [_check_array(n, getattr(vs, n), (X, Y, nz)) for n in ("K_11", "K_22", "K_33")]
     dT = _check_array("dtemp_iso", vs.dtemp_iso, (X, Y, nz))
     dS = _check_array("dsalt_iso", vs.dsalt_iso, (X, Y, nz))
     if st.enable_conserve_energy:
-        IT = _check_array("int_drhodT", vs.int_drhodT, (X, Y, nz, 3))
-        IS = _check_array("int_drhodS", vs.int_drhodS, (X, Y, nz, 3))
+        IT = _check_levels("int_drhodT", vs.int_drhodT, (X, Y, nz, 3))
+        IS = _check_levels("int_drhodS", vs.int_drhodS, (X, Y, nz, 3))
         P = _check_array("P_diss_iso", vs.P_diss_iso, (X, Y, nz))
     else:
         IT = IS = P = None
+    g = grid_struct(vs, keep, common_layout(temp=T, salt=S, int_drhodT=IT, int_drhodS=IS))
     L = _native.lib()
     ws = workspace(L.vx_iso_pre_diffusion_ts_workspace_bytes(g), T.device)
     rc = L.vx_iso_pre_diffusion_ts_f64(stream_handle(), g, p, ptr(T), ptr(S), int(vs.tau), int(vs.taup1), ptr(K),

@@ -10,14 +10,19 @@ import numpy as np
 import torch
 
 from ._arrays import device
+from .layout import LEVELS_INNER, LEVELS_OUTER, set_state_layout
 
 INT_VARS = ("kbot",)
 
 
-def to_device_state(state, dev=None):
+def to_device_state(state, dev=None, levels=LEVELS_INNER):
     """Copy a host state (``.variables`` with NumPy arrays) to the GPU.
     Float arrays become float64, kbot int32, boolean masks stay boolean;
-    scalars (tau, taup1, ...) are kept as Python values."""
+    scalars (tau, taup1, ...) are kept as Python values. ``levels`` selects
+    the storage of the time-level variables (veros_amd.layout): the
+    reference's ``LEVELS_INNER`` or ``LEVELS_OUTER`` ("inner"/"outer" also
+    accepted)."""
+    levels = {"inner": LEVELS_INNER, "outer": LEVELS_OUTER}.get(levels, levels)
     dev = dev or device()
     vs = state.variables
     out = {}
@@ -31,7 +36,10 @@ def to_device_state(state, dev=None):
                 out[k] = torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).to(dev)
         else:
             out[k] = v
-    return SimpleNamespace(variables=SimpleNamespace(**out), settings=state.settings)
+    ds = SimpleNamespace(variables=SimpleNamespace(**out), settings=state.settings)
+    if levels != LEVELS_INNER:
+        set_state_layout(ds, levels)
+    return ds
 
 
 def to_host_arrays(tensors: dict):

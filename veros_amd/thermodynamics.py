@@ -21,7 +21,8 @@ import torch
 
 from . import _native
 from ._arrays import ptr, stream_handle
-from .isoneutral import KernelOutput, _check_array
+from .isoneutral import KernelOutput, _check_array, _check_levels
+from .layout import common_layout
 
 
 def vertmix_tempsalt(state):
@@ -30,8 +31,8 @@ def vertmix_tempsalt(state):
     kbot = _check_array("kbot", vs.kbot, (X, Y), dtype=torch.int32)
     dzt = _check_array("dzt", vs.dzt, (nz,))
     dzw = _check_array("dzw", vs.dzw, (nz,))
-    T = _check_array("temp", vs.temp, (X, Y, nz, 3))
-    S = _check_array("salt", vs.salt, (X, Y, nz, 3))
+    T = _check_levels("temp", vs.temp, (X, Y, nz, 3))
+    S = _check_levels("salt", vs.salt, (X, Y, nz, 3))
     K = _check_array("kappaH", vs.kappaH, (X, Y, nz))
     fT = _check_array("forc_temp_surface", vs.forc_temp_surface, (X, Y))
     fS = _check_array("forc_salt_surface", vs.forc_salt_surface, (X, Y))
@@ -40,6 +41,7 @@ def vertmix_tempsalt(state):
     g = _native.VxGrid()
     g.kbot, g.dzt, g.dzw = ptr(kbot), ptr(dzt), ptr(dzw)
     g.nx, g.ny, g.nz = X - 4, Y - 4, nz
+    g.time_layout = common_layout(temp=T, salt=S)
     rc = _native.lib().vx_vertmix_tempsalt_f64(
         stream_handle(), ctypes.byref(g), float(st.dt_tracer), ptr(T), ptr(S), int(vs.taup1), ptr(K), ptr(fT),
         ptr(fS), ptr(dT), ptr(dS), int(bool(st.enable_cyclic_x)))
