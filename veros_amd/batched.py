@@ -1,8 +1,7 @@
-"""Many independent kbot-masked tridiagonal problems in one launch per depth
-bucket (C ABI: vx_tdma_plan_* in include/vxmix.h). ``precision="mixed"``
-(float32 sweeps, one float64 refinement step) runs the single-grid mixed
-kernel once per system instead, on the current stream or replayed as one
-captured hipGraph.
+"""Many independent kbot-masked tridiagonal problems in one persistent launch
+per depth bucket (C ABI: vx_tdma_plan_* in include/vxmix.h), in float64 or
+mixed precision (``precision="mixed"``: float32 sweeps, one float64
+refinement step), on the current stream or replayed as one captured hipGraph.
 
 Each system is (a, b, c, d, kbot, x[, b_edge, d_edge]) with the semantics of
 ``solve_implicit_kbot`` (veros/core/utilities.py:51-59): torch float64 tensors

@@ -1,14 +1,16 @@
 // Batched TDMA over many independent grids of different shapes and depths.
 //
 // A plan is built once from the list of systems (the arrays of every grid
-// stay where they are): systems are grouped by 16-level nz bucket and each
-// bucket's columns form one global column space. vx_tdma_plan_run launches
-// one persistent grid-stride kernel per non-empty bucket; a lane owns one
-// global column, finds its system by binary search over the bucket's column
-// offsets, and runs the same per-column solve as the single-system kernels
-// (tdma_core.hpp). vx_tdma_plan_tune times block-size / grid-size candidates
-// per bucket on the plan's own data and keeps the fastest (a process-wide
-// cache keyed by bucket, mode and size reuses earlier choices);
+// stay where they are): systems are grouped by 16-level nz bucket (8 levels
+// per lane in the segmented solve: S = bucket / 8) and each bucket's columns
+// form one global column space. vx_tdma_plan_run launches one persistent
+// grid-stride kernel per non-empty bucket; a group of kSegP lanes owns one
+// global column at a time, finds its system by binary search over the
+// bucket's column offsets, and runs the segmented solve of the single-grid
+// kernels (tdma_seg.hpp) in float64 (mode 0) or in mixed precision with one
+// refinement step (mode 1). vx_tdma_plan_tune times block-size / grid-size
+// candidates per bucket on the plan's own data and keeps the fastest (a
+// process-wide cache keyed by bucket, mode and size reuses earlier choices);
 // vx_tdma_plan_graph_launch captures run() once into a hipGraph and replays
 // it.
 #include <algorithm>
@@ -18,7 +20,7 @@
 #include <tuple>
 #include <vector>
 
-#include "tdma_core.hpp"
+#include "tdma_seg.hpp"
 
 struct vx_tdma_plan;
 
@@ -31,37 +33,68 @@ struct SysDesc {
     int32_t _pad;
 };
 
-template <int NZB, int MODE>
-__device__ __forceinline__ void solve_any(const TdmaArgs& p, const ColumnRef& cr, bool vec) {
-    if constexpr (MODE == 0) {
-        if (vec) solve_f64_column<NZB, true>(p, cr);
-        else solve_f64_column<NZB, false>(p, cr);
-    } else {
-        if (vec) solve_mixed_column<NZB, true, MODE>(p, cr);
-        else solve_mixed_column<NZB, false, MODE>(p, cr);
-    }
-}
-
-template <int NZB, int MODE>
-__global__ __launch_bounds__(256) void tdma_batch_kernel(const SysDesc* __restrict__ sys, int nsys, int64_t total) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
+// One launch per bucket: S levels per lane, MODE 0 = float64, 1 = mixed with
+// one refinement step. Every system's columns start at a multiple of 8 in
+// the bucket's column space (its count padded to a multiple of 8), so the 8
+// columns of a wave's tile always belong to one system: a wave finds it once
+// and, when its arrays allow 16-byte copies, runs the LDS-staged segmented
+// solve of the single-grid kernels; otherwise the register variant.
+template <int S, int MODE>
+__global__ __launch_bounds__(128) void tdma_seg_batch_kernel(const SysDesc* __restrict__ sys, int nsys,
+                                                             int64_t ntiles) {
+    extern __shared__ double seg_lds[];
+    double* lds = seg_lds + (threadIdx.x / 64) * (seg_lds_wave_bytes(S) / 8);
+    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / 64);
+    const int lane = threadIdx.x & 63;
+    const int s = lane % kSegP;
+    const int c = lane / kSegP;
+    // the tile index is the same in every lane of a wave: keep it (and so
+    // the system search and descriptor loads) in scalar registers
+    const int64_t w0 = (int64_t)blockIdx.x * (blockDim.x / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+    for (int64_t w = w0; w < ntiles; w += wstride) {
+        const int64_t g0 = w * kSegCols;  // the tile's first column
         int lo = 0, hi = nsys - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (sys[mid].col0 <= g) lo = mid;
+            if (sys[mid].col0 <= g0) lo = mid;
             else hi = mid - 1;
         }
+        lo = __builtin_amdgcn_readfirstlane(lo);
         const SysDesc& sd = sys[lo];
         const TdmaArgs p = sd.args;
-        const int64_t col = g - sd.col0;
-        const int64_t i = col / p.ny;
-        const int64_t j = col - i * p.ny;
-        ColumnRef cr;
-        cr.base = i * p.sx + j * p.sy;
-        const int kb = p.kbot[i * p.kx + j * p.ky];
-        cr.k0 = (kb >= 1 && kb <= p.nz) ? kb - 1 : p.nz;
-        solve_any<NZB, MODE>(p, cr, sd.vec != 0);
+        const int64_t col = g0 + c - sd.col0;
+        SegLane L;
+        L.s = s;
+        L.lo = s * S;
+        L.valid = col < p.ncol;
+        L.base = 0;
+        L.k0 = p.nz;
+        if (L.valid) {
+            const int64_t i = col / p.ny;
+            const int64_t j = col - i * p.ny;
+            L.base = i * p.sx + j * p.sy;
+            const int kb = p.kbot[i * p.kx + j * p.ky];
+            L.k0 = (kb >= 1 && kb <= p.nz) ? kb - 1 : p.nz;
+        }
+        double va[S], vb[S], vc[S], vd[S];
+        if (sd.vec) {
+            seg_rows_lds<S>(p, L, lds, va, vb, vc, vd);
+        } else {
+            seg_rows<S, false>(p, L, va, vb, vc, vd);
+        }
+        double x[S];
+        if constexpr (MODE == 0) {
+            seg_solve_f64<S>(L, va, vb, vc, vd);
+#pragma unroll
+            for (int t = 0; t < S; ++t) x[t] = vd[t];
+        } else {
+            seg_mixed_core<S, 1>(L, va, vb, vc, vd, x);
+        }
+        if (sd.vec) {
+            seg_store_lds<S>(p, L, lds, x);
+        } else if (L.valid) {
+            seg_store<S, false>(p.x + L.base, L.lo, p.nz, x);
+        }
     }
 }
 
@@ -69,15 +102,16 @@ namespace {
 
 constexpr int kBuckets[] = {16, 32, 48, 64, 80, 96, 112, 128};
 constexpr int kNumBuckets = 8;
-constexpr int kBlockSizes[] = {64, 128, 256};
+constexpr int kBlockSizes[] = {64, 128};  // one or two waves (LDS: seg_lds_wave_bytes(S) per wave)
 constexpr int kGridFactors[] = {1, 2, 4, 8, 0};  // blocks per CU; 0 = one block per 256 columns
 
 struct Bucket {
     int nzb = 0;
     int first = 0;       // index of the first system of this bucket in the device table
     int nsys = 0;
-    int64_t total = 0;   // columns
-    int bs = 256;
+    int64_t total = 0;   // columns, each system's count padded to a multiple of 8
+    int64_t cols = 0;    // columns without the padding
+    int bs = 128;
     int factor = 0;
 };
 
@@ -94,22 +128,23 @@ int ilog2(int64_t v) {
     return r;
 }
 
-template <int NZB, int MODE>
+template <int S, int MODE>
 void launch_bs(int bs, const SysDesc* d, int nsys, int64_t total, unsigned grid, hipStream_t s) {
-    hipLaunchKernelGGL((tdma_batch_kernel<NZB, MODE>), dim3(grid), dim3(bs), 0, s, d, nsys, total);
+    const size_t lds = (size_t)seg_lds_wave_bytes(S) * (bs / 64);
+    hipLaunchKernelGGL((tdma_seg_batch_kernel<S, MODE>), dim3(grid), dim3(bs), lds, s, d, nsys, total / kSegCols);
 }
 
 template <int MODE>
 void launch_mode(int nzb, int bs, const SysDesc* d, int nsys, int64_t total, unsigned grid, hipStream_t s) {
     switch (nzb) {
-        case 16: launch_bs<16, MODE>(bs, d, nsys, total, grid, s); break;
-        case 32: launch_bs<32, MODE>(bs, d, nsys, total, grid, s); break;
-        case 48: launch_bs<48, MODE>(bs, d, nsys, total, grid, s); break;
-        case 64: launch_bs<64, MODE>(bs, d, nsys, total, grid, s); break;
-        case 80: launch_bs<80, MODE>(bs, d, nsys, total, grid, s); break;
-        case 96: launch_bs<96, MODE>(bs, d, nsys, total, grid, s); break;
-        case 112: launch_bs<112, MODE>(bs, d, nsys, total, grid, s); break;
-        default: launch_bs<128, MODE>(bs, d, nsys, total, grid, s); break;
+        case 16: launch_bs<2, MODE>(bs, d, nsys, total, grid, s); break;
+        case 32: launch_bs<4, MODE>(bs, d, nsys, total, grid, s); break;
+        case 48: launch_bs<6, MODE>(bs, d, nsys, total, grid, s); break;
+        case 64: launch_bs<8, MODE>(bs, d, nsys, total, grid, s); break;
+        case 80: launch_bs<10, MODE>(bs, d, nsys, total, grid, s); break;
+        case 96: launch_bs<12, MODE>(bs, d, nsys, total, grid, s); break;
+        case 112: launch_bs<14, MODE>(bs, d, nsys, total, grid, s); break;
+        default: launch_bs<16, MODE>(bs, d, nsys, total, grid, s); break;
     }
 }
 
@@ -124,8 +159,6 @@ struct vx_tdma_plan {
     hipStream_t capture_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    // mode 1: the systems, each solved by the single-grid mixed kernel
-    std::vector<vx_tdma_system> mixed;
     // every non-empty system (tune checks that none solves in place)
     std::vector<vx_tdma_system> all;
 };
@@ -134,7 +167,7 @@ namespace vx {
 namespace {
 
 unsigned grid_for(const vx_tdma_plan* P, const Bucket& b, int bs, int factor) {
-    const int64_t full = (b.total + bs - 1) / bs;
+    const int64_t full = (b.total * kSegP + bs - 1) / bs;  // one 8-column tile per wave
     if (factor == 0) return (unsigned)std::max<int64_t>(1, full);
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(full, (int64_t)factor * P->num_cu));
 }
@@ -142,7 +175,8 @@ unsigned grid_for(const vx_tdma_plan* P, const Bucket& b, int bs, int factor) {
 void launch_bucket(const vx_tdma_plan* P, const Bucket& b, int bs, int factor, hipStream_t s) {
     const SysDesc* d = P->d_sys + b.first;
     const unsigned grid = grid_for(P, b, bs, factor);
-    launch_mode<0>(b.nzb, bs, d, b.nsys, b.total, grid, s);
+    if (P->mode == 0) launch_mode<0>(b.nzb, bs, d, b.nsys, b.total, grid, s);
+    else launch_mode<1>(b.nzb, bs, d, b.nsys, b.total, grid, s);
 }
 
 }  // namespace
@@ -157,14 +191,8 @@ extern "C" int vx_tdma_plan_create(const vx_tdma_system* systems, int32_t nsys, 
     if (!systems || nsys < 1) return fail(VX_ERR_INVALID_ARGUMENT, "%s: need at least one system", who);
     if (mode < 0 || mode > 1)
         return fail(VX_ERR_INVALID_ARGUMENT, "%s: mode must be 0 (fp64) or 1 (mixed, one refinement step)", who);
-    // Mode 1 (mixed) does not use a persistent batched kernel: that kernel
-    // (tdma_batch_kernel<NZB, 1>) took a device memory fault in its
-    // nz-bucket-80 instantiation although its index arithmetic stays in bounds
-    // in a host build under AddressSanitizer, and is not built. A mode-1 plan
-    // runs the single-grid mixed kernel (vx_tdma_mixed, one refinement step)
-    // once per system, on the caller's stream or captured into one hipGraph.
     std::vector<std::vector<SysDesc>> per(kNumBuckets);
-    std::vector<vx_tdma_system> mixed, all;
+    std::vector<vx_tdma_system> all;
     for (int q = 0; q < nsys; ++q) {
         const vx_tdma_system& y = systems[q];
         if (y.nz < 1 || y.nx < 0 || y.ny < 0)
@@ -186,12 +214,10 @@ extern "C" int vx_tdma_plan_create(const vx_tdma_system* systems, int32_t nsys, 
                      ? 1
                      : 0;
         per[(y.nz + 15) / 16 - 1].push_back(sd);
-        if (mode == 1) mixed.push_back(y);
         all.push_back(y);
     }
     auto* P = new vx_tdma_plan();
     P->mode = mode;
-    P->mixed = std::move(mixed);
     P->all = std::move(all);
     int dev = 0;
     hipDeviceProp_t prop;
@@ -210,7 +236,8 @@ extern "C" int vx_tdma_plan_create(const vx_tdma_system* systems, int32_t nsys, 
         int64_t col = 0;
         for (SysDesc& sd : per[bk]) {
             sd.col0 = col;
-            col += sd.args.ncol;
+            col += (sd.args.ncol + kSegCols - 1) / kSegCols * kSegCols;
+            b.cols += sd.args.ncol;
             table.push_back(sd);
         }
         b.total = col;
@@ -224,7 +251,7 @@ extern "C" int vx_tdma_plan_create(const vx_tdma_system* systems, int32_t nsys, 
         }
         P->buckets.push_back(b);
     }
-    if (!table.empty() && mode == 0) {
+    if (!table.empty()) {
         if (hipMalloc(&P->d_sys, table.size() * sizeof(SysDesc)) != hipSuccess) {
             delete P;
             return fail(VX_ERR_LAUNCH, "%s: hipMalloc of the system table failed", who);
@@ -264,14 +291,6 @@ bool solves_in_place(const std::vector<vx_tdma_system>& sys) {
 }
 
 int run_plan(vx_tdma_plan* plan, hipStream_t s, const char* who) {
-    if (plan->mode == 1) {
-        for (const vx_tdma_system& y : plan->mixed) {
-            const int rc = vx_tdma_mixed(s, y.a, y.b, y.c, y.d, y.b_edge, y.d_edge, y.kbot, y.kbot_sx, y.kbot_sy,
-                                         y.x, y.nx, y.ny, y.nz, y.sx, y.sy, 1);
-            if (rc) return rc;
-        }
-        return check_launch(who);
-    }
     for (const Bucket& b : plan->buckets) launch_bucket(plan, b, b.bs, b.factor, s);
     return check_launch(who);
 }
@@ -293,7 +312,6 @@ extern "C" int vx_tdma_plan_tune(vx_tdma_plan* plan, void* stream) {
     if (solves_in_place(plan->all))
         return fail(VX_ERR_INVALID_ARGUMENT, "%s: a system's x overlaps an input (in-place solve); tuning would "
                     "overwrite that input", who);
-    if (plan->mode == 1) return VX_OK;  // per-system launches of a fixed shape: nothing to choose
     hipStream_t s = (hipStream_t)stream;
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
@@ -385,7 +403,7 @@ extern "C" int vx_tdma_plan_info(const vx_tdma_plan* plan, int32_t* nbuckets, in
     int64_t tot = 0;
     int q = 0;
     for (const Bucket& b : plan->buckets) {
-        tot += b.total;
+        tot += b.cols;
         if (choices && q < max_choices) {
             choices[3 * q + 0] = b.nzb;
             choices[3 * q + 1] = b.bs;
