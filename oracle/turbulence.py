@@ -1,0 +1,259 @@
+"""Oracle: implicit vertical integration of TKE, EKE and internal-wave energy
+(TEST INFRASTRUCTURE ONLY).
+
+Restates, in NumPy float64,
+* ``integrate_tke_kernel``     veros/core/tke.py:123-321
+* ``integrate_eke_kernel``     veros/core/eke.py:93-255
+* ``integrate_idemix_kernel``  veros/core/idemix.py:47-265
+on top of this package's ``solve_implicit`` (veros/core/utilities.py:51-59):
+the dissipation forcing from the settings branches, the implicit vertical
+diffusion/dissipation system with its edge row (and, for IDEMIX, the edge
+right-hand side d_edge), the stored dissipation diagnostics, the surface
+correction of TKE and the lateral diffusion terms. The advection branches
+(superbee / upwind, a separate component of veros) are not restated: the
+functions raise NotImplementedError when a state enables them.
+
+Pins (tests/test_oracle_turbulence.py): the reference's own outputs on seeded
+states, several settings combinations each (tests/golden/tke_*.npz, eke_*.npz,
+idemix_*.npz); plus, independent of the reference, a dense solve of every
+column's matrix built from the formulas, and the fixed point of a vertically
+uniform field without forcing or dissipation.
+
+``state`` has ``.variables`` (veros-shaped arrays, 2-cell halos, z
+contiguous, k = 0 deepest) and ``.settings``. Inputs are never modified.
+"""
+from __future__ import annotations
+
+from types import SimpleNamespace
+
+import numpy as np
+
+from .isoneutral import KernelOutput
+from .tdma import create_water_masks, solve_implicit
+
+
+def _f(x):
+    return np.asarray(x, dtype=np.float64)
+
+
+def _masks(vs):
+    return _f(vs.maskU), _f(vs.maskV), _f(vs.maskW)
+
+
+def _lateral(vs, F, coef_e, coef_n):
+    """Divergence of the lateral fluxes coef_e * (F[i+1] - F[i]) / (cost dxu)
+    * maskU and coef_n * (F[j+1] - F[j]) / dyu * maskV * cosu on [2:-2, 2:-2]
+    (veros/core/tke.py:250-276 and the same pattern in eke.py, idemix.py);
+    ``coef_e``/``coef_n`` are the face coefficients on [:-1] / [:, :-1]."""
+    maskU, maskV, _ = _masks(vs)
+    cost, cosu = _f(vs.cost), _f(vs.cosu)
+    dxt, dxu, dyt, dyu = _f(vs.dxt), _f(vs.dxu), _f(vs.dyt), _f(vs.dyu)
+    fe = np.zeros(F.shape)
+    fn = np.zeros(F.shape)
+    fe[:-1] = coef_e * (F[1:] - F[:-1]) / (cost[None, :, None] * dxu[:-1, None, None]) * maskU[:-1]
+    fn[:, :-1] = coef_n * (F[:, 1:] - F[:, :-1]) / dyu[None, :-1, None] * maskV[:, :-1] * cosu[None, :-1, None]
+    fn[:, -1] = 0.0
+    return ((fe[2:-2, 2:-2] - fe[1:-3, 2:-2]) / (cost[None, 2:-2, None] * dxt[2:-2, None, None])
+            + (fn[2:-2, 2:-2] - fn[2:-2, 1:-3]) / (cost[None, 2:-2, None] * dyt[None, 2:-2, None]))
+
+
+def tke_forcing(vs, st):
+    """The dissipation terms that feed TKE (veros/core/tke.py:139-177)."""
+    forc = _f(vs.K_diss_v) - _f(vs.P_diss_v) - _f(vs.P_diss_adv)
+    if not st.enable_store_cabbeling_heat:
+        forc = forc - _f(vs.P_diss_nonlin)
+    if st.enable_eke:
+        forc = forc + _f(vs.eke_diss_tke)
+    if st.enable_idemix:
+        forc = forc + _f(vs.iw_diss)
+        if st.enable_store_bottom_friction_tke:
+            forc = forc + _f(vs.K_diss_bot)
+    else:
+        if st.enable_eke:
+            forc = forc + _f(vs.eke_diss_iw)
+        else:
+            if st.enable_store_cabbeling_heat:
+                forc = (forc + _f(vs.K_diss_gm) + _f(vs.K_diss_h) - _f(vs.P_diss_skew) - _f(vs.P_diss_hmix)
+                        - _f(vs.P_diss_iso))
+            else:
+                forc = forc + _f(vs.K_diss_gm) + _f(vs.K_diss_h) - _f(vs.P_diss_skew)
+        forc = forc + _f(vs.K_diss_bot)
+    return forc
+
+
+def integrate_tke_kernel(state, solver="dgtsv"):
+    """veros/core/tke.py:123-321 without the advection branch: returns
+    KernelOutput(tke, tke_surf_corr, tke_diss) as new arrays."""
+    vs, st = state.variables, state.settings
+    if st.enable_tke_superbee_advection or st.enable_tke_upwind_advection:
+        raise NotImplementedError("TKE advection is not part of the oracle")
+    tau, taup1 = int(vs.tau), int(vs.taup1)
+    nz = st.nz
+    dt = st.dt_mom
+    dzt, dzw = _f(vs.dzt), _f(vs.dzw)
+    kappaM, sqrttke, mxl = _f(vs.kappaM), _f(vs.sqrttke), _f(vs.mxl)
+    tke = np.array(vs.tke, dtype=np.float64, copy=True)
+    forc = tke_forcing(vs, st)
+    _, water, edge = create_water_masks(np.asarray(vs.kbot)[2:-2, 2:-2], nz)
+    km, sq, ml = kappaM[2:-2, 2:-2], sqrttke[2:-2, 2:-2], mxl[2:-2, 2:-2]
+    shape = km.shape
+    delta, a, b, c = (np.zeros(shape) for _ in range(4))
+    delta[:, :, :-1] = dt / dzt[None, None, 1:] * st.alpha_tke * 0.5 * (km[:, :, :-1] + km[:, :, 1:])
+    a[:, :, 1:-1] = -delta[:, :, :-2] / dzw[None, None, 1:-1]
+    a[:, :, -1] = -delta[:, :, -2] / (0.5 * dzw[-1])
+    b[:, :, 1:-1] = (1 + (delta[:, :, 1:-1] + delta[:, :, :-2]) / dzw[None, None, 1:-1]
+                     + dt * st.c_eps * sq[:, :, 1:-1] / ml[:, :, 1:-1])
+    b[:, :, -1] = 1 + delta[:, :, -2] / (0.5 * dzw[-1]) + dt * st.c_eps / ml[:, :, -1] * sq[:, :, -1]
+    b_edge = 1 + delta / dzw[None, None, :] + dt * st.c_eps / ml * sq
+    c[:, :, :-1] = -delta[:, :, :-1] / dzw[None, None, :-1]
+    d = tke[2:-2, 2:-2, :, tau] + dt * forc[2:-2, 2:-2, :]
+    d[:, :, -1] += dt * _f(vs.forc_tke_surface)[2:-2, 2:-2] / (0.5 * dzw[-1])
+    sol = solve_implicit(a, b, c, d, water, edge, b_edge=b_edge, solver=solver)
+    tke[2:-2, 2:-2, :, taup1] = np.where(water, sol, tke[2:-2, 2:-2, :, taup1])
+    tke_diss = st.c_eps / mxl * sqrttke * tke[:, :, :, taup1]
+    surf = tke[2:-2, 2:-2, -1, taup1]
+    tke_surf_corr = np.array(vs.tke_surf_corr, dtype=np.float64, copy=True)
+    tke_surf_corr[2:-2, 2:-2] = np.where(surf < 0.0, -surf * 0.5 * dzw[-1] / dt, 0.0)
+    tke[2:-2, 2:-2, -1, taup1] = np.maximum(0.0, surf)
+    if st.enable_tke_hor_diffusion:
+        T = tke[:, :, :, tau]
+        maskW = _masks(vs)[2]
+        div = _lateral(vs, T, st.K_h_tke, st.K_h_tke)
+        tke[2:-2, 2:-2, :, taup1] += dt * maskW[2:-2, 2:-2] * div
+    return KernelOutput(tke=tke, tke_surf_corr=tke_surf_corr, tke_diss=tke_diss)
+
+
+def eke_forcing(vs, st):
+    """veros/core/eke.py:109-118."""
+    forc = _f(vs.K_diss_gm) + _f(vs.K_diss_h) - _f(vs.P_diss_skew)
+    if not st.enable_store_cabbeling_heat:
+        forc = forc - _f(vs.P_diss_hmix) - _f(vs.P_diss_iso)
+    return forc
+
+
+def integrate_eke_kernel(state, solver="dgtsv"):
+    """veros/core/eke.py:93-255 without the advection branch: returns
+    KernelOutput(eke, eke_diss_iw, eke_diss_tke) as new arrays."""
+    vs, st = state.variables, state.settings
+    if st.enable_eke_superbee_advection or st.enable_eke_upwind_advection:
+        raise NotImplementedError("EKE advection is not part of the oracle")
+    tau, taup1 = int(vs.tau), int(vs.taup1)
+    nz = st.nz
+    dt = st.dt_tracer
+    dzt, dzw = _f(vs.dzt), _f(vs.dzw)
+    _, _, maskW = _masks(vs)
+    eke = np.array(vs.eke, dtype=np.float64, copy=True)
+    forc = eke_forcing(vs, st)
+    c_int = st.eke_c_eps * _f(vs.sqrteke) / _f(vs.eke_len) * maskW
+    _, water, edge = create_water_masks(np.asarray(vs.kbot)[2:-2, 2:-2], nz)
+    km = _f(vs.kappaM)[2:-2, 2:-2]
+    ci = c_int[2:-2, 2:-2]
+    shape = km.shape
+    delta, a, b, c = (np.zeros(shape) for _ in range(4))
+    delta[:, :, :-1] = dt / dzt[None, None, 1:] * 0.5 * (km[:, :, :-1] + km[:, :, 1:]) * st.alpha_eke
+    a[:, :, 1:-1] = -delta[:, :, :-2] / dzw[1:-1]
+    a[:, :, -1] = -delta[:, :, -2] / (0.5 * dzw[-1])
+    b[:, :, 1:-1] = 1 + (delta[:, :, 1:-1] + delta[:, :, :-2]) / dzw[1:-1] + dt * ci[:, :, 1:-1]
+    b[:, :, -1] = 1 + delta[:, :, -2] / (0.5 * dzw[-1]) + dt * ci[:, :, -1]
+    b_edge = 1 + delta / dzw[None, None, :] + dt * ci
+    c[:, :, :-1] = -delta[:, :, :-1] / dzw[None, None, :-1]
+    d = eke[2:-2, 2:-2, :, tau] + dt * forc[2:-2, 2:-2, :]
+    sol = solve_implicit(a, b, c, d, water, edge, b_edge=b_edge, solver=solver)
+    eke[2:-2, 2:-2, :, taup1] = np.where(water, sol, eke[2:-2, 2:-2, :, taup1])
+    eke_diss_iw = c_int * eke[:, :, :, taup1]
+    eke_diss_tke = np.zeros(eke_diss_iw.shape)
+    K_gm = _f(vs.K_gm)
+    E = eke[:, :, :, tau]
+    ce = 0.5 * np.maximum(500.0, K_gm[:-1] + K_gm[1:])
+    cn = 0.5 * np.maximum(500.0, K_gm[:, :-1] + K_gm[:, 1:])
+    eke[2:-2, 2:-2, :, taup1] += dt * maskW[2:-2, 2:-2] * _lateral(vs, E, ce, cn)
+    return KernelOutput(eke=eke, eke_diss_iw=eke_diss_iw, eke_diss_tke=eke_diss_tke)
+
+
+def idemix_forcing(vs, st):
+    """veros/core/idemix.py:59-115: the EKE dissipation (or its shortcut),
+    optionally moved to the bottom / surface cells, plus bottom friction."""
+    _, _, maskW = _masks(vs)
+    dzw = _f(vs.dzw)
+    nz = st.nz
+    if st.enable_eke:
+        forc = _f(vs.eke_diss_iw).copy()
+    else:
+        forc = _f(vs.K_diss_gm) + _f(vs.K_diss_h) - _f(vs.P_diss_skew)
+        if st.enable_store_cabbeling_heat:
+            forc += -_f(vs.P_diss_hmix) - _f(vs.P_diss_iso)
+    if st.enable_eke and (st.enable_eke_diss_bottom or st.enable_eke_diss_surfbot):
+        a_loc = np.sum(dzw[None, None, :-1] * forc[:, :, :-1] * maskW[:, :, :-1], axis=2)
+        a_loc += 0.5 * forc[:, :, -1] * maskW[:, :, -1] * dzw[-1]
+        forc = np.zeros(forc.shape)
+        ks = np.maximum(0, np.asarray(vs.kbot)[2:-2, 2:-2] - 1)
+        mask = ks[:, :, None] == np.arange(nz)[None, None, :]
+        if st.enable_eke_diss_bottom:
+            forc[2:-2, 2:-2, :] = np.where(mask, a_loc[2:-2, 2:-2, None] / dzw[None, None, :], forc[2:-2, 2:-2, :])
+        else:
+            forc[2:-2, 2:-2, :] = np.where(mask, st.eke_diss_surfbot_frac * a_loc[2:-2, 2:-2, None]
+                                           / dzw[None, None, :], forc[2:-2, 2:-2, :])
+            forc[2:-2, 2:-2, -1] = (1.0 - st.eke_diss_surfbot_frac) * a_loc[2:-2, 2:-2] / (0.5 * dzw[-1])
+    if not st.enable_store_bottom_friction_tke:
+        forc = forc + _f(vs.K_diss_bot)
+    return forc
+
+
+def integrate_idemix_kernel(state, solver="dgtsv"):
+    """veros/core/idemix.py:47-265 without the advection branch: returns
+    KernelOutput(E_iw, dE_iw, iw_diss) as new arrays (dE_iw unchanged)."""
+    vs, st = state.variables, state.settings
+    if st.enable_idemix_superbee_advection or st.enable_idemix_upwind_advection:
+        raise NotImplementedError("IDEMIX advection is not part of the oracle")
+    tau, taup1 = int(vs.tau), int(vs.taup1)
+    nz = st.nz
+    dt = st.dt_tracer
+    dzt, dzw = _f(vs.dzt), _f(vs.dzw)
+    E_iw = np.array(vs.E_iw, dtype=np.float64, copy=True)
+    forc = idemix_forcing(vs, st)
+    maxE = np.maximum(0.0, E_iw[:, :, :, tau])
+    _, water, edge = create_water_masks(np.asarray(vs.kbot)[2:-2, 2:-2], nz)
+    c0 = _f(vs.c0)[2:-2, 2:-2]
+    al = _f(vs.alpha_c)[2:-2, 2:-2]
+    mx = maxE[2:-2, 2:-2]
+    shape = c0.shape
+    delta, a, b, c = (np.zeros(shape) for _ in range(4))
+    delta[:, :, :-1] = dt * st.tau_v / dzt[None, None, 1:] * 0.5 * (c0[:, :, :-1] + c0[:, :, 1:])
+    a[:, :, 1:-1] = -delta[:, :, :-2] * c0[:, :, :-2] / dzw[None, None, 1:-1]
+    a[:, :, -1] = -delta[:, :, -2] / (0.5 * dzw[-1:]) * c0[:, :, -2]
+    b[:, :, 1:-1] = (1 + delta[:, :, 1:-1] * c0[:, :, 1:-1] / dzw[None, None, 1:-1]
+                     + delta[:, :, :-2] * c0[:, :, 1:-1] / dzw[None, None, 1:-1] + dt * al[:, :, 1:-1] * mx[:, :, 1:-1])
+    b[:, :, -1] = 1 + delta[:, :, -2] / (0.5 * dzw[-1:]) * c0[:, :, -1] + dt * al[:, :, -1] * mx[:, :, -1]
+    b_edge = 1 + delta / dzw * c0 + dt * al * mx
+    c[:, :, :-1] = -delta[:, :, :-1] / dzw[None, None, :-1] * c0[:, :, 1:]
+    d = E_iw[2:-2, 2:-2, :, tau] + dt * forc[2:-2, 2:-2, :]
+    d_edge = d + dt * _f(vs.forc_iw_bottom)[2:-2, 2:-2, None] / dzw[None, None, :]
+    d[:, :, -1] += dt * _f(vs.forc_iw_surface)[2:-2, 2:-2] / (0.5 * dzw[-1:])
+    sol = solve_implicit(a, b, c, d, water, edge, b_edge=b_edge, d_edge=d_edge, solver=solver)
+    E_iw[2:-2, 2:-2, :, taup1] = np.where(water, sol, E_iw[2:-2, 2:-2, :, taup1])
+    iw_diss = _f(vs.alpha_c) * maxE * E_iw[..., taup1]
+    if st.enable_idemix_hor_diffusion:
+        v0 = _f(vs.v0)
+        E = E_iw[:, :, :, tau]
+        maskW = _masks(vs)[2]
+        ce = st.tau_h * 0.5 * (v0[1:] + v0[:-1])
+        cn = st.tau_h * 0.5 * (v0[:, 1:] + v0[:, :-1])
+        # the fluxes difference v0 * E, not E
+        VE = v0 * E
+        E_iw[2:-2, 2:-2, :, taup1] += dt * maskW[2:-2, 2:-2] * _lateral(vs, VE, ce, cn)
+    return KernelOutput(E_iw=E_iw, dE_iw=np.array(vs.dE_iw, dtype=np.float64, copy=True), iw_diss=iw_diss)
+
+
+def turbulence_settings(**kw):
+    """The settings the three kernels read, with veros's defaults
+    (veros/settings.py:28-128)."""
+    d = dict(nz=None, dt_mom=3600.0, dt_tracer=3600.0, alpha_tke=1.0, c_eps=0.7, K_h_tke=2000.0,
+             enable_tke_hor_diffusion=False, enable_tke_superbee_advection=False, enable_tke_upwind_advection=False,
+             enable_store_cabbeling_heat=False, enable_store_bottom_friction_tke=False, enable_eke=False,
+             enable_idemix=False, eke_c_eps=1.0, alpha_eke=1.0, enable_eke_superbee_advection=False,
+             enable_eke_upwind_advection=False, tau_v=2.0 * 86400.0, tau_h=15.0 * 86400.0,
+             enable_idemix_hor_diffusion=False, enable_eke_diss_bottom=False, enable_eke_diss_surfbot=False,
+             eke_diss_surfbot_frac=1.0, enable_idemix_superbee_advection=False,
+             enable_idemix_upwind_advection=False, enable_cyclic_x=True)
+    d.update(kw)
+    return SimpleNamespace(**d)

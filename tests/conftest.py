@@ -75,3 +75,15 @@ def column_state_from_golden(rec):
     vs.taup1 = int(rec["set_taup1"])
     kw = {k[4:]: v.item() for k, v in rec.items() if k.startswith("set_") and k not in ("set_tau", "set_taup1")}
     return SimpleNamespace(variables=vs, settings=MixSettings(**kw))
+
+
+def turbulence_state_from_golden(rec):
+    """The state of a tke_* / eke_* / idemix_* fixture: inputs in_*, the
+    settings set_* and the time levels."""
+    vs = SimpleNamespace(**{k[3:]: np.array(v) for k, v in rec.items() if k.startswith("in_")})
+    vs.tau = int(rec["set_tau"])
+    vs.taup1 = int(rec["set_taup1"])
+    vs.taum1 = int(rec["set_taum1"])
+    st = SimpleNamespace(**{k[4:]: v.item() for k, v in rec.items()
+                            if k.startswith("set_") and k not in ("set_tau", "set_taup1", "set_taum1")})
+    return SimpleNamespace(variables=vs, settings=st)

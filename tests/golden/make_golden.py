@@ -16,8 +16,11 @@ Functions covered (reference file:line):
 * vertmix_tempsalt             veros/core/thermodynamics.py:248-300 (with enforce_boundaries)
 * implicit_vert_friction       veros/core/friction.py:90-205 (with ugrid_to_tgrid /
                                vgrid_to_tgrid, veros/core/numerics.py:313-336)
+* integrate_tke_kernel         veros/core/tke.py:123-321
+* integrate_eke_kernel         veros/core/eke.py:93-255
+* integrate_idemix_kernel      veros/core/idemix.py:47-265
 
-Usage: python tests/golden/make_golden.py [column]   (no argument: every fixture)
+Usage: python tests/golden/make_golden.py [column] [turbulence]   (no argument: every fixture)
 """
 from __future__ import annotations
 
@@ -53,9 +56,10 @@ def _import_reference():
     os.environ["VEROS_FLOAT_TYPE"] = "float64"
     from veros.core import operators, utilities  # noqa: F401
     from veros.core.isoneutral import isoneutral, diffusion
-    from veros.core import thermodynamics, friction
+    from veros.core import thermodynamics, friction, tke, eke, idemix
     return SimpleNamespace(operators=operators, utilities=utilities, isoneutral=isoneutral,
-                           diffusion=diffusion, thermodynamics=thermodynamics, friction=friction)
+                           diffusion=diffusion, thermodynamics=thermodynamics, friction=friction, tke=tke, eke=eke,
+                           idemix=idemix)
 
 
 def _ref_state(state):
@@ -246,6 +250,71 @@ def column_cases(ref):
     return out
 
 
+TURB_CASES = {
+    # name: (model, grid spec, settings overrides)
+    "tke_default": ("tke", "12x10x8_cyclic", dict()),
+    "tke_hordiff_cabbeling": ("tke", "16x12x20_strong", dict(enable_tke_hor_diffusion=True,
+                                                             enable_store_cabbeling_heat=True)),
+    "tke_idemix_botfric": ("tke", "9x7x5_closed", dict(enable_idemix=True, enable_store_bottom_friction_tke=True,
+                                                       enable_eke=True, enable_tke_hor_diffusion=True)),
+    "tke_eke": ("tke", "12x10x8_cyclic", dict(enable_eke=True)),
+    "eke_default": ("eke", "12x10x8_cyclic", dict(enable_eke=True)),
+    "eke_cabbeling": ("eke", "16x12x20_strong", dict(enable_eke=True, enable_store_cabbeling_heat=True)),
+    "eke_closed": ("eke", "9x7x5_closed", dict(enable_eke=True)),
+    "idemix_diss_bottom": ("idemix", "12x10x8_cyclic", dict(enable_eke=True, enable_eke_diss_bottom=True,
+                                                           enable_idemix_hor_diffusion=True)),
+    "idemix_surfbot": ("idemix", "16x12x20_strong", dict(enable_eke=True, enable_eke_diss_surfbot=True,
+                                                         eke_diss_surfbot_frac=0.3)),
+    "idemix_noeke": ("idemix", "9x7x5_closed", dict(enable_store_cabbeling_heat=True,
+                                                    enable_idemix_hor_diffusion=True)),
+}
+TURB_OUT = {"tke": ("tke", "tke_surf_corr", "tke_diss"), "eke": ("eke", "eke_diss_iw", "eke_diss_tke"),
+            "idemix": ("E_iw", "dE_iw", "iw_diss")}
+_GRID_IN = ("kbot", "maskU", "maskV", "maskW", "dxt", "dxu", "dyt", "dyu", "cost", "cosu", "dzt", "dzw")
+TURB_IN = {
+    "tke": _GRID_IN + ("kappaM", "tke", "sqrttke", "mxl", "forc_tke_surface", "tke_surf_corr", "tke_diss",
+                       "K_diss_v", "P_diss_v", "P_diss_adv", "P_diss_nonlin", "eke_diss_tke", "iw_diss",
+                       "K_diss_bot", "eke_diss_iw", "K_diss_gm", "K_diss_h", "P_diss_skew", "P_diss_hmix",
+                       "P_diss_iso"),
+    "eke": _GRID_IN + ("kappaM", "eke", "sqrteke", "eke_len", "K_gm", "K_diss_gm", "K_diss_h", "P_diss_skew",
+                       "P_diss_hmix", "P_diss_iso"),
+    "idemix": _GRID_IN + ("E_iw", "dE_iw", "c0", "v0", "alpha_c", "forc_iw_bottom", "forc_iw_surface",
+                          "eke_diss_iw", "K_diss_bot", "K_diss_gm", "K_diss_h", "P_diss_skew", "P_diss_hmix",
+                          "P_diss_iso"),
+}
+
+
+def turbulence_cases(ref):
+    from oracle.turbulence import turbulence_settings
+    from veros_amd import workloads as W
+    out = {}
+    for name, (model, grid, over) in TURB_CASES.items():
+        st = column_state(grid)
+        sp = COLUMN_SPECS[grid]
+        rng = np.random.default_rng(sp["seed"] + 1000)
+        W.add_turbulence_fields(rng, st, random_aux=True)
+        vs = st.variables
+        vs.taum1 = 3 - vs.tau - vs.taup1
+        sett = turbulence_settings(nz=sp["nz"], enable_cyclic_x=sp["cyclic"], dt_mom=1800.0, **over)
+        v = SimpleNamespace(**{k: (np.array(x, copy=True) if isinstance(x, np.ndarray) else x)
+                               for k, x in vars(vs).items()})
+        dims = {"xt": sp["nx"], "xu": sp["nx"], "yt": sp["ny"], "yu": sp["ny"], "zt": sp["nz"], "zw": sp["nz"]}
+        rs = SimpleNamespace(variables=v, settings=SimpleNamespace(**vars(sett)), dimensions=dims)
+        fn = {"tke": ref.tke.integrate_tke_kernel, "eke": ref.eke.integrate_eke_kernel,
+              "idemix": ref.idemix.integrate_idemix_kernel}[model]
+        res = fn(rs)
+        rec = {"in_" + k: np.asarray(getattr(vs, k)) for k in TURB_IN[model]}
+        rec.update({"set_" + k: np.asarray(x) for k, x in vars(sett).items()})
+        rec["set_tau"] = np.asarray(vs.tau)
+        rec["set_taup1"] = np.asarray(vs.taup1)
+        rec["set_taum1"] = np.asarray(vs.taum1)
+        rec["model"] = np.asarray(model)
+        for k in TURB_OUT[model]:
+            rec["out_" + k] = np.asarray(getattr(res, k))
+        out[name] = rec
+    return out
+
+
 def main():
     sys.path.insert(0, str(REPO))
     ref = _import_reference()
@@ -256,8 +325,12 @@ def main():
             for name, rec in column_cases(ref).items():
                 np.savez_compressed(HERE / f"{name}.npz", **rec)
                 print("wrote", f"{name}.npz")
-        if only - {"column"}:
-            raise SystemExit(f"unknown groups {sorted(only - {'column'})}")
+        if "turbulence" in only:
+            for name, rec in turbulence_cases(ref).items():
+                np.savez_compressed(HERE / f"{name}.npz", **rec)
+                print("wrote", f"{name}.npz")
+        if only - {"column", "turbulence"}:
+            raise SystemExit(f"unknown groups {sorted(only - {'column', 'turbulence'})}")
         return
     for name, rec in tdma_cases(ref).items():
         np.savez_compressed(HERE / f"tdma_{name}.npz", **rec)
@@ -266,6 +339,9 @@ def main():
         np.savez_compressed(HERE / f"{name}.npz", **rec)
         print("wrote", f"{name}.npz")
     for name, rec in column_cases(ref).items():
+        np.savez_compressed(HERE / f"{name}.npz", **rec)
+        print("wrote", f"{name}.npz")
+    for name, rec in turbulence_cases(ref).items():
         np.savez_compressed(HERE / f"{name}.npz", **rec)
         print("wrote", f"{name}.npz")
 

@@ -373,6 +373,48 @@ def add_friction_fields(rng, state, *, kappa_lo=1e-5, kappa_hi=1e-2, u_scale=0.1
     return state
 
 
+def add_turbulence_fields(rng, state, *, random_aux=False):
+    """Add the fields the TKE, EKE and IDEMIX integrations read and write
+    (veros/core/tke.py:123-321, eke.py:93-255, idemix.py:47-265) to an
+    isoneutral state: energies tke, eke, E_iw ``(X, Y, nz, 3)`` (E_iw with a
+    few negative values, so that max(0, E_iw) matters; the surface TKE
+    forcing with both signs, so that the surface correction acts), their
+    diffusivities and scales (kappaM ~ LogU(1e-5, 1e-2), sqrttke =
+    sqrt(tke[tau]), mxl ~ U(1, 100) m, sqrteke, eke_len ~ U(1e3, 1e5) m,
+    K_gm ~ U(150, 450)), the IDEMIX parameters c0, v0, alpha_c (times maskW,
+    as set_idemix_parameter forms them), and the dissipation terms that feed
+    the three (~1e-9 W/kg). ``random_aux`` fills the outputs with noise so
+    tests see which cells are written."""
+    vs = state.variables
+    X, Y, nz = vs.K_iso.shape
+    tau = int(vs.tau)
+    maskW = vs.maskW.astype(np.float64)
+    f3 = (X, Y, nz)
+    if not hasattr(vs, "kappaM"):
+        vs.kappaM = np.exp(rng.uniform(np.log(1e-5), np.log(1e-2), size=f3))
+    vs.tke = rng.uniform(1e-5, 1e-2, size=(X, Y, nz, 3))
+    vs.sqrttke = np.sqrt(np.maximum(0.0, vs.tke[..., tau]))
+    vs.mxl = rng.uniform(1.0, 100.0, size=f3)
+    vs.forc_tke_surface = 2e-5 * rng.standard_normal((X, Y))
+    vs.eke = rng.uniform(1e-4, 1e-2, size=(X, Y, nz, 3))
+    vs.sqrteke = np.sqrt(vs.eke[..., tau])
+    vs.eke_len = rng.uniform(1e3, 1e5, size=f3)
+    vs.K_gm = rng.uniform(150.0, 450.0, size=f3)
+    vs.E_iw = rng.uniform(-1e-5, 1e-3, size=(X, Y, nz, 3))
+    vs.dE_iw = rng.standard_normal((X, Y, nz, 3)) if random_aux else np.zeros((X, Y, nz, 3))
+    vs.c0 = rng.uniform(0.0, 1.0, size=f3) * maskW
+    vs.v0 = rng.uniform(0.0, 0.5, size=f3) * maskW
+    vs.alpha_c = rng.uniform(0.0, 1e-3, size=f3) * maskW
+    vs.forc_iw_bottom = 1e-6 * rng.uniform(0.0, 1.0, size=(X, Y))
+    vs.forc_iw_surface = 1e-6 * rng.uniform(0.0, 1.0, size=(X, Y))
+    for k in ("K_diss_v", "P_diss_v", "P_diss_adv", "P_diss_nonlin", "eke_diss_tke", "iw_diss", "K_diss_bot",
+              "eke_diss_iw", "K_diss_gm", "K_diss_h", "P_diss_skew", "P_diss_hmix", "P_diss_iso"):
+        setattr(vs, k, 1e-9 * rng.standard_normal(f3) * maskW)
+    vs.tke_surf_corr = rng.standard_normal((X, Y)) if random_aux else np.zeros((X, Y))
+    vs.tke_diss = rng.standard_normal(f3) if random_aux else np.zeros(f3)
+    return state
+
+
 def friction_config2(seed=1002, nx=1440, ny=720, nz=80):
     """The 1440x720x80 grid of the mixing step with the vertical-friction fields."""
     st = iso_config2(seed, nx, ny, nz)
