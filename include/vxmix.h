@@ -157,8 +157,8 @@ void vx_tdma_plan_destroy(vx_tdma_plan* plan);
  *       (VX_LEVELS_OUTER: level l of cell (i, j, k) at l*X*Y*nz +
  *       (i*Y + j)*nz + k; the same arrays seen as the (X, Y, nz, 3) view with
  *       strides (Y*nz, nz, 1, X*Y*nz)). One layout for every time-level
- *       array of a call; the velocities u, v of the vertical friction follow
- *       it too.
+ *       array of a call; the velocities u, v of the vertical friction and
+ *       the energies tke, eke, E_iw follow it too.
  *   K_iso, K_11, K_22, K_33, dtr_iso, P_diss : (X, Y, nz)
  *   Ai_ez, Ai_nz, Ai_bx, Ai_by   : (X, Y, nz, 2, 2)
  *   kbot : (X, Y) int32;  dxt, dxu : (X);  dyt, dyu, cost, cosu : (Y);
@@ -281,6 +281,71 @@ int vx_iso_pre_diffusion_ts_f64_noalloc(void* stream, const vx_grid* grid, const
                                         double* K_11, double* K_22, double* K_33,
                                         double* dtemp_iso, double* dsalt_iso,
                                         const double* int_drhodT, const double* int_drhodS, double* P_diss);
+
+/* ------------------------------------------------------------------------
+ * Implicit vertical integration of TKE, EKE and internal-wave energy: the
+ * remaining solve_implicit callers (veros/core/tke.py:227, eke.py:157,
+ * idemix.py:167), each with its coefficient build, dissipation diagnostic
+ * and lateral diffusion in one call. The advection branches of the
+ * reference kernels are not covered (callers reject them). Energies are
+ * time-level arrays (grid->time_layout); 3-D fields (X, Y, nz); surface and
+ * bottom fluxes (X, Y). The forcing is given as terms summed left to right,
+ * forcing = sign[0]*arr[0] + sign[1]*arr[1] + ..., in the order the
+ * reference's settings branches add them.
+ * ------------------------------------------------------------------------ */
+typedef struct vx_terms {
+    const double* arr[12];
+    double sign[12];
+    int32_t n;
+    int32_t _pad;
+} vx_terms;
+
+typedef struct vx_tke_params {
+    double dt_mom, alpha_tke, c_eps, K_h_tke;
+    int32_t hor_diffusion;  /* enable_tke_hor_diffusion */
+    int32_t _pad;
+} vx_tke_params;
+
+/* integrate_tke_kernel (veros/core/tke.py:123-321): tke[..., taup1] on the
+ * wet cells of the interior columns, then the surface correction
+ * tke_surf_corr and max(0, .) of the top level (every interior column),
+ * then the lateral diffusion (hor_diffusion); tke_diss over the whole array
+ * from the solved, unclamped tke. */
+int vx_integrate_tke_f64(void* stream, const vx_grid* grid, const vx_tke_params* prm, double* tke,
+                         int32_t tau, int32_t taup1, const double* kappaM, const double* sqrttke,
+                         const double* mxl, const double* forc_tke_surface, const vx_terms* forc,
+                         double* tke_surf_corr, double* tke_diss);
+
+typedef struct vx_eke_params {
+    double dt_tracer, alpha_eke, eke_c_eps;
+} vx_eke_params;
+
+/* integrate_eke_kernel (veros/core/eke.py:93-255): eke[..., taup1], then
+ * the lateral diffusion with 0.5*max(500, K_gm + K_gm'); eke_diss_iw =
+ * c_int * eke[taup1] and eke_diss_tke = 0 over the whole array. */
+int vx_integrate_eke_f64(void* stream, const vx_grid* grid, const vx_eke_params* prm, double* eke,
+                         int32_t tau, int32_t taup1, const double* kappaM, const double* sqrteke,
+                         const double* eke_len, const double* K_gm, const vx_terms* forc,
+                         double* eke_diss_iw, double* eke_diss_tke);
+
+typedef struct vx_idemix_params {
+    double dt_tracer, tau_v, tau_h, eke_diss_surfbot_frac;
+    int32_t hor_diffusion;   /* enable_idemix_hor_diffusion */
+    int32_t eke_diss_mode;   /* 0: forc as given; 1: its column integral at the
+                                bottom cell (enable_eke_diss_bottom); 2: split
+                                between bottom and surface (enable_eke_diss_surfbot) */
+} vx_idemix_params;
+
+/* integrate_idemix_kernel (veros/core/idemix.py:47-265): E_iw[..., taup1]
+ * with the edge right-hand side d_edge = d + dt*forc_iw_bottom/dzw, then the
+ * lateral diffusion (hor_diffusion); iw_diss = alpha_c * max(0, E_iw[tau]) *
+ * E_iw[taup1] over the whole array. `forc` is moved per eke_diss_mode, then
+ * `forc_post` (nullable: bottom friction) is added. */
+int vx_integrate_idemix_f64(void* stream, const vx_grid* grid, const vx_idemix_params* prm, double* E_iw,
+                            int32_t tau, int32_t taup1, const double* c0, const double* v0,
+                            const double* alpha_c, const double* forc_iw_bottom,
+                            const double* forc_iw_surface, const vx_terms* forc,
+                            const vx_terms* forc_post, double* iw_diss);
 
 /*
  * vertmix_tempsalt (veros/core/thermodynamics.py:248-300): implicit vertical

@@ -48,6 +48,24 @@ class VxIsoParams(ctypes.Structure):
         "drhodT", "drhodS", "K_iso_steep", "iso_slopec", "iso_dslope", "dt_tracer", "grav", "rho_0")]
 
 
+class VxTerms(ctypes.Structure):
+    _fields_ = [("arr", c_void_p * 12), ("sign", c_f64 * 12), ("n", c_i32), ("_pad", c_i32)]
+
+
+class VxTkeParams(ctypes.Structure):
+    _fields_ = [("dt_mom", c_f64), ("alpha_tke", c_f64), ("c_eps", c_f64), ("K_h_tke", c_f64),
+                ("hor_diffusion", c_i32), ("_pad", c_i32)]
+
+
+class VxEkeParams(ctypes.Structure):
+    _fields_ = [("dt_tracer", c_f64), ("alpha_eke", c_f64), ("eke_c_eps", c_f64)]
+
+
+class VxIdemixParams(ctypes.Structure):
+    _fields_ = [("dt_tracer", c_f64), ("tau_v", c_f64), ("tau_h", c_f64), ("eke_diss_surfbot_frac", c_f64),
+                ("hor_diffusion", c_i32), ("eke_diss_mode", c_i32)]
+
+
 # name -> (restype, argtypes)
 _SIGNATURES = {
     "vx_version": (ctypes.c_char_p, []),
@@ -90,6 +108,15 @@ _SIGNATURES = {
                                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_i32]),
     "vx_implicit_vert_friction_f64": (ctypes.c_int, [c_void_p, ctypes.POINTER(VxGrid), c_f64, c_void_p, c_void_p,
                                                      c_i32, c_i32] + [c_void_p] * 6),
+    "vx_integrate_tke_f64": (ctypes.c_int, [c_void_p, ctypes.POINTER(VxGrid), ctypes.POINTER(VxTkeParams), c_void_p,
+                                            c_i32, c_i32] + [c_void_p] * 4 + [ctypes.POINTER(VxTerms), c_void_p,
+                                                                              c_void_p]),
+    "vx_integrate_eke_f64": (ctypes.c_int, [c_void_p, ctypes.POINTER(VxGrid), ctypes.POINTER(VxEkeParams), c_void_p,
+                                            c_i32, c_i32] + [c_void_p] * 4 + [ctypes.POINTER(VxTerms), c_void_p,
+                                                                              c_void_p]),
+    "vx_integrate_idemix_f64": (ctypes.c_int, [c_void_p, ctypes.POINTER(VxGrid), ctypes.POINTER(VxIdemixParams),
+                                               c_void_p, c_i32, c_i32] + [c_void_p] * 5
+                                + [ctypes.POINTER(VxTerms), ctypes.POINTER(VxTerms), c_void_p]),
     "vx_iso_adjoint_workspace_bytes": (c_i64, [ctypes.POINTER(VxGrid)]),
     "vx_iso_diffusion_adjoint_f64": (ctypes.c_int, [c_void_p, ctypes.POINTER(VxGrid),
                                                     ctypes.POINTER(VxIsoParams), c_void_p, c_i32]

@@ -1,0 +1,595 @@
+// Implicit vertical integration of TKE, EKE and internal-wave energy on
+// gfx950: integrate_tke_kernel (veros/core/tke.py:123-321),
+// integrate_eke_kernel (veros/core/eke.py:93-255) and integrate_idemix_kernel
+// (veros/core/idemix.py:47-265), without their advection branches.
+//
+// One kernel per model. A group of kSegP lanes owns one (i, j) column of the
+// whole (X, Y) grid and lane s holds levels [s*S, s*S + S) (tdma_seg.hpp):
+// each lane sums the forcing terms, forms its rows of the implicit system
+// (delta from the diffusivity at k and k+1, the dissipation on the diagonal,
+// the edge row at kbot-1, the surface flux in the top rhs) in registers,
+// the group solves the column by the segmented elimination, and each lane
+// then writes its levels: the new energy at taup1 on the wet levels of the
+// interior columns, the dissipation diagnostic over the whole array, the
+// lateral diffusion of the tau level added on top (its fluxes read the four
+// neighbouring columns), and for TKE the surface correction. Halo columns
+// only produce the diagnostics. Masks come from kbot (maskT = maskW = wet
+// suffix, maskU / maskV the products of the two adjacent maskT).
+#include "tdma_seg.hpp"
+
+namespace vx {
+
+constexpr int kMaxTerms = 12;
+
+struct TurbGrid {
+    const int32_t* kbot;
+    const double *dxt, *dxu, *dyt, *dyu, *cost, *cosu, *dzt, *dzw;
+    int32_t X, Y, nz;
+    int64_t cs, ls;  // time levels: level l of cell c at l * ls + c * cs
+};
+
+// forcing = sgn[0] * p[0] + sgn[1] * p[1] + ... , summed left to right
+struct Terms {
+    const double* p[kMaxTerms];
+    double sgn[kMaxTerms];
+    int n;
+};
+
+__device__ __forceinline__ double sum_terms(const Terms& T, int64_t c) {
+    double f = 0.0;
+    for (int q = 0; q < T.n; ++q) f = q == 0 ? T.sgn[0] * T.p[0][c] : f + T.sgn[q] * T.p[q][c];
+    return f;
+}
+
+__device__ __forceinline__ bool wet_at(int kb, int k) { return kb > 0 && k >= kb - 1; }
+
+// The column of a lane group, over the whole (X, Y) grid.
+struct TurbCol {
+    int i, j, s, lo;
+    int64_t col;    // i * Y + j
+    bool valid;     // inside the grid
+    bool interior;  // 2 <= i < X-2, 2 <= j < Y-2
+    int kb;         // kbot
+    int k0;         // first solved level (nz: none)
+};
+
+__device__ __forceinline__ TurbCol turb_col(const TurbGrid& g, int S) {
+    TurbCol t;
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t col = gt / kSegP;
+    t.s = (int)(gt % kSegP);
+    t.lo = t.s * S;
+    t.valid = col < (int64_t)g.X * g.Y;
+    t.col = t.valid ? col : 0;
+    t.i = (int)(t.col / g.Y);
+    t.j = (int)(t.col - (int64_t)t.i * g.Y);
+    t.interior = t.valid && t.i >= 2 && t.i < g.X - 2 && t.j >= 2 && t.j < g.Y - 2;
+    t.kb = t.valid ? g.kbot[t.col] : 0;
+    t.k0 = (t.interior && t.kb >= 1 && t.kb <= g.nz) ? t.kb - 1 : g.nz;
+    return t;
+}
+
+// Lateral diffusion of the tau level at (i, j, k) of an interior column
+// (veros/core/tke.py:250-276; eke.py:168-200; idemix.py:186-218):
+//   fe(i) = ce(i) * (F(i+1) - F(i)) / (cost dxu(i)) * maskU(i)
+//   fn(j) = cn(j) * (F(j+1) - F(j)) / dyu(j) * maskV(j) * cosu(j)
+//   div   = (fe(i) - fe(i-1)) / (cost dxt) + (fn(j) - fn(j-1)) / (cost dyt)
+// F(ii, jj) and the coefficients come from functors of the cell offset.
+template <class FV, class CE, class CN>
+__device__ __forceinline__ double lateral_div(const TurbGrid& g, int i, int j, int k, FV F, CE ce, CN cn) {
+    const int64_t Y = g.Y, nz = g.nz;
+    const int64_t c = ((int64_t)i * Y + j) * nz + k;
+    const int64_t cE = c + Y * nz, cW = c - Y * nz, cN = c + nz, cS = c - nz;
+    const bool mC = wet_at(g.kbot[(int64_t)i * Y + j], k);
+    const bool mE = wet_at(g.kbot[(int64_t)(i + 1) * Y + j], k), mW = wet_at(g.kbot[(int64_t)(i - 1) * Y + j], k);
+    const bool mN = wet_at(g.kbot[(int64_t)i * Y + j + 1], k), mS = wet_at(g.kbot[(int64_t)i * Y + j - 1], k);
+    const double cost = g.cost[j];
+    const double Fc = F(c), Fe = F(cE), Fw = F(cW), Fn = F(cN), Fs = F(cS);
+    const double fe = ce(c, cE) * (Fe - Fc) / (cost * g.dxu[i]) * ((mC && mE) ? 1.0 : 0.0);
+    const double fw = ce(cW, c) * (Fc - Fw) / (cost * g.dxu[i - 1]) * ((mW && mC) ? 1.0 : 0.0);
+    const double fn = cn(c, cN) * (Fn - Fc) / g.dyu[j] * ((mC && mN) ? 1.0 : 0.0) * g.cosu[j];
+    const double fs = cn(cS, c) * (Fc - Fs) / g.dyu[j - 1] * ((mS && mC) ? 1.0 : 0.0) * g.cosu[j - 1];
+    return (fe - fw) / (cost * g.dxt[i]) + (fn - fs) / (cost * g.dyt[j]);
+}
+
+// levels lo-1 .. lo+S of a (X, Y, nz) array at column `base` (0 outside [0, nz))
+template <int S>
+__device__ __forceinline__ void load_halo(const double* __restrict__ p, int64_t base, int lo, int nz,
+                                          double (&v)[S + 2]) {
+#pragma unroll
+    for (int t = 0; t < S + 2; ++t) {
+        const int k = lo - 1 + t;
+        v[t] = (k >= 0 && k < nz) ? p[base + k] : 0.0;
+    }
+}
+
+template <int S>
+__device__ __forceinline__ void load_lv(const double* __restrict__ p, int64_t base, int lo, int nz, double (&v)[S]) {
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+        const int k = lo + t;
+        v[t] = k < nz ? p[base + k] : 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// TKE (veros/core/tke.py:123-321)
+// ---------------------------------------------------------------------------
+struct TkeArgs {
+    TurbGrid g;
+    double* tke;  // time levels
+    int tau, taup1;
+    const double *kappaM, *sqrttke, *mxl, *forc_surf;
+    Terms forc;
+    double *surf_corr, *diss;
+    double dt, alpha, c_eps, K_h;
+    int hor;
+};
+
+template <int S>
+__global__ __launch_bounds__(256) void tke_kernel(TkeArgs a) {
+    const TurbGrid& g = a.g;
+    const TurbCol tc = turb_col(g, S);
+    const int nz = g.nz;
+    const int64_t base = tc.col * nz;
+    double km[S + 2];
+    load_halo<S>(a.kappaM, base, tc.lo, nz, km);
+    double sq[S], ml[S];
+    load_lv<S>(a.sqrttke, base, tc.lo, nz, sq);
+    load_lv<S>(a.mxl, base, tc.lo, nz, ml);
+    const double* __restrict__ E0 = a.tke + a.tau * g.ls;
+    double* __restrict__ E1 = a.tke + a.taup1 * g.ls;
+    const double dt = a.dt;
+    double va[S], vb[S], vc[S], vd[S];
+    auto delta = [&](int t) {  // delta at level lo - 1 + t
+        const int k = tc.lo - 1 + t;
+        return (k >= 0 && k < nz - 1) ? dt / g.dzt[k + 1] * a.alpha * 0.5 * (km[t] + km[t + 1]) : 0.0;
+    };
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+        const int k = tc.lo + t;
+        const bool w = tc.valid && k < nz && k >= tc.k0;
+        double A = 0.0, B = 1.0, C = 0.0, D = 0.0;
+        if (w) {
+            const double dk = delta(t + 1), dm = delta(t);
+            const double forc = sum_terms(a.forc, base + k);
+            const double dzw = g.dzw[k];
+            D = E0[(base + k) * g.cs] + dt * forc;
+            if (k == tc.k0) {
+                A = 0.0;
+                B = 1 + dk / dzw + dt * a.c_eps / ml[t] * sq[t];
+            } else if (k == nz - 1) {
+                A = -dm / (0.5 * dzw);
+                B = 1 + dm / (0.5 * dzw) + dt * a.c_eps / ml[t] * sq[t];
+            } else {
+                A = -dm / dzw;
+                B = 1 + (dk + dm) / dzw + dt * a.c_eps * sq[t] / ml[t];
+            }
+            C = k < nz - 1 ? -dk / dzw : 0.0;
+            if (k == nz - 1) D += dt * a.forc_surf[tc.col] / (0.5 * dzw);
+        }
+        va[t] = A;
+        vb[t] = B;
+        vc[t] = C;
+        vd[t] = D;
+    }
+    SegLane L;
+    L.s = tc.s;
+    L.lo = tc.lo;
+    L.k0 = tc.k0;
+    L.valid = tc.valid;
+    L.base = base;
+    seg_solve_f64<S>(L, va, vb, vc, vd);
+    if (!tc.valid) return;
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+        const int k = tc.lo + t;
+        if (k >= nz) continue;
+        const int64_t c = base + k;
+        const bool w = k >= tc.k0;
+        double v = w ? vd[t] : E1[c * g.cs];
+        a.diss[c] = a.c_eps / ml[t] * sq[t] * v;  // tke_diss over the whole array (:230)
+        if (!tc.interior) continue;
+        bool store = w;
+        if (k == nz - 1) {  // surface correction and clamp of every interior column (:235-244)
+            a.surf_corr[tc.col] = v < 0.0 ? -v * 0.5 * g.dzw[k] / dt : 0.0;
+            v = fmax(0.0, v);
+            store = true;
+        }
+        if (a.hor && wet_at(tc.kb, k)) {
+            const double Kh = a.K_h;
+            v += dt * 1.0 *
+                 lateral_div(
+                     g, tc.i, tc.j, k, [&](int64_t cc) { return E0[cc * g.cs]; },
+                     [&](int64_t, int64_t) { return Kh; }, [&](int64_t, int64_t) { return Kh; });
+            store = true;
+        }
+        if (store) E1[c * g.cs] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// EKE (veros/core/eke.py:93-255)
+// ---------------------------------------------------------------------------
+struct EkeArgs {
+    TurbGrid g;
+    double* eke;
+    int tau, taup1;
+    const double *kappaM, *sqrteke, *eke_len, *K_gm;
+    Terms forc;
+    double *diss_iw, *diss_tke;
+    double dt, alpha, c_eps;
+};
+
+template <int S>
+__global__ __launch_bounds__(256) void eke_kernel(EkeArgs a) {
+    const TurbGrid& g = a.g;
+    const TurbCol tc = turb_col(g, S);
+    const int nz = g.nz;
+    const int64_t base = tc.col * nz;
+    double km[S + 2];
+    load_halo<S>(a.kappaM, base, tc.lo, nz, km);
+    double sq[S], el[S];
+    load_lv<S>(a.sqrteke, base, tc.lo, nz, sq);
+    load_lv<S>(a.eke_len, base, tc.lo, nz, el);
+    double cint[S];  // eke_c_eps * sqrteke / eke_len * maskW (:121)
+#pragma unroll
+    for (int t = 0; t < S; ++t) cint[t] = a.c_eps * sq[t] / el[t] * (wet_at(tc.kb, tc.lo + t) ? 1.0 : 0.0);
+    const double* __restrict__ E0 = a.eke + a.tau * g.ls;
+    double* __restrict__ E1 = a.eke + a.taup1 * g.ls;
+    const double dt = a.dt;
+    double va[S], vb[S], vc[S], vd[S];
+    auto delta = [&](int t) {
+        const int k = tc.lo - 1 + t;
+        return (k >= 0 && k < nz - 1) ? dt / g.dzt[k + 1] * 0.5 * (km[t] + km[t + 1]) * a.alpha : 0.0;
+    };
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+        const int k = tc.lo + t;
+        const bool w = tc.valid && k < nz && k >= tc.k0;
+        double A = 0.0, B = 1.0, C = 0.0, D = 0.0;
+        if (w) {
+            const double dk = delta(t + 1), dm = delta(t);
+            const double dzw = g.dzw[k];
+            D = E0[(base + k) * g.cs] + dt * sum_terms(a.forc, base + k);
+            if (k == tc.k0) {
+                B = 1 + dk / dzw + dt * cint[t];
+            } else if (k == nz - 1) {
+                A = -dm / (0.5 * dzw);
+                B = 1 + dm / (0.5 * dzw) + dt * cint[t];
+            } else {
+                A = -dm / dzw;
+                B = 1 + (dk + dm) / dzw + dt * cint[t];
+            }
+            C = k < nz - 1 ? -dk / dzw : 0.0;
+        }
+        va[t] = A;
+        vb[t] = B;
+        vc[t] = C;
+        vd[t] = D;
+    }
+    SegLane L;
+    L.s = tc.s;
+    L.lo = tc.lo;
+    L.k0 = tc.k0;
+    L.valid = tc.valid;
+    L.base = base;
+    seg_solve_f64<S>(L, va, vb, vc, vd);
+    if (!tc.valid) return;
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+        const int k = tc.lo + t;
+        if (k >= nz) continue;
+        const int64_t c = base + k;
+        const bool w = k >= tc.k0;
+        double v = w ? vd[t] : E1[c * g.cs];
+        a.diss_iw[c] = cint[t] * v;  // (:151)
+        a.diss_tke[c] = 0.0;         // (:152)
+        if (!tc.interior || !wet_at(tc.kb, k)) continue;
+        const double* __restrict__ K = a.K_gm;
+        v += dt * 1.0 *
+             lateral_div(
+                 g, tc.i, tc.j, k, [&](int64_t cc) { return E0[cc * g.cs]; },
+                 [&](int64_t c0, int64_t c1) { return 0.5 * fmax(500.0, K[c0] + K[c1]); },
+                 [&](int64_t c0, int64_t c1) { return 0.5 * fmax(500.0, K[c0] + K[c1]); });
+        E1[c * g.cs] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// IDEMIX (veros/core/idemix.py:47-265)
+// ---------------------------------------------------------------------------
+struct IdemixArgs {
+    TurbGrid g;
+    double* E;
+    int tau, taup1;
+    const double *c0, *v0, *alpha_c, *forc_bot, *forc_surf;
+    Terms forc;       // before the optional move to the bottom / surface cells
+    Terms forc_post;  // added afterwards (bottom friction)
+    double* iw_diss;
+    double dt, tau_v, tau_h, frac;
+    int hor;
+    int diss_mode;  // 0: as is; 1: column integral at the bottom cell; 2: split bottom / surface
+};
+
+template <int S>
+__global__ __launch_bounds__(256) void idemix_kernel(IdemixArgs a) {
+    const TurbGrid& g = a.g;
+    const TurbCol tc = turb_col(g, S);
+    const int nz = g.nz;
+    const int64_t base = tc.col * nz;
+    double c0[S + 2];
+    load_halo<S>(a.c0, base, tc.lo, nz, c0);
+    double al[S];
+    load_lv<S>(a.alpha_c, base, tc.lo, nz, al);
+    const double* __restrict__ E0 = a.E + a.tau * g.ls;
+    double* __restrict__ E1 = a.E + a.taup1 * g.ls;
+    const double dt = a.dt;
+    double mx[S];  // max(0, E_iw[tau]) (:118)
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+        const int k = tc.lo + t;
+        mx[t] = (tc.valid && k < nz) ? fmax(0.0, E0[(base + k) * g.cs]) : 0.0;
+    }
+    // the forcing of the lane's levels (interior columns; wet levels feed the
+    // solve, the column integral sums maskW-weighted levels)
+    double fo[S];
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+        const int k = tc.lo + t;
+        fo[t] = (tc.interior && k < nz && wet_at(tc.kb, k)) ? sum_terms(a.forc, base + k) : 0.0;
+    }
+    if (a.diss_mode != 0) {  // (:70-103)
+        double part = 0.0;
+#pragma unroll
+        for (int t = 0; t < S; ++t) {
+            const int k = tc.lo + t;
+            if (k < nz - 1) part += g.dzw[k] * fo[t];
+            else if (k == nz - 1) part += 0.5 * fo[t] * g.dzw[k];
+        }
+        part += __shfl_xor(part, 1, kSegP);
+        part += __shfl_xor(part, 2, kSegP);
+        part += __shfl_xor(part, 4, kSegP);
+        const double a_loc = part;
+        const int ks = tc.kb - 1 > 0 ? tc.kb - 1 : 0;
+#pragma unroll
+        for (int t = 0; t < S; ++t) {
+            const int k = tc.lo + t;
+            double f = 0.0;
+            if (k == ks) f = (a.diss_mode == 1 ? 1.0 : a.frac) * a_loc / g.dzw[k];
+            if (a.diss_mode == 2 && k == nz - 1) f = (1.0 - a.frac) * a_loc / (0.5 * g.dzw[k]);
+            fo[t] = f;
+        }
+        // the moved term applies to every interior column; rows of other
+        // columns are never solved
+    }
+    if (a.forc_post.n > 0) {
+#pragma unroll
+        for (int t = 0; t < S; ++t) {
+            const int k = tc.lo + t;
+            if (tc.interior && k < nz) {
+                const int64_t c = base + k;
+                double f = fo[t];
+                for (int q = 0; q < a.forc_post.n; ++q) f = f + a.forc_post.sgn[q] * a.forc_post.p[q][c];
+                fo[t] = f;
+            }
+        }
+    }
+    double va[S], vb[S], vc[S], vd[S];
+    auto delta = [&](int t) {
+        const int k = tc.lo - 1 + t;
+        return (k >= 0 && k < nz - 1) ? dt * a.tau_v / g.dzt[k + 1] * 0.5 * (c0[t] + c0[t + 1]) : 0.0;
+    };
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+        const int k = tc.lo + t;
+        const bool w = tc.valid && k < nz && k >= tc.k0;
+        double A = 0.0, B = 1.0, C = 0.0, D = 0.0;
+        if (w) {
+            const double dk = delta(t + 1), dm = delta(t);
+            const double dzw = g.dzw[k];
+            const double diss = dt * al[t] * mx[t];
+            const double cc = c0[t + 1];
+            D = E0[(base + k) * g.cs] + dt * fo[t];
+            if (k == tc.k0) {
+                B = 1 + dk / dzw * cc + diss;
+                D = D + dt * a.forc_bot[tc.col] / dzw;  // d_edge (:157-159)
+            } else if (k == nz - 1) {
+                A = -dm / (0.5 * dzw) * c0[t];
+                B = 1 + dm / (0.5 * dzw) * cc + diss;
+                D += dt * a.forc_surf[tc.col] / (0.5 * dzw);
+            } else {
+                A = -dm * c0[t] / dzw;
+                B = 1 + dk * cc / dzw + dm * cc / dzw + diss;
+            }
+            C = k < nz - 1 ? -dk / dzw * c0[t + 2] : 0.0;
+        }
+        va[t] = A;
+        vb[t] = B;
+        vc[t] = C;
+        vd[t] = D;
+    }
+    SegLane L;
+    L.s = tc.s;
+    L.lo = tc.lo;
+    L.k0 = tc.k0;
+    L.valid = tc.valid;
+    L.base = base;
+    seg_solve_f64<S>(L, va, vb, vc, vd);
+    if (!tc.valid) return;
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+        const int k = tc.lo + t;
+        if (k >= nz) continue;
+        const int64_t c = base + k;
+        const bool w = k >= tc.k0;
+        double v = w ? vd[t] : E1[c * g.cs];
+        a.iw_diss[c] = al[t] * mx[t] * v;  // (:174)
+        if (!tc.interior) continue;
+        bool store = w;
+        if (a.hor && wet_at(tc.kb, k)) {
+            const double* __restrict__ V = a.v0;
+            const double th = a.tau_h;
+            v += dt * 1.0 *
+                 lateral_div(
+                     g, tc.i, tc.j, k, [&](int64_t cc) { return V[cc] * E0[cc * g.cs]; },
+                     [&](int64_t c0_, int64_t c1) { return th * 0.5 * (V[c1] + V[c0_]); },
+                     [&](int64_t c0_, int64_t c1) { return th * 0.5 * (V[c1] + V[c0_]); });
+            store = true;
+        }
+        if (store) E1[c * g.cs] = v;
+    }
+}
+
+namespace {
+
+int make_turb_grid(const vx_grid* in, TurbGrid& g, const char* who) {
+    if (!in) return fail(VX_ERR_INVALID_ARGUMENT, "%s: null grid", who);
+    if (in->nx < 1 || in->ny < 1 || in->nz < 2)
+        return fail(VX_ERR_INVALID_ARGUMENT, "%s: need nx, ny >= 1 and nz >= 2", who);
+    if (in->nz > kMaxNz) return fail(VX_ERR_UNSUPPORTED, "%s: nz=%d exceeds %d", who, in->nz, kMaxNz);
+    if (!in->kbot || !in->dxt || !in->dxu || !in->dyt || !in->dyu || !in->cost || !in->cosu || !in->dzt || !in->dzw)
+        return fail(VX_ERR_INVALID_ARGUMENT, "%s: null grid array", who);
+    if (in->time_layout != VX_LEVELS_INNER && in->time_layout != VX_LEVELS_OUTER)
+        return fail(VX_ERR_INVALID_ARGUMENT, "%s: time_layout must be VX_LEVELS_INNER or VX_LEVELS_OUTER", who);
+    g.kbot = in->kbot;
+    g.dxt = in->dxt, g.dxu = in->dxu, g.dyt = in->dyt, g.dyu = in->dyu;
+    g.cost = in->cost, g.cosu = in->cosu, g.dzt = in->dzt, g.dzw = in->dzw;
+    g.X = (int32_t)(in->nx + 4);
+    g.Y = (int32_t)(in->ny + 4);
+    g.nz = in->nz;
+    const bool outer = in->time_layout == VX_LEVELS_OUTER;
+    g.cs = outer ? 1 : 3;
+    g.ls = outer ? (int64_t)g.X * g.Y * g.nz : 1;
+    return VX_OK;
+}
+
+int make_terms(const vx_terms* in, Terms& t, bool allow_empty, const char* who) {
+    t.n = 0;
+    if (!in) {
+        if (allow_empty) return VX_OK;
+        return fail(VX_ERR_INVALID_ARGUMENT, "%s: null forcing terms", who);
+    }
+    if (in->n < (allow_empty ? 0 : 1) || in->n > kMaxTerms)
+        return fail(VX_ERR_INVALID_ARGUMENT, "%s: forcing needs 1..%d terms (got %d)", who, kMaxTerms, in->n);
+    for (int q = 0; q < in->n; ++q) {
+        if (!in->arr[q]) return fail(VX_ERR_INVALID_ARGUMENT, "%s: null forcing term %d", who, q);
+        t.p[q] = in->arr[q];
+        t.sgn[q] = in->sign[q];
+    }
+    t.n = in->n;
+    return VX_OK;
+}
+
+template <class Args, template <int> class K>
+int launch_turb(const Args& a, const TurbGrid& g, hipStream_t s, const char* who) {
+    const int S = seg_levels(g.nz);
+    const int64_t threads = (int64_t)g.X * g.Y * kSegP;
+    const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
+#define VX_TURB_CASE(N) \
+    case N: hipLaunchKernelGGL(K<N>::fn, grid, block, 0, s, a); break;
+    switch (S) {
+        VX_TURB_CASE(2)
+        VX_TURB_CASE(4)
+        VX_TURB_CASE(6)
+        VX_TURB_CASE(8)
+        VX_TURB_CASE(10)
+        VX_TURB_CASE(12)
+        VX_TURB_CASE(14)
+        VX_TURB_CASE(16)
+        default: return fail(VX_ERR_UNSUPPORTED, "%s: nz=%d", who, g.nz);
+    }
+#undef VX_TURB_CASE
+    return check_launch(who);
+}
+
+template <int N>
+struct TkeK {
+    static constexpr auto fn = tke_kernel<N>;
+};
+template <int N>
+struct EkeK {
+    static constexpr auto fn = eke_kernel<N>;
+};
+template <int N>
+struct IdemixK {
+    static constexpr auto fn = idemix_kernel<N>;
+};
+
+bool levels_ok(int tau, int taup1) { return tau >= 0 && tau <= 2 && taup1 >= 0 && taup1 <= 2 && tau != taup1; }
+
+}  // namespace
+}  // namespace vx
+
+extern "C" int vx_integrate_tke_f64(void* stream, const vx_grid* grid, const vx_tke_params* prm, double* tke,
+                                    int32_t tau, int32_t taup1, const double* kappaM, const double* sqrttke,
+                                    const double* mxl, const double* forc_tke_surface, const vx_terms* forc,
+                                    double* tke_surf_corr, double* tke_diss) {
+    using namespace vx;
+    const char* who = "vx_integrate_tke_f64";
+    TkeArgs a{};
+    int rc = make_turb_grid(grid, a.g, who);
+    if (rc) return rc;
+    if (!prm) return fail(VX_ERR_INVALID_ARGUMENT, "%s: null parameters", who);
+    if (!levels_ok(tau, taup1)) return fail(VX_ERR_INVALID_ARGUMENT, "%s: tau, taup1 must be distinct in 0..2", who);
+    if (!tke || !kappaM || !sqrttke || !mxl || !forc_tke_surface || !tke_surf_corr || !tke_diss)
+        return fail(VX_ERR_INVALID_ARGUMENT, "%s: null array pointer", who);
+    if ((rc = make_terms(forc, a.forc, false, who))) return rc;
+    if (!(prm->dt_mom > 0.0)) return fail(VX_ERR_INVALID_ARGUMENT, "%s: dt_mom must be positive", who);
+    a.tke = tke, a.tau = tau, a.taup1 = taup1;
+    a.kappaM = kappaM, a.sqrttke = sqrttke, a.mxl = mxl, a.forc_surf = forc_tke_surface;
+    a.surf_corr = tke_surf_corr, a.diss = tke_diss;
+    a.dt = prm->dt_mom, a.alpha = prm->alpha_tke, a.c_eps = prm->c_eps, a.K_h = prm->K_h_tke;
+    a.hor = prm->hor_diffusion != 0;
+    return launch_turb<TkeArgs, TkeK>(a, a.g, (hipStream_t)stream, who);
+}
+
+extern "C" int vx_integrate_eke_f64(void* stream, const vx_grid* grid, const vx_eke_params* prm, double* eke,
+                                    int32_t tau, int32_t taup1, const double* kappaM, const double* sqrteke,
+                                    const double* eke_len, const double* K_gm, const vx_terms* forc,
+                                    double* eke_diss_iw, double* eke_diss_tke) {
+    using namespace vx;
+    const char* who = "vx_integrate_eke_f64";
+    EkeArgs a{};
+    int rc = make_turb_grid(grid, a.g, who);
+    if (rc) return rc;
+    if (!prm) return fail(VX_ERR_INVALID_ARGUMENT, "%s: null parameters", who);
+    if (!levels_ok(tau, taup1)) return fail(VX_ERR_INVALID_ARGUMENT, "%s: tau, taup1 must be distinct in 0..2", who);
+    if (!eke || !kappaM || !sqrteke || !eke_len || !K_gm || !eke_diss_iw || !eke_diss_tke)
+        return fail(VX_ERR_INVALID_ARGUMENT, "%s: null array pointer", who);
+    if ((rc = make_terms(forc, a.forc, false, who))) return rc;
+    if (!(prm->dt_tracer > 0.0)) return fail(VX_ERR_INVALID_ARGUMENT, "%s: dt_tracer must be positive", who);
+    a.eke = eke, a.tau = tau, a.taup1 = taup1;
+    a.kappaM = kappaM, a.sqrteke = sqrteke, a.eke_len = eke_len, a.K_gm = K_gm;
+    a.diss_iw = eke_diss_iw, a.diss_tke = eke_diss_tke;
+    a.dt = prm->dt_tracer, a.alpha = prm->alpha_eke, a.c_eps = prm->eke_c_eps;
+    return launch_turb<EkeArgs, EkeK>(a, a.g, (hipStream_t)stream, who);
+}
+
+extern "C" int vx_integrate_idemix_f64(void* stream, const vx_grid* grid, const vx_idemix_params* prm, double* E_iw,
+                                       int32_t tau, int32_t taup1, const double* c0, const double* v0,
+                                       const double* alpha_c, const double* forc_iw_bottom,
+                                       const double* forc_iw_surface, const vx_terms* forc,
+                                       const vx_terms* forc_post, double* iw_diss) {
+    using namespace vx;
+    const char* who = "vx_integrate_idemix_f64";
+    IdemixArgs a{};
+    int rc = make_turb_grid(grid, a.g, who);
+    if (rc) return rc;
+    if (!prm) return fail(VX_ERR_INVALID_ARGUMENT, "%s: null parameters", who);
+    if (!levels_ok(tau, taup1)) return fail(VX_ERR_INVALID_ARGUMENT, "%s: tau, taup1 must be distinct in 0..2", who);
+    if (!E_iw || !c0 || !v0 || !alpha_c || !forc_iw_bottom || !forc_iw_surface || !iw_diss)
+        return fail(VX_ERR_INVALID_ARGUMENT, "%s: null array pointer", who);
+    if ((rc = make_terms(forc, a.forc, false, who))) return rc;
+    if ((rc = make_terms(forc_post, a.forc_post, true, who))) return rc;
+    if (prm->eke_diss_mode < 0 || prm->eke_diss_mode > 2)
+        return fail(VX_ERR_INVALID_ARGUMENT, "%s: eke_diss_mode must be 0, 1 or 2", who);
+    if (!(prm->dt_tracer > 0.0)) return fail(VX_ERR_INVALID_ARGUMENT, "%s: dt_tracer must be positive", who);
+    a.E = E_iw, a.tau = tau, a.taup1 = taup1;
+    a.c0 = c0, a.v0 = v0, a.alpha_c = alpha_c, a.forc_bot = forc_iw_bottom, a.forc_surf = forc_iw_surface;
+    a.iw_diss = iw_diss;
+    a.dt = prm->dt_tracer, a.tau_v = prm->tau_v, a.tau_h = prm->tau_h, a.frac = prm->eke_diss_surfbot_frac;
+    a.hor = prm->hor_diffusion != 0;
+    a.diss_mode = prm->eke_diss_mode;
+    return launch_turb<IdemixArgs, IdemixK>(a, a.g, (hipStream_t)stream, who);
+}

@@ -86,7 +86,7 @@ def grid_struct(vs, keep, layout=LEVELS_INNER):
     collects the tensors it points into."""
     kbot = vs.kbot
     X, Y = kbot.shape
-    nz = vs.K_iso.shape[2]
+    nz = vs.dzt.shape[0]
     kb = kbot if (kbot.dtype == torch.int32 and kbot.is_contiguous()) else kbot.to(torch.int32).contiguous()
     _check_array("kbot", kb, (X, Y), torch.int32)
     g = _native.VxGrid()

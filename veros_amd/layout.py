@@ -23,7 +23,7 @@ from ._native import VX_LEVELS_INNER as LEVELS_INNER  # (X, Y, nz, 3) C-contiguo
 from ._native import VX_LEVELS_OUTER as LEVELS_OUTER  # the (X, Y, nz, 3) view of a (3, X, Y, nz) buffer
 
 # the time-level variables of the mixing path
-TIME_LEVEL_VARS = ("temp", "salt", "int_drhodT", "int_drhodS", "u", "v")
+TIME_LEVEL_VARS = ("temp", "salt", "int_drhodT", "int_drhodS", "u", "v", "tke", "eke", "E_iw")
 
 
 def levels_layout(t: torch.Tensor) -> int:
