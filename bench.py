@@ -122,6 +122,38 @@ def build_workload(nx, ny, nz, seed=1002, levels="outer"):
     vs["area_v"] = vs["cosu"][None, :] * vs["dyu"][None, :] * vs["dxt"][:, None]
     for k in ("du_mix", "dv_mix", "K_diss_v"):
         vs[k] = torch.zeros((X, Y, nz), **f64)
+    # TKE / EKE / IDEMIX inputs (veros_amd.workloads.add_turbulence_fields,
+    # drawn on the device): energies (X, Y, nz, 3), scales, IDEMIX parameters
+    # times maskW; the dissipation terms share four random fields
+    kbx = vs["kbot"].long()
+    kk = torch.arange(nz, device=dev)
+    maskW = ((kbx[..., None] >= 1) & (kk[None, None, :] >= kbx[..., None] - 1)).to(torch.float64)
+    del kbx
+    u3 = lambda lo_, hi_: lo_ + (hi_ - lo_) * torch.rand((X, Y, nz), generator=g, **f64)  # noqa: E731
+    vs["tke"] = 1e-5 + (1e-2 - 1e-5) * torch.rand((X, Y, nz, 3), generator=g, **f64)
+    vs["sqrttke"] = torch.sqrt(vs["tke"][..., 0])
+    vs["mxl"] = u3(1.0, 100.0)
+    vs["forc_tke_surface"] = 2e-5 * torch.randn((X, Y), generator=g, **f64)
+    vs["eke"] = 1e-4 + (1e-2 - 1e-4) * torch.rand((X, Y, nz, 3), generator=g, **f64)
+    vs["sqrteke"] = torch.sqrt(vs["eke"][..., 0])
+    vs["eke_len"] = u3(1e3, 1e5)
+    vs["K_gm"] = u3(150.0, 450.0)
+    vs["E_iw"] = -1e-5 + (1e-3 + 1e-5) * torch.rand((X, Y, nz, 3), generator=g, **f64)
+    vs["dE_iw"] = torch.zeros((X, Y, nz, 3), **f64)
+    vs["c0"] = u3(0.0, 1.0) * maskW
+    vs["v0"] = u3(0.0, 0.5) * maskW
+    vs["alpha_c"] = u3(0.0, 1e-3) * maskW
+    vs["forc_iw_bottom"] = 1e-6 * torch.rand((X, Y), generator=g, **f64)
+    vs["forc_iw_surface"] = 1e-6 * torch.rand((X, Y), generator=g, **f64)
+    shared = [1e-9 * torch.randn((X, Y, nz), generator=g, **f64) * maskW for _ in range(4)]
+    del maskW
+    for q, k in enumerate(("K_diss_v", "P_diss_v", "P_diss_adv", "P_diss_nonlin", "eke_diss_tke", "K_diss_bot",
+                           "K_diss_gm", "K_diss_h", "P_diss_skew", "P_diss_hmix")):
+        vs[k] = shared[q % 4]
+    for k in ("tke_diss", "eke_diss_iw", "iw_diss"):
+        vs[k] = 1e-9 * torch.randn((X, Y, nz), generator=g, **f64)
+    vs["eke_diss_tke_out"] = torch.zeros((X, Y, nz), **f64)
+    vs["tke_surf_corr"] = torch.zeros((X, Y), **f64)
     from types import SimpleNamespace
     state = SimpleNamespace(variables=SimpleNamespace(**vs), settings=settings)
     if levels == "outer":
@@ -469,6 +501,31 @@ def main():
     def run_friction():
         friction.implicit_vert_friction(state)
 
+    from veros_amd.workloads import turbulence_settings
+    turb_settings = {
+        # TKE with IDEMIX and EKE on (forcing K_diss_v - P_diss_v - P_diss_adv - P_diss_nonlin + eke_diss_tke +
+        # iw_diss) and lateral diffusion; EKE as veros runs it; IDEMIX with the EKE dissipation moved to the
+        # bottom cell and lateral diffusion
+        "tke": turbulence_settings(nz=nz, enable_eke=True, enable_idemix=True, enable_tke_hor_diffusion=True),
+        "eke": turbulence_settings(nz=nz, enable_eke=True),
+        "idemix": turbulence_settings(nz=nz, enable_eke=True, enable_eke_diss_bottom=True,
+                                      enable_idemix_hor_diffusion=True),
+    }
+
+    def run_turb(model):
+        from types import SimpleNamespace
+        from veros_amd import turbulence
+        st_ = SimpleNamespace(variables=vs, settings=turb_settings[model])
+        fn = {"tke": turbulence.integrate_tke_kernel, "eke": turbulence.integrate_eke_kernel,
+              "idemix": turbulence.integrate_idemix_kernel}[model]
+        if model == "eke":  # its eke_diss_tke output is a separate buffer from TKE's input
+            saved = vs.eke_diss_tke
+            vs.eke_diss_tke = vs.eke_diss_tke_out
+            fn(st_)
+            vs.eke_diss_tke = saved
+        else:
+            fn(st_)
+
     def run_diff_two_calls():
         isoneutral.isoneutral_diffusion(state, vs.temp, True)
         isoneutral.isoneutral_diffusion(state, vs.salt, False)
@@ -479,7 +536,9 @@ def main():
     for name, fn in (("tdma_fp64", lambda: run_tdma("fp64")), ("tdma_mixed", lambda: run_tdma("mixed")),
                      ("iso_pre", run_pre), ("iso_diffusion_TS", run_diff),
                      ("iso_diffusion_T_then_S", run_diff_two_calls), ("iso_pre_and_diffusion_TS", run_pre_diff),
-                     ("vertmix_TS", run_vertmix), ("friction_UV", run_friction)):
+                     ("vertmix_TS", run_vertmix), ("friction_UV", run_friction),
+                     ("tke", lambda: run_turb("tke")), ("eke", lambda: run_turb("eke")),
+                     ("idemix", lambda: run_turb("idemix"))):
         if only is not None and name not in only:
             continue
         for _ in range(args.warmup):
@@ -539,6 +598,13 @@ def main():
     # at taup1 written or kept (4 * 8), du_mix and dv_mix written (16),
     # K_diss_v read and written (16), over the X*Y*nz array
     B_fric = 72 * cols_halo * nz
+    # TKE / EKE / IDEMIX, over the X*Y*nz array, 8 B per array and cell: TKE
+    # reads kappaM, sqrttke, mxl, tke[tau], tke[taup1] and 6 forcing terms and
+    # writes tke[taup1], tke_diss (13); EKE reads kappaM, sqrteke, eke_len,
+    # K_gm, eke[tau], eke[taup1] and 5 terms and writes eke[taup1],
+    # eke_diss_iw, eke_diss_tke (14); IDEMIX reads c0, v0, alpha_c, E_iw[tau],
+    # E_iw[taup1] and 2 terms and writes E_iw[taup1], iw_diss (9)
+    B_turb = {"tke": 13 * 8 * cols_halo * nz, "eke": 14 * 8 * cols_halo * nz, "idemix": 9 * 8 * cols_halo * nz}
     kern = {
         "tdma_fp64": (B_td, parts["tdma_fp64"][0]),
         "tdma_mixed": (B_td, parts["tdma_mixed"][0]),
@@ -548,6 +614,7 @@ def main():
         "iso_pre_and_diffusion_TS": (B8_step, parts["iso_pre_and_diffusion_TS"][0]),
         "vertmix_TS": (B_vmix, parts["vertmix_TS"][0]),
         "friction_UV": (B_fric, parts["friction_UV"][0]),
+        **{m: (B_turb[m], parts[m][0]) for m in ("tke", "eke", "idemix")},
     }
     gbs = {k: b / (t * 1e-3) / 1e9 for k, (b, t) in kern.items()}
     # roofline: pre + the temperature + salinity diffusion in one C-ABI call,

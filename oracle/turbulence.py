@@ -246,14 +246,6 @@ def integrate_idemix_kernel(state, solver="dgtsv"):
 
 def turbulence_settings(**kw):
     """The settings the three kernels read, with veros's defaults
-    (veros/settings.py:28-128)."""
-    d = dict(nz=None, dt_mom=3600.0, dt_tracer=3600.0, alpha_tke=1.0, c_eps=0.7, K_h_tke=2000.0,
-             enable_tke_hor_diffusion=False, enable_tke_superbee_advection=False, enable_tke_upwind_advection=False,
-             enable_store_cabbeling_heat=False, enable_store_bottom_friction_tke=False, enable_eke=False,
-             enable_idemix=False, eke_c_eps=1.0, alpha_eke=1.0, enable_eke_superbee_advection=False,
-             enable_eke_upwind_advection=False, tau_v=2.0 * 86400.0, tau_h=15.0 * 86400.0,
-             enable_idemix_hor_diffusion=False, enable_eke_diss_bottom=False, enable_eke_diss_surfbot=False,
-             eke_diss_surfbot_frac=1.0, enable_idemix_superbee_advection=False,
-             enable_idemix_upwind_advection=False, enable_cyclic_x=True)
-    d.update(kw)
-    return SimpleNamespace(**d)
+    (veros/settings.py:28-128); the record lives with the workloads."""
+    from veros_amd.workloads import turbulence_settings as ts
+    return ts(**kw)

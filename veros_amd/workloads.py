@@ -415,6 +415,21 @@ def add_turbulence_fields(rng, state, *, random_aux=False):
     return state
 
 
+def turbulence_settings(**kw):
+    """The settings the TKE / EKE / IDEMIX integrations read, with veros's
+    defaults (veros/settings.py:28-128), overridden by ``kw``."""
+    d = dict(nz=None, dt_mom=3600.0, dt_tracer=3600.0, alpha_tke=1.0, c_eps=0.7, K_h_tke=2000.0,
+             enable_tke_hor_diffusion=False, enable_tke_superbee_advection=False, enable_tke_upwind_advection=False,
+             enable_store_cabbeling_heat=False, enable_store_bottom_friction_tke=False, enable_eke=False,
+             enable_idemix=False, eke_c_eps=1.0, alpha_eke=1.0, enable_eke_superbee_advection=False,
+             enable_eke_upwind_advection=False, tau_v=2.0 * 86400.0, tau_h=15.0 * 86400.0,
+             enable_idemix_hor_diffusion=False, enable_eke_diss_bottom=False, enable_eke_diss_surfbot=False,
+             eke_diss_surfbot_frac=1.0, enable_idemix_superbee_advection=False,
+             enable_idemix_upwind_advection=False, enable_cyclic_x=True)
+    d.update(kw)
+    return SimpleNamespace(**d)
+
+
 def friction_config2(seed=1002, nx=1440, ny=720, nz=80):
     """The 1440x720x80 grid of the mixing step with the vertical-friction fields."""
     st = iso_config2(seed, nx, ny, nz)
