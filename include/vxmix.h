@@ -348,6 +348,28 @@ int vx_integrate_idemix_f64(void* stream, const vx_grid* grid, const vx_idemix_p
                             const vx_terms* forc_post, double* iw_diss);
 
 /*
+ * enforce_boundaries (veros/core/utilities.py:7-21) on one process: with
+ * enable_cyclic_x, arr[-2:] = arr[2:4] and arr[:2] = arr[-4:-2] along the
+ * first axis (X = nx + 4 >= 8); element r of x row i at
+ * arr[i*row_stride + r*elem_stride] for r < row_elems (e.g. a contiguous
+ * (X, Y, nz) array: row_elems = row_stride = Y*nz, elem_stride = 1; level l
+ * of a levels-innermost (X, Y, nz, 3) array: pass arr + l, row_stride =
+ * 3*Y*nz, elem_stride = 3). Without cyclic x there is nothing to exchange.
+ */
+int vx_enforce_boundaries_f64(void* stream, double* arr, int64_t X, int64_t row_elems, int64_t row_stride,
+                              int64_t elem_stride, int32_t enable_cyclic_x);
+
+/*
+ * isoneutral_diag_streamfunction_kernel (veros/core/isoneutral/isoneutral.py:232-255):
+ * B2_gm[1:-2, 2:-2] = 0.25 * diffloc_e * sum(Ai_ez) and
+ * B1_gm[2:-2, 1:-2] = -0.25 * diffloc_n * sum(Ai_nz), diffloc the 4-point
+ * average of K_gm (edge-padded in z) around the face; other entries keep
+ * their values. All arrays (X, Y, nz[, 2, 2]).
+ */
+int vx_iso_diag_streamfunction_f64(void* stream, const vx_grid* grid, const double* K_gm, const double* Ai_ez,
+                                   const double* Ai_nz, double* B1_gm, double* B2_gm);
+
+/*
  * vertmix_tempsalt (veros/core/thermodynamics.py:248-300): implicit vertical
  * mixing of temperature and salinity with diffusivity kappaH (X, Y, nz) and
  * the surface forcing forc_temp_surface / forc_salt_surface (X, Y) on the top

@@ -365,3 +365,20 @@ def isoneutral_diffusion(state, tr_name, istemp):
     for k, v in out._asdict().items():
         setattr(state.variables, k, v)
     return out
+
+
+def isoneutral_diag_streamfunction_kernel(state):
+    """B1_gm, B2_gm (veros/core/isoneutral/isoneutral.py:232-255) as new
+    arrays: the 4-point average of the z edge-padded K_gm around each face
+    times the sum of the face's four slopes, on [1:-2, 2:-2] (B2) and
+    [2:-2, 1:-2] (B1)."""
+    vs = state.variables
+    K = np.asarray(vs.K_gm, dtype=np.float64)
+    Kp = np.concatenate([K[..., :1], K, K[..., -1:]], axis=-1)  # pad_z_edges (utilities.py:24-35)
+    B1 = np.array(vs.B1_gm, dtype=np.float64, copy=True)
+    B2 = np.array(vs.B2_gm, dtype=np.float64, copy=True)
+    d = 0.25 * (Kp[1:-2, 2:-2, 1:-1] + Kp[1:-2, 2:-2, :-2] + Kp[2:-1, 2:-2, 1:-1] + Kp[2:-1, 2:-2, :-2])
+    B2[1:-2, 2:-2, :] = 0.25 * d * np.sum(np.asarray(vs.Ai_ez)[1:-2, 2:-2, ...], axis=(3, 4))
+    d = 0.25 * (Kp[2:-2, 1:-2, 1:-1] + Kp[2:-2, 1:-2, :-2] + Kp[2:-2, 2:-1, 1:-1] + Kp[2:-2, 2:-1, :-2])
+    B1[2:-2, 1:-2, :] = -0.25 * d * np.sum(np.asarray(vs.Ai_nz)[2:-2, 1:-2, ...], axis=(3, 4))
+    return KernelOutput(B1_gm=B1, B2_gm=B2)

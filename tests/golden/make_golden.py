@@ -16,11 +16,13 @@ Functions covered (reference file:line):
 * vertmix_tempsalt             veros/core/thermodynamics.py:248-300 (with enforce_boundaries)
 * implicit_vert_friction       veros/core/friction.py:90-205 (with ugrid_to_tgrid /
                                vgrid_to_tgrid, veros/core/numerics.py:313-336)
+* isoneutral_diag_streamfunction_kernel  veros/core/isoneutral/isoneutral.py:232-255
+* enforce_boundaries           veros/core/utilities.py:7-21
 * integrate_tke_kernel         veros/core/tke.py:123-321
 * integrate_eke_kernel         veros/core/eke.py:93-255
 * integrate_idemix_kernel      veros/core/idemix.py:47-265
 
-Usage: python tests/golden/make_golden.py [column] [turbulence]   (no argument: every fixture)
+Usage: python tests/golden/make_golden.py [column] [turbulence] [halo]   (no argument: every fixture)
 """
 from __future__ import annotations
 
@@ -284,6 +286,32 @@ TURB_IN = {
 }
 
 
+def halo_cases(ref):
+    """The streamfunction diagnostic on the isoneutral fixtures' states (with
+    pre's slopes and noisy B1_gm / B2_gm), and enforce_boundaries of a noisy
+    3-D array, cyclic and not."""
+    out = {}
+    for name in ("iso_12x10x8_cyclic_energy", "iso_9x7x5_closed"):
+        with np.load(HERE / f"{name}.npz", allow_pickle=False) as z:
+            rec = {k: z[k] for k in z.files}
+        K = rec["in_K_gm"]
+        rng = np.random.default_rng(41)
+        v = SimpleNamespace(K_gm=K, Ai_ez=rec["pre_Ai_ez"], Ai_nz=rec["pre_Ai_nz"],
+                            B1_gm=rng.standard_normal(K.shape), B2_gm=rng.standard_normal(K.shape))
+        ins = {k: np.array(getattr(v, k), copy=True) for k in ("K_gm", "Ai_ez", "Ai_nz", "B1_gm", "B2_gm")}
+        res = ref.isoneutral.isoneutral_diag_streamfunction_kernel(SimpleNamespace(variables=v))
+        o = {"in_" + k: a for k, a in ins.items()}
+        o.update({"out_B1_gm": np.asarray(res.B1_gm), "out_B2_gm": np.asarray(res.B2_gm)})
+        out["streamfunction_" + name[4:]] = o
+    rng = np.random.default_rng(42)
+    arr = rng.standard_normal((13, 9, 6))
+    for cyc in (True, False):
+        res = ref.utilities.enforce_boundaries(np.array(arr, copy=True), cyc)
+        out[f"boundaries_13x9x6_{'cyclic' if cyc else 'closed'}"] = {"in_arr": arr, "out_arr": np.asarray(res),
+                                                                      "set_cyclic": np.asarray(cyc)}
+    return out
+
+
 def turbulence_cases(ref):
     from oracle.turbulence import turbulence_settings
     from veros_amd import workloads as W
@@ -325,12 +353,16 @@ def main():
             for name, rec in column_cases(ref).items():
                 np.savez_compressed(HERE / f"{name}.npz", **rec)
                 print("wrote", f"{name}.npz")
+        if "halo" in only:
+            for name, rec in halo_cases(ref).items():
+                np.savez_compressed(HERE / f"{name}.npz", **rec)
+                print("wrote", f"{name}.npz")
         if "turbulence" in only:
             for name, rec in turbulence_cases(ref).items():
                 np.savez_compressed(HERE / f"{name}.npz", **rec)
                 print("wrote", f"{name}.npz")
-        if only - {"column", "turbulence"}:
-            raise SystemExit(f"unknown groups {sorted(only - {'column', 'turbulence'})}")
+        if only - {"column", "turbulence", "halo"}:
+            raise SystemExit(f"unknown groups {sorted(only - {'column', 'turbulence', 'halo'})}")
         return
     for name, rec in tdma_cases(ref).items():
         np.savez_compressed(HERE / f"tdma_{name}.npz", **rec)
@@ -342,6 +374,9 @@ def main():
         np.savez_compressed(HERE / f"{name}.npz", **rec)
         print("wrote", f"{name}.npz")
     for name, rec in turbulence_cases(ref).items():
+        np.savez_compressed(HERE / f"{name}.npz", **rec)
+        print("wrote", f"{name}.npz")
+    for name, rec in halo_cases(ref).items():
         np.savez_compressed(HERE / f"{name}.npz", **rec)
         print("wrote", f"{name}.npz")
 

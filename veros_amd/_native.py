@@ -117,6 +117,8 @@ _SIGNATURES = {
     "vx_integrate_idemix_f64": (ctypes.c_int, [c_void_p, ctypes.POINTER(VxGrid), ctypes.POINTER(VxIdemixParams),
                                                c_void_p, c_i32, c_i32] + [c_void_p] * 5
                                 + [ctypes.POINTER(VxTerms), ctypes.POINTER(VxTerms), c_void_p]),
+    "vx_enforce_boundaries_f64": (ctypes.c_int, [c_void_p, c_void_p, c_i64, c_i64, c_i64, c_i64, c_i32]),
+    "vx_iso_diag_streamfunction_f64": (ctypes.c_int, [c_void_p, ctypes.POINTER(VxGrid)] + [c_void_p] * 5),
     "vx_iso_adjoint_workspace_bytes": (c_i64, [ctypes.POINTER(VxGrid)]),
     "vx_iso_diffusion_adjoint_f64": (ctypes.c_int, [c_void_p, ctypes.POINTER(VxGrid),
                                                     ctypes.POINTER(VxIsoParams), c_void_p, c_i32]

@@ -251,3 +251,30 @@ def isoneutral_pre_and_diffusion_temp_salt(state):
                                        *(ptr(t) for t in tens), ptr(dT), ptr(dS), ptr(IT), ptr(IS), ptr(P),
                                        ptr(ws))
     _native.check(rc, "isoneutral_diffusion_pre + isoneutral_diffusion (temp, salt)")
+
+
+def isoneutral_diag_streamfunction_kernel(state):
+    """B1_gm, B2_gm from K_gm and the slopes (veros/core/isoneutral/isoneutral.py:232-255),
+    in place (vx_iso_diag_streamfunction_f64)."""
+    vs = state.variables
+    X, Y, nz = vs.K_gm.shape
+    keep = []
+    g = grid_struct(vs, keep)
+    K = _check_array("K_gm", vs.K_gm, (X, Y, nz))
+    Aez = _check_array("Ai_ez", vs.Ai_ez, (X, Y, nz, 2, 2))
+    Anz = _check_array("Ai_nz", vs.Ai_nz, (X, Y, nz, 2, 2))
+    B1 = _check_array("B1_gm", vs.B1_gm, (X, Y, nz))
+    B2 = _check_array("B2_gm", vs.B2_gm, (X, Y, nz))
+    rc = _native.lib().vx_iso_diag_streamfunction_f64(stream_handle(), g, ptr(K), ptr(Aez), ptr(Anz), ptr(B1),
+                                                      ptr(B2))
+    _native.check(rc, "isoneutral_diag_streamfunction")
+    return KernelOutput(B1_gm=B1, B2_gm=B2)
+
+
+def isoneutral_diag_streamfunction(state):
+    """The routine (veros/core/isoneutral/isoneutral.py:257-269): only with
+    neutral and skew diffusion enabled."""
+    st = state.settings
+    if not (getattr(st, "enable_neutral_diffusion", True) and getattr(st, "enable_skew_diffusion", True)):
+        return
+    _store(state, isoneutral_diag_streamfunction_kernel(state))

@@ -107,3 +107,34 @@ def solve_implicit(a, b, c, d, water_mask=None, edge_mask=None, b_edge=None, d_e
         kbot = kbot_from_masks(water_mask, edge_mask)
     return solve_implicit_kbot(a, b, c, d, kbot, b_edge=b_edge, d_edge=d_edge, precision=precision,
                                refine_steps=refine_steps, out=out)
+
+
+def enforce_boundaries(arr, enable_cyclic_x, local=False):
+    """The one-process halo update of veros/core/utilities.py:7-21, in place
+    on the GPU (vx_enforce_boundaries_f64): with cyclic x, arr[-2:] =
+    arr[2:4] and arr[:2] = arr[-4:-2] along the first axis. ``arr`` is a
+    float64 torch tensor on the GPU whose trailing axes form one evenly
+    strided run per x row (any contiguous array, or one time level of a
+    levels-innermost or levels-outermost variable). Returns ``arr``."""
+    if not isinstance(arr, torch.Tensor) or arr.device.type != "cuda":
+        raise TypeError("arr must be a torch tensor on the GPU")
+    if arr.dtype != torch.float64:
+        raise TypeError(f"arr must be torch.float64 (got {arr.dtype})")
+    if not enable_cyclic_x:
+        return arr
+    X = arr.shape[0]
+    inner = arr.shape[1:]
+    n = 1
+    for d in inner:
+        n *= int(d)
+    # the trailing axes must collapse to one run of stride `es`
+    es = arr.stride(-1) if arr.dim() > 1 else 1
+    expect = es
+    for d, st in zip(reversed(inner), reversed(arr.stride()[1:])):
+        if d > 1 and st != expect:
+            raise ValueError(f"trailing axes of strides {tuple(arr.stride()[1:])} are not one evenly strided run")
+        expect *= int(d)
+    rc = _native.lib().vx_enforce_boundaries_f64(stream_handle(), arr.data_ptr(), int(X), int(n),
+                                                 int(arr.stride(0)), int(es), 1)
+    _native.check(rc, "enforce_boundaries")
+    return arr
