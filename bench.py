@@ -27,9 +27,10 @@ sys.path.insert(0, str(REPO))
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per call of vx_iso_pre_diffusion_ts_f64 (its seven kernels) on
 # 1440x720x80, from the FETCH_SIZE and WRITE_SIZE rocprofv3 passes recorded in
-# profiles/ (read bytes = 2.0 x FETCH_SIZE, write bytes = 1.0 x WRITE_SIZE,
-# calibrated on a 1 GiB device copy)
-PRE_TS_TRAFFIC_BYTES = {(1440, 720, 80): 27_204_216_832}
+# profiles/mixing_step_1440x720x80.md (read bytes = 2.0 x FETCH_SIZE, write
+# bytes = 1.0 x WRITE_SIZE, calibrated on a 1 GiB device copy), per layout of
+# the time-level variables
+PRE_TS_TRAFFIC_BYTES = {(1440, 720, 80, "outer"): 21_187_375_991, (1440, 720, 80, "inner"): 27_204_216_832}
 # the minimum bytes per interior cell of pre + the temperature + salinity step
 # in one call: K_iso read (8), Ai_* and K_11/22/33 written once and not read
 # back (128 + 24), P_diss read and written (16), and per tracer tr[tau] (8),
@@ -621,7 +622,7 @@ def main():
     # the largest launch of the step
     dom = "iso_pre_and_diffusion_TS"
     t_dom = parts[dom][0]
-    traffic = PRE_TS_TRAFFIC_BYTES.get((nx, ny, nz))
+    traffic = PRE_TS_TRAFFIC_BYTES.get((nx, ny, nz, args.levels))
     roof = {"bound": "hbm",
             "kernel": "vx_iso_pre_diffusion_ts_f64 = wet_scan_{count,chunks,columns}_kernel + "
                       "(iso_pre_dry_kernel || iso_pre_kernel<true> + k33_top_zero_kernel + iso_step_wet_kernel<2>)",
@@ -629,8 +630,8 @@ def main():
             "frac": round(gbs[dom] / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over this call on this workload "
-                              "(read = 2.0 x FETCH_SIZE, write = 1.0 x WRITE_SIZE, factors from a 1 GiB copy); "
-                              "profiles/mixing_step_1440x720x80.md",
+                              f"with the time levels {args.levels}most (read = 2.0 x FETCH_SIZE, write = 1.0 x "
+                              "WRITE_SIZE, factors from a 1 GiB copy); profiles/mixing_step_1440x720x80.md",
             "algorithmic_bytes_per_launch": B8_step,
             "algorithmic_bytes_per_cell": 176 + 2 * 224,
             "avg_launch_ms": round(t_dom, 4),
