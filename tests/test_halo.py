@@ -54,9 +54,56 @@ def test_gpu_boundaries_levels(shape, levels):
         enforce_boundaries(t[..., lev], True)
         host[..., lev] = OV.enforce_boundaries(host[..., lev], True)
     assert np.array_equal(t.cpu().numpy(), host)
-    if t.dim() >= 4:  # trailing axes that are not one evenly strided run
-        with pytest.raises(ValueError):
-            enforce_boundaries(t[:, :, :-1, ..., 0], True)
+    # the whole variable (veros/veros.py:285-295 passes u, v, tke, eke, E_iw
+    # with all three levels), and a view whose rows split into several runs
+    host = rng.standard_normal(shape + (3,))
+    t = torch.from_numpy(host).cuda()
+    if levels == "outer" and len(shape) == 3:
+        t = to_levels_outer(t)
+    enforce_boundaries(t, True)
+    host = OV.enforce_boundaries(host, True)
+    assert np.array_equal(t.cpu().numpy(), host)
+    if t.dim() >= 4:
+        sub = t[:, :, :-1, ..., 0]
+        ref = OV.enforce_boundaries(sub.cpu().numpy(), True)
+        enforce_boundaries(sub, True)
+        assert np.array_equal(sub.cpu().numpy(), ref)
+
+
+def _offsets(t):
+    idx = np.zeros(1, dtype=np.int64)
+    for d, st in zip(t.shape[1:], t.stride()[1:]):
+        idx = (idx[:, None] + st * np.arange(d)[None, :]).ravel()
+    return np.sort(idx + (t.storage_offset() if hasattr(t, "storage_offset") else 0))
+
+
+@pytest.mark.parametrize("case", ["contiguous", "level_inner", "level_outer", "outer_whole", "sliced", "1d"])
+def test_boundary_runs_cover_each_row_once(case):
+    """The runs one enforce_boundaries call launches cover every element of
+    an x row exactly once, each run being one evenly strided run."""
+    from veros_amd.layout import to_levels_outer
+    from veros_amd.utilities import _boundary_runs, _even_run
+    base = torch.zeros((9, 7, 5, 3), dtype=torch.float64)
+    t = {"contiguous": base[..., 0].contiguous(), "level_inner": base[..., 1],
+         "level_outer": to_levels_outer(base)[..., 2], "outer_whole": to_levels_outer(base),
+         "sliced": base[:, 1:-1, :-1, :2], "1d": base[:, 0, 0, 0]}[case]
+    runs = []
+    _boundary_runs(t, runs)
+    got = []
+    for view, n, es in runs:
+        assert _even_run(view) == (n, es)
+        assert view.stride(0) == t.stride(0)
+        got.append(view.storage_offset() + es * np.arange(n))
+    assert np.array_equal(np.sort(np.concatenate(got)), _offsets(t))
+    assert len(runs) == {"contiguous": 1, "level_inner": 1, "level_outer": 1, "outer_whole": 3,
+                         "sliced": 20, "1d": 1}[case]
+
+
+def test_boundary_runs_limit():
+    from veros_amd.utilities import _boundary_runs
+    t = torch.zeros((8, 200, 3), dtype=torch.float64)[:, :, :2]
+    with pytest.raises(ValueError):
+        _boundary_runs(t, [])
 
 
 @pytest.mark.gpu

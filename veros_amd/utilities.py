@@ -109,13 +109,47 @@ def solve_implicit(a, b, c, d, water_mask=None, edge_mask=None, b_edge=None, d_e
                                refine_steps=refine_steps, out=out)
 
 
+# most launches one enforce_boundaries call may split a view into
+_MAX_BOUNDARY_RUNS = 64
+
+
+def _even_run(t):
+    """(elements, stride) when the offsets of one x row of ``t`` (all axes
+    but the first) are exactly {e * stride : 0 <= e < elements}, else None."""
+    dims = sorted((int(st), int(d)) for d, st in zip(t.shape[1:], t.stride()[1:]) if d > 1)
+    if not dims:
+        return 1, 1
+    es, n = dims[0][0], 1
+    for st, d in dims:
+        if st != n * es:
+            return None
+        n *= d
+    return n, es
+
+
+def _boundary_runs(t, out):
+    """Split ``t`` along its widest-strided row axis until every piece's rows
+    are one evenly strided run; append (view, elements, stride) to ``out``."""
+    run = _even_run(t)
+    if run is not None:
+        out.append((t,) + run)
+        return
+    ax = max((a for a in range(1, t.dim()) if t.shape[a] > 1), key=lambda a: t.stride(a))
+    for i in range(t.shape[ax]):
+        _boundary_runs(t.select(ax, i), out)
+        if len(out) > _MAX_BOUNDARY_RUNS:
+            raise ValueError(f"a view of strides {tuple(t.stride())} splits into more than "
+                             f"{_MAX_BOUNDARY_RUNS} evenly strided runs per x row")
+
+
 def enforce_boundaries(arr, enable_cyclic_x, local=False):
     """The one-process halo update of veros/core/utilities.py:7-21, in place
     on the GPU (vx_enforce_boundaries_f64): with cyclic x, arr[-2:] =
     arr[2:4] and arr[:2] = arr[-4:-2] along the first axis. ``arr`` is a
-    float64 torch tensor on the GPU whose trailing axes form one evenly
-    strided run per x row (any contiguous array, or one time level of a
-    levels-innermost or levels-outermost variable). Returns ``arr``."""
+    float64 torch tensor on the GPU: any contiguous array, a whole
+    time-level variable or one level of it in either layout
+    (veros_amd.layout), or another view whose x rows split into at most 64
+    evenly strided runs (one launch each). Returns ``arr``."""
     if not isinstance(arr, torch.Tensor) or arr.device.type != "cuda":
         raise TypeError("arr must be a torch tensor on the GPU")
     if arr.dtype != torch.float64:
@@ -123,18 +157,10 @@ def enforce_boundaries(arr, enable_cyclic_x, local=False):
     if not enable_cyclic_x:
         return arr
     X = arr.shape[0]
-    inner = arr.shape[1:]
-    n = 1
-    for d in inner:
-        n *= int(d)
-    # the trailing axes must collapse to one run of stride `es`
-    es = arr.stride(-1) if arr.dim() > 1 else 1
-    expect = es
-    for d, st in zip(reversed(inner), reversed(arr.stride()[1:])):
-        if d > 1 and st != expect:
-            raise ValueError(f"trailing axes of strides {tuple(arr.stride()[1:])} are not one evenly strided run")
-        expect *= int(d)
-    rc = _native.lib().vx_enforce_boundaries_f64(stream_handle(), arr.data_ptr(), int(X), int(n),
-                                                 int(arr.stride(0)), int(es), 1)
-    _native.check(rc, "enforce_boundaries")
+    runs = []
+    _boundary_runs(arr, runs)
+    for view, n, es in runs:
+        rc = _native.lib().vx_enforce_boundaries_f64(stream_handle(), view.data_ptr(), int(X), int(n),
+                                                     int(view.stride(0)), int(es), 1)
+        _native.check(rc, "enforce_boundaries")
     return arr
