@@ -53,11 +53,14 @@ def _dev(st, levels="inner"):
 @pytest.mark.parametrize("shape,cyclic,land,levels", [((9, 7, 6), True, 0.3, (0, 1)), ((14, 11, 33), False, 0.3, (1, 2)),
                                                       ((12, 9, 2), True, 0.0, (2, 0)), ((8, 10, 80), True, 0.35, (0, 1)),
                                                       ((6, 5, 128), False, 0.2, (1, 0)), ((7, 6, 17), True, 1.0, (0, 2)),
-                                                      ((10, 12, 100), True, 0.3, (2, 1)), ((5, 4, 3), False, 0.2, (0, 1))])
-def test_friction_random_against_oracle(shape, cyclic, land, levels):
+                                                      ((10, 12, 100), True, 0.3, (2, 1)), ((5, 4, 3), False, 0.2, (0, 1)),
+                                                      ((11, 9, 40), False, 0.25, (1, 2))])
+@pytest.mark.parametrize("layout", ["inner", "outer"])
+def test_friction_random_against_oracle(shape, cyclic, land, levels, layout):
+    """Levels outermost with nz even runs the 16-byte load/store variant."""
     st = _state(sum(shape), *shape, cyclic=cyclic, land=land, tau=levels[0], taup1=levels[1])
     ref = F.implicit_vert_friction(st)
-    ds = _dev(st)
+    ds = _dev(st, layout)
     from veros_amd import friction
     out = friction.implicit_vert_friction(ds)
     _check(out, ref, st)
@@ -86,14 +89,15 @@ def test_friction_channel_360x160x60_against_oracle():
     _check(out, ref, st)
 
 
-def test_friction_full_grid_properties():
+@pytest.mark.parametrize("layout", ["inner", "outer"])
+def test_friction_full_grid_properties(layout):
     """1440x720x80: the solve conserves sum_k dzt * u over the wet levels of
     each face column; du_mix / dv_mix and the halos outside the face range
     are untouched; a vertically uniform velocity at tau is a fixed point
     and dissipates nothing."""
     from veros_amd import friction
     st = W.friction_config2()
-    ds = _dev(st)
+    ds = _dev(st, layout)
     vs = ds.variables
     tau, t1 = st.variables.tau, st.variables.taup1
     u_tau = vs.u[..., tau].clone()
@@ -114,7 +118,7 @@ def test_friction_full_grid_properties():
     assert torch.equal(vs.du_mix[:, 0], du0[:, 0]) and torch.equal(vs.du_mix[:, -2:], du0[:, -2:])
 
     # uniform columns: the rows of the friction matrix sum to one
-    ds = _dev(st)
+    ds = _dev(st, layout)
     vs = ds.variables
     vs.u[..., tau] = 0.2
     vs.v[..., tau] = -0.1

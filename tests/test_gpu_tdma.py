@@ -62,6 +62,36 @@ def test_golden(path, mode):
     assert np.all(x[~g["water"]] == 0)
 
 
+def test_strongly_coupled_friction_systems(monkeypatch):
+    """The u and v systems of the strong-viscosity friction fixture
+    (tests/golden/friction_16x12x20_strong.npz), captured from the oracle and
+    solved through vx_tdma_f64: a few columns have interface rows that are
+    not diagonally dominant at all (|a| + |c| up to 5 b), where parallel
+    cyclic reduction alone leaves 4e-12 and the interface refinement step
+    brings the solve back to the accuracy of a sequential sweep (dgtsv
+    against a column Thomas sweep differ by 4e-15 here)."""
+    from conftest import column_state_from_golden
+    from oracle import friction as F
+    rec = load_golden([p for p in golden_files("friction_") if "strong" in p.stem][0])
+    captured = []
+    dgtsv = O.solve_tridiagonal_dgtsv
+
+    def spy(a, b, c, d, water, edge):
+        x = dgtsv(a, b, c, d, water, edge)
+        captured.append((a, b, c, d, water, edge, x))
+        return x
+
+    monkeypatch.setattr(O, "solve_tridiagonal_dgtsv", spy)
+    F.implicit_vert_friction(column_state_from_golden(rec))
+    assert len(captured) == 2
+    for a, b, c, d, water, edge, ref in captured:
+        kbot = np.where(edge.any(-1), np.argmax(edge, axis=-1) + 1, 0).astype(np.int32)
+        thomas = O.solve_tridiagonal_thomas(a, b, c, d, water, edge)
+        assert rel_err(thomas, ref) <= 1e-14
+        x = _solve((a, b, c, d), kbot)
+        assert rel_err(np.where(water, x, 0.0), np.where(water, ref, 0.0)) <= TOL
+
+
 def test_reference_mask_signature():
     from veros_amd.utilities import create_water_masks, solve_implicit
     g = load_golden(golden_files("tdma_diffusion")[0])

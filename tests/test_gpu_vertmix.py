@@ -53,11 +53,14 @@ def _np(t):
 @pytest.mark.parametrize("shape,cyclic,land", [((9, 7, 6), True, 0.3), ((14, 11, 33), False, 0.3),
                                                ((12, 9, 2), True, 0.0), ((8, 10, 80), True, 0.35),
                                                ((6, 5, 128), False, 0.2), ((7, 6, 17), True, 1.0),
-                                               ((10, 12, 100), True, 0.3)])
-def test_vertmix_random_against_oracle(shape, cyclic, land):
+                                               ((10, 12, 100), True, 0.3), ((11, 9, 15), False, 0.25)])
+@pytest.mark.parametrize("levels", ["inner", "outer"])
+def test_vertmix_random_against_oracle(shape, cyclic, land, levels):
+    """levels outermost with nz even runs the LDS-staged kernel; the others
+    the kernel whose lanes load their own levels."""
     st = _state(sum(shape), *shape, cyclic=cyclic, land=land)
     ref = V.vertmix_tempsalt(st)
-    ds = _dev(st)
+    ds = _dev(st, levels)
     from veros_amd import thermodynamics
     out = thermodynamics.vertmix_tempsalt(ds)
     _check(out, ref, st)
@@ -86,13 +89,14 @@ def test_vertmix_channel_360x160x60_against_oracle():
     _check(out, ref, st)
 
 
-def test_vertmix_full_grid_properties():
+@pytest.mark.parametrize("levels", ["inner", "outer"])
+def test_vertmix_full_grid_properties(levels):
     """1440x720x80: the column content sum_k dzt * tracer over the wet cells
     changes by exactly dt * forcing; a vertically uniform tracer without
     forcing is a fixed point; the tendencies vanish off the wet cells."""
     from veros_amd import thermodynamics
     st = W.vertmix_config2()
-    ds = _dev(st)
+    ds = _dev(st, levels)
     vs = ds.variables
     t1, dt = vs.taup1, st.settings.dt_tracer
     nz = vs.kappaH.shape[2]

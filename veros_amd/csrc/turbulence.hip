@@ -55,7 +55,7 @@ struct TurbCol {
 
 __device__ __forceinline__ TurbCol turb_col(const TurbGrid& g, int S) {
     TurbCol t;
-    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t gt = xcd_contiguous_block() * blockDim.x + threadIdx.x;
     const int64_t col = gt / kSegP;
     t.s = (int)(gt % kSegP);
     t.lo = t.s * S;
@@ -485,7 +485,7 @@ template <class Args, template <int> class K>
 int launch_turb(const Args& a, const TurbGrid& g, hipStream_t s, const char* who) {
     const int S = seg_levels(g.nz);
     const int64_t threads = (int64_t)g.X * g.Y * kSegP;
-    const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
+    const dim3 grid(blocks_mult8(threads, 256)), block(256);
 #define VX_TURB_CASE(N) \
     case N: hipLaunchKernelGGL(K<N>::fn, grid, block, 0, s, a); break;
     switch (S) {

@@ -7,12 +7,19 @@
 // b_edge = 1 + delta / dzt), adds the surface forcing to the top level of the
 // right-hand side, solves it with solve_implicit for temp and for salt, and
 // then forms the tendencies (new - old) / dt over the whole array and copies
-// the cyclic x halos. Here one lane per (column, tracer) forms the rows in
-// registers while it sweeps, so the coefficients never touch memory (the
-// reference's six coefficient passes and the separate tendency pass fold into
-// the solve): it reads kappaH and the tracer column once and writes the tracer
-// and the tendency once.
-#include "vx_common.hpp"
+// the cyclic x halos.
+//
+// Here a group of kSegP lanes owns a column (the segmented solve of
+// tdma_seg.hpp, lane s holding levels [s*S, s*S + S)): each lane forms its
+// rows in registers, the group solves temperature and salinity with one
+// factorisation, and each lane writes the new tracers on the wet levels and
+// both tendencies on every level. The coefficients never touch memory: the
+// kernel reads kappaH and the two tracer columns once and writes them and the
+// tendencies once. With the time levels outermost and 16-byte aligned arrays
+// the wave's 8 columns are copied into LDS and written back through it with
+// coalesced 16-byte accesses (vertmix_segl_kernel); otherwise each lane
+// loads and stores its own levels (vertmix_seg_kernel).
+#include "tdma_seg.hpp"
 
 namespace vx {
 namespace {
@@ -28,116 +35,205 @@ struct VmixArgs {
     int64_t X, Y;
     int32_t nz, taup1;
     double dt;
-    bool vec_dv;  // nz even and both tendency arrays 16-byte aligned
     // time levels of temp/salt: level l of cell c at l * ls + c * cs (cs = 3,
     // ls = 1: levels innermost; cs = 1, ls = X*Y*nz: levels outermost)
     int64_t cs, ls;
 };
 
-constexpr int kVmixBlock = 256;  // 128 columns x 2 tracers
-constexpr int kVmixChunk = 8;    // levels per load batch (NZB is a multiple)
-
-// Lane 2*col + t solves tracer t of column col (the two lanes of a column read
-// the same kappaH addresses in the same instruction). Columns outside the
-// interior [2, X-2) x [2, Y-2) and land columns only write their zero
-// tendencies: the reference's (tr - tr) / dt there.
-template <int NZB>
-__global__ __launch_bounds__(kVmixBlock) void vertmix_kernel(VmixArgs p) {
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int t = (int)(gid & 1);
-    const int64_t col = gid >> 1;
-    if (col >= p.X * p.Y) return;
+// The lane's column over the whole (X, Y) grid; only interior columns
+// [2, X-2) x [2, Y-2) are solved (k0 = nz elsewhere, and on land).
+__device__ __forceinline__ SegLane vmix_lane(const VmixArgs& p, int S, int64_t& col) {
+    SegLane L;
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    col = gt / kSegP;
+    L.s = (int)(gt % kSegP);
+    L.lo = L.s * S;
+    L.valid = col < p.X * p.Y;
+    if (!L.valid) col = 0;
     const int64_t i = col / p.Y, j = col - (col / p.Y) * p.Y;
-    const int nz = p.nz;
-    const int64_t base = col * nz;
-    double* __restrict__ dv = p.dvmix[t] + base;
-    const bool interior = i >= 2 && i < p.X - 2 && j >= 2 && j < p.Y - 2;
+    const bool interior = L.valid && i >= 2 && i < p.X - 2 && j >= 2 && j < p.Y - 2;
     const int kb = interior ? p.kbot[col] : 0;
-    const int k0 = (kb >= 1 && kb <= nz) ? kb - 1 : nz;  // first wet level (nz: none)
-    const double dt = p.dt;
-    double* __restrict__ tr = p.tr[t] + p.taup1 * p.ls;  // level taup1 of cell c at tr[cs c]
-    const double* __restrict__ kap = p.kappaH + base;
-    const bool vec = p.vec_dv;
+    L.k0 = (kb >= 1 && kb <= p.nz) ? kb - 1 : p.nz;
+    L.base = col * p.nz;
+    return L;
+}
 
-    // forward elimination with the rows formed on the fly
-    // (veros/core/thermodynamics.py:267-276, veros/core/operators.py:60-77);
-    // each chunk of kVmixChunk levels is loaded in one batch (levels past nz-1
-    // re-read level nz-1, dry levels are read and ignored) before its sweep
-    double cp[NZB], dp[NZB];
-    double cpp = 0.0, dpp = 0.0, dlm = 0.0;  // dlm = delta[k-1]
-    const double forc = k0 < nz ? p.forc[t][col] : 0.0;
+// The rows of the lane's levels (veros/core/thermodynamics.py:267-277),
+// identity rows off the wet suffix; z1, z2 get the surface forcing on the top
+// level. KM(k) is kappaH at level k of the column, for k in [lo-1, lo+S); its
+// value matters for k0 <= k < nz-1 only.
+template <int S, class KMF>
+__device__ __forceinline__ void vmix_rows(const VmixArgs& p, const SegLane& L, KMF KM, double fT, double fS,
+                                          double (&va)[S], double (&vb)[S], double (&vc)[S], double (&z1)[S],
+                                          double (&z2)[S]) {
+    const int nz = p.nz;
+    const double dt = p.dt;
+    const int kp = L.lo - 1;  // delta at the level below the segment
+    double dlm = (kp >= L.k0 && kp < nz - 1) ? dt / p.dzw[kp > 0 ? kp : 0] * KM(kp) : 0.0;
 #pragma unroll
-    for (int q = 0; q < NZB; q += kVmixChunk) {
-        double vk[kVmixChunk], vt[kVmixChunk];
-        const bool live = q < nz && q + kVmixChunk > k0;
-        if (live) {
+    for (int t = 0; t < S; ++t) {
+        const int k = L.lo + t;
+        const bool wet = k >= L.k0 && k < nz;
+        const int kc = k < nz ? k : nz - 1;
+        const double dz = p.dzt[kc];
+        const double dl = (wet && k < nz - 1) ? dt / p.dzw[kc] * KM(k) : 0.0;
+        const bool edge = k == L.k0;
+        va[t] = (wet && !edge) ? -dlm / dz : 0.0;
+        vb[t] = wet ? (edge ? 1 + dl / dz : 1 + (dl + dlm) / dz) : 1.0;
+        vc[t] = (wet && k < nz - 1) ? -dl / dz : 0.0;
+        if (k == nz - 1) {  // surface forcing (:277, :284)
+            z1[t] = z1[t] + dt * fT / dz;
+            z2[t] = z2[t] + dt * fS / dz;
+        }
+        z1[t] = wet ? z1[t] : 0.0;
+        z2[t] = wet ? z2[t] : 0.0;
+        dlm = dl;
+    }
+}
+
+// Register variant: each lane reads and writes its own levels (any layout
+// and alignment; VEC: 16-byte tendency stores, nz even and both tendency
+// arrays 16-byte aligned).
+template <int S, bool VEC>
+__global__ __launch_bounds__(kSegBlock) void vertmix_seg_kernel(VmixArgs p) {
+    int64_t col;
+    const SegLane L = vmix_lane(p, S, col);
+    const int nz = p.nz;
+    const int64_t cs = p.cs;
+    double* __restrict__ T = p.tr[0] + p.taup1 * p.ls;
+    double* __restrict__ Sa = p.tr[1] + p.taup1 * p.ls;
+    double km[S + 1], oT[S], oS[S];  // km[t]: kappaH at level lo-1+t
 #pragma unroll
-            for (int u = 0; u < kVmixChunk; ++u) {
-                const int kk = q + u < nz ? q + u : nz - 1;
-                vk[u] = kap[kk];
-                vt[u] = tr[p.cs * (base + kk)];
+    for (int t = 0; t <= S; ++t) {
+        const int k = L.lo - 1 + t;
+        km[t] = (k >= L.k0 && k < nz) ? p.kappaH[L.base + k] : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+        const int k = L.lo + t;
+        const bool w = k >= L.k0 && k < nz;
+        oT[t] = w ? T[cs * (L.base + k)] : 0.0;
+        oS[t] = w ? Sa[cs * (L.base + k)] : 0.0;
+    }
+    const bool solve = L.k0 < nz;
+    const double fT = solve ? p.forc[0][col] : 0.0, fS = solve ? p.forc[1][col] : 0.0;
+    double va[S], vb[S], vc[S], z1[S], z2[S];
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+        z1[t] = oT[t];
+        z2[t] = oS[t];
+    }
+    vmix_rows<S>(p, L, [&](int k) { return km[k - L.lo + 1]; }, fT, fS, va, vb, vc, z1, z2);
+    seg_solve2_f64<S>(L, va, vb, vc, z1, z2);
+    if (!L.valid) return;
+    const double dt = p.dt;
+    double* __restrict__ dT = p.dvmix[0] + L.base;
+    double* __restrict__ dS = p.dvmix[1] + L.base;
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+        const int k = L.lo + t;
+        const bool w = k >= L.k0 && k < nz;
+        if (w) {
+            T[cs * (L.base + k)] = z1[t];
+            Sa[cs * (L.base + k)] = z2[t];
+        }
+        z1[t] = w ? (z1[t] - oT[t]) / dt : 0.0;  // (new - old) / dt; 0 where unchanged
+        z2[t] = w ? (z2[t] - oS[t]) / dt : 0.0;
+    }
+    if (VEC) {
+#pragma unroll
+        for (int t = 0; t < S; t += 2) {
+            const int k = L.lo + t;
+            if (k < nz) {
+                *reinterpret_cast<double2*>(dT + k) = make_double2(z1[t], z1[t + 1]);
+                *reinterpret_cast<double2*>(dS + k) = make_double2(z2[t], z2[t + 1]);
             }
         }
+    } else {
 #pragma unroll
-        for (int u = 0; u < kVmixChunk; ++u) {
-            const int k = q + u;
-            if (live && k < nz && k >= k0) {
-                const double dl = (k < nz - 1) ? dt / p.dzw[k] * vk[u] : 0.0;
-                const double dz = p.dzt[k];
-                const bool edge = k == k0;
-                const double ak = edge ? 0.0 : -dlm / dz;
-                const double bk = edge ? 1 + dl / dz : 1 + (dl + dlm) / dz;
-                const double ck = (k < nz - 1) ? -dl / dz : 0.0;
-                double dk = vt[u];
-                if (k == nz - 1) dk = dk + dt * forc / dz;  // surface forcing (:277, :284)
-                const double r = 1.0 / (bk - ak * cpp);
-                cpp = ck * r;
-                dpp = (dk - ak * dpp) * r;
-                dlm = dl;
+        for (int t = 0; t < S; ++t) {
+            const int k = L.lo + t;
+            if (k < nz) {
+                dT[k] = z1[t];
+                dS[k] = z2[t];
             }
-            cp[k] = cpp;
-            dp[k] = dpp;
         }
     }
-    // back substitution from the top, a chunk at a time; the tracer and its
-    // tendency (:279-288), the old values re-read in one batch per chunk
-    double xn = 0.0;
+}
+
+// Stage the lane's levels v into its column's row of a wave image, then store
+// the image's pairs selected by the plan, coalesced (a wave's LDS operations
+// complete in order, so the image may be reused right after).
+template <int S>
+__device__ __forceinline__ void vmix_store_pairs(double* img, const SegCopyPlan<S>& cp, double* __restrict__ dst,
+                                                 const SegLane& L, const double (&v)[S]) {
+    double* row = img + ((threadIdx.x & 63) / kSegP) * (kSegP * S) + L.lo;
 #pragma unroll
-    for (int q = (NZB - 1) / kVmixChunk * kVmixChunk; q >= 0; q -= kVmixChunk) {
-        if (q < nz) {
-            double vo[kVmixChunk];
-            const bool live = q + kVmixChunk > k0;
-            if (live) {
+    for (int t = 0; t < S; t += 2) *reinterpret_cast<double2*>(row + t) = make_double2(v[t], v[t + 1]);
 #pragma unroll
-                for (int u = 0; u < kVmixChunk; ++u) vo[u] = tr[p.cs * (base + (q + u < nz ? q + u : nz - 1))];
-            }
-            double dvv[kVmixChunk];
-#pragma unroll
-            for (int u = kVmixChunk - 1; u >= 0; --u) {
-                const int k = q + u;
-                dvv[u] = 0.0;
-                if (k < nz && live && k >= k0) {
-                    const double x = dp[k] - cp[k] * xn;
-                    xn = x;
-                    tr[p.cs * (base + k)] = x;
-                    dvv[u] = (x - vo[u]) / dt;
-                }
-            }
-            // the chunk's tendencies: 16-byte stores when the column start is
-            // 16-byte aligned (nz even)
-            if (vec) {
-#pragma unroll
-                for (int u = 0; u < kVmixChunk; u += 2) {
-                    if (q + u < nz) *reinterpret_cast<double2*>(dv + q + u) = make_double2(dvv[u], dvv[u + 1]);
-                }
-            } else {
-#pragma unroll
-                for (int u = 0; u < kVmixChunk; ++u) {
-                    if (q + u < nz) dv[q + u] = dvv[u];
-                }
-            }
+    for (int i = 0; i < SegCopyPlan<S>::NI; ++i) {
+        if (cp.pred[i]) {
+            const int pidx = i * 64 + (threadIdx.x & 63);
+            *reinterpret_cast<double2*>(dst + cp.off[i]) = *reinterpret_cast<const double2*>(img + 2 * pidx);
         }
     }
+}
+
+// LDS bytes of one wave: kappaH, temp and salt images of its 8 columns
+__host__ __device__ constexpr int vmix_lds_wave_bytes(int S) { return 3 * kSegCols * kSegP * S * 8; }
+
+// LDS variant (time levels outermost, nz even, kappaH, both tracer levels and
+// both tendencies 16-byte aligned): the wave copies the wet pairs of its 8
+// columns' kappaH, temp and salt into LDS, each lane reads its levels (kappaH
+// from level lo-1), and the results go back through the images: tracers on
+// the pairs that touch the wet suffix (a pair's dry level gets its old value
+// back), tendencies on every pair.
+template <int S>
+__global__ __launch_bounds__(kSegLdsBlock) void vertmix_segl_kernel(VmixArgs p) {
+    extern __shared__ double vmix_lds[];
+    constexpr int IMG = kSegCols * kSegP * S;
+    double* lds = vmix_lds + (threadIdx.x / 64) * 3 * IMG;
+    int64_t col;
+    const SegLane L = vmix_lane(p, S, col);
+    const int nz = p.nz;
+    double* __restrict__ T = p.tr[0] + p.taup1 * p.ls;
+    double* __restrict__ Sa = p.tr[1] + p.taup1 * p.ls;
+    SegCopyPlan<S> wet;
+    seg_copy_plan<S>(L, threadIdx.x & 63, nz, true, wet);
+    seg_lds_copy<S>(p.kappaH, lds + 0 * IMG, wet);
+    seg_lds_copy<S>(T, lds + 1 * IMG, wet);
+    seg_lds_copy<S>(Sa, lds + 2 * IMG, wet);
+    const bool solve = L.k0 < nz;
+    const double fT = solve ? p.forc[0][col] : 0.0, fS = solve ? p.forc[1][col] : 0.0;
+    seg_wait_vmem();
+    const double* km = lds + ((threadIdx.x & 63) / kSegP) * (kSegP * S);  // the column's kappaH
+    double va[S], vb[S], vc[S], z1[S], z2[S];
+    seg_lds_read<S>(lds + 1 * IMG, L, z1);
+    seg_lds_read<S>(lds + 2 * IMG, L, z2);
+    vmix_rows<S>(p, L, [&](int k) { return km[k > 0 ? k : 0]; }, fT, fS, va, vb, vc, z1, z2);
+    seg_solve2_f64<S>(L, va, vb, vc, z1, z2);
+    // tracers: new values on the wet levels, the old ones (still in the
+    // images) elsewhere
+    const double dt = p.dt;
+    double oT[S], oS[S];
+    seg_lds_read<S>(lds + 1 * IMG, L, oT);
+    seg_lds_read<S>(lds + 2 * IMG, L, oS);
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+        const int k = L.lo + t;
+        const bool w = k >= L.k0 && k < nz;
+        const double nT = w ? z1[t] : oT[t], nS = w ? z2[t] : oS[t];
+        z1[t] = nT;
+        z2[t] = nS;
+        oT[t] = w ? (nT - oT[t]) / dt : 0.0;  // (new - old) / dt; 0 where unchanged
+        oS[t] = w ? (nS - oS[t]) / dt : 0.0;
+    }
+    vmix_store_pairs<S>(lds + 0 * IMG, wet, T, L, z1);
+    vmix_store_pairs<S>(lds + 1 * IMG, wet, Sa, L, z2);
+    SegCopyPlan<S> all;
+    seg_copy_plan<S>(L, threadIdx.x & 63, nz, false, all);
+    vmix_store_pairs<S>(lds + 2 * IMG, all, p.dvmix[0], L, oT);
+    vmix_store_pairs<S>(lds + 0 * IMG, all, p.dvmix[1], L, oS);
 }
 
 // enforce_boundaries on one process (veros/core/utilities.py:7-21): the x
@@ -157,11 +253,20 @@ __global__ __launch_bounds__(256) void vertmix_cyclic_kernel(VmixArgs p) {
     tr[p.cs * (dst * plane + off)] = tr[p.cs * (src * plane + off)];
 }
 
-template <int NZB>
-void launch_vmix(const VmixArgs& p, hipStream_t s) {
-    const int64_t lanes = 2 * p.X * p.Y;
-    hipLaunchKernelGGL(vertmix_kernel<NZB>, dim3((unsigned)((lanes + kVmixBlock - 1) / kVmixBlock)),
-                       dim3(kVmixBlock), 0, s, p);
+template <int S>
+void launch_vmix(const VmixArgs& p, bool lds, bool vec, hipStream_t s) {
+    const int64_t threads = p.X * p.Y * kSegP;
+    if (lds) {
+        const size_t bytes = (size_t)vmix_lds_wave_bytes(S) * (kSegLdsBlock / 64);
+        hipLaunchKernelGGL(vertmix_segl_kernel<S>, dim3((unsigned)((threads + kSegLdsBlock - 1) / kSegLdsBlock)),
+                           dim3(kSegLdsBlock), bytes, s, p);
+    } else if (vec) {
+        hipLaunchKernelGGL((vertmix_seg_kernel<S, true>), dim3((unsigned)((threads + kSegBlock - 1) / kSegBlock)),
+                           dim3(kSegBlock), 0, s, p);
+    } else {
+        hipLaunchKernelGGL((vertmix_seg_kernel<S, false>), dim3((unsigned)((threads + kSegBlock - 1) / kSegBlock)),
+                           dim3(kSegBlock), 0, s, p);
+    }
 }
 
 }  // namespace
@@ -205,23 +310,24 @@ extern "C" int vx_vertmix_tempsalt_f64(void* stream, const vx_grid* grid, double
     p.nz = grid->nz;
     p.taup1 = taup1;
     p.dt = dt_tracer;
-    p.vec_dv = !(p.nz & 1) && aligned16(dtemp_vmix) && aligned16(dsalt_vmix);
     if (grid->time_layout != VX_LEVELS_INNER && grid->time_layout != VX_LEVELS_OUTER)
         return fail(VX_ERR_INVALID_ARGUMENT, "%s: time_layout must be VX_LEVELS_INNER or VX_LEVELS_OUTER", who);
     const bool outer = grid->time_layout == VX_LEVELS_OUTER;
     p.cs = outer ? 1 : 3;
     p.ls = outer ? p.X * p.Y * p.nz : 1;
+    const bool vec = !(p.nz & 1) && aligned16(dtemp_vmix) && aligned16(dsalt_vmix);
+    const bool lds = vec && outer && aligned16(kappaH) && aligned16(temp + taup1 * p.ls) &&
+                     aligned16(salt + taup1 * p.ls);
     hipStream_t s = (hipStream_t)stream;
-    const int nzb = ((p.nz + 15) / 16) * 16;
-    switch (nzb) {
-        case 16: launch_vmix<16>(p, s); break;
-        case 32: launch_vmix<32>(p, s); break;
-        case 48: launch_vmix<48>(p, s); break;
-        case 64: launch_vmix<64>(p, s); break;
-        case 80: launch_vmix<80>(p, s); break;
-        case 96: launch_vmix<96>(p, s); break;
-        case 112: launch_vmix<112>(p, s); break;
-        default: launch_vmix<128>(p, s); break;
+    switch (seg_levels(p.nz)) {
+        case 2: launch_vmix<2>(p, lds, vec, s); break;
+        case 4: launch_vmix<4>(p, lds, vec, s); break;
+        case 6: launch_vmix<6>(p, lds, vec, s); break;
+        case 8: launch_vmix<8>(p, lds, vec, s); break;
+        case 10: launch_vmix<10>(p, lds, vec, s); break;
+        case 12: launch_vmix<12>(p, lds, vec, s); break;
+        case 14: launch_vmix<14>(p, lds, vec, s); break;
+        default: launch_vmix<16>(p, lds, vec, s); break;
     }
     int rc = check_launch(who);
     if (rc) return rc;
