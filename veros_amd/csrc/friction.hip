@@ -66,46 +66,6 @@ __device__ inline Face face_of(int kb0, int kb1, int nz) {
     return f;
 }
 
-// Levels lo .. lo+S-1 of a column with time-level stride cs: VEC uses
-// 16-byte loads of level pairs (cs = 1, nz even, aligned arrays). Levels
-// for which want(k) is false are left as they are.
-template <int S, bool VEC, class W>
-__device__ __forceinline__ void fric_load(const double* __restrict__ a, int64_t cs, int lo, int nz, W want,
-                                          double (&v)[S]) {
-    if constexpr (VEC) {
-#pragma unroll
-        for (int t = 0; t < S; t += 2) {
-            const int k = lo + t;
-            if (k < nz && (want(k) || want(k + 1))) {
-                const double2 w = *reinterpret_cast<const double2*>(a + k);
-                v[t] = w.x;
-                v[t + 1] = w.y;
-            }
-        }
-    } else {
-#pragma unroll
-        for (int t = 0; t < S; ++t) {
-            const int k = lo + t;
-            if (k < nz && want(k)) v[t] = a[cs * k];
-        }
-    }
-}
-
-template <int S, bool VEC>
-__device__ __forceinline__ void fric_store(double* __restrict__ a, int lo, int nz, const double (&v)[S]) {
-    if constexpr (VEC) {
-#pragma unroll
-        for (int t = 0; t < S; t += 2) {
-            if (lo + t < nz) *reinterpret_cast<double2*>(a + lo + t) = make_double2(v[t], v[t + 1]);
-        }
-    } else {
-#pragma unroll
-        for (int t = 0; t < S; ++t) {
-            if (lo + t < nz) a[lo + t] = v[t];
-        }
-    }
-}
-
 // One face column of component t (face range only when `live`): rows from
 // the face viscosity (veros/core/friction.py:111-126, 161-176), the solve,
 // the velocity at taup1 on the wet levels and the tendency on every level
@@ -125,13 +85,13 @@ __device__ __forceinline__ void fric_face(const FricArgs& p, SegLane L, int t, i
     double ka[S], kn[S], x0[S], x1[S];
 #pragma unroll
     for (int q = 0; q < S; ++q) ka[q] = kn[q] = x0[q] = x1[q] = 0.0;
-    fric_load<S, VEC>(p.kappaM + base, 1, lo, nz, [&](int k) { return k >= k0; }, ka);
-    fric_load<S, VEC>(p.kappaM + nb * nz, 1, lo, nz, [&](int k) { return k >= k0; }, kn);
+    col_load<S, VEC>(p.kappaM + base, 1, lo, nz, [&](int k) { return k >= k0; }, ka);
+    col_load<S, VEC>(p.kappaM + nb * nz, 1, lo, nz, [&](int k) { return k >= k0; }, kn);
     // tau on every level, taup1 below the solve (kept there: du_mix reads it)
     const double* u0p = p.vel[t] + p.tau * p.ls + p.cs * base;
     double* u1p = p.vel[t] + p.taup1 * p.ls + p.cs * base;
-    fric_load<S, VEC>(u0p, p.cs, lo, nz, [&](int) { return live; }, x0);
-    fric_load<S, VEC>(u1p, p.cs, lo, nz, [&](int k) { return live && k < k0; }, x1);
+    col_load<S, VEC>(u0p, p.cs, lo, nz, [&](int) { return live; }, x0);
+    col_load<S, VEC>(u1p, p.cs, lo, nz, [&](int k) { return live && k < k0; }, x1);
     double fxa[S];
 #pragma unroll
     for (int q = 0; q < S; ++q) fxa[q] = 0.5 * (ka[q] + kn[q]);
@@ -180,8 +140,8 @@ __device__ __forceinline__ void fric_face(const FricArgs& p, SegLane L, int t, i
         ds[q] = (both && k >= k0 && k < nz - 1) ? ((a0 - x0[q]) * ((fxa[q] * (a1 - x1[q])) / dzw)) / dzw : 0.0;
         dm[q] = (x1[q] - x0[q]) / dt;
     }
-    fric_store<S, VEC>(p.dmix[t] + base, lo, nz, dm);
-    fric_store<S, VEC>(p.diss[t] + base, lo, nz, ds);
+    col_store<S, VEC>(p.dmix[t] + base, 1, lo, nz, dm);
+    col_store<S, VEC>(p.diss[t] + base, 1, lo, nz, ds);
 }
 
 // VEC: levels outermost, nz even and every array 16-byte aligned.

@@ -492,6 +492,67 @@ __device__ __forceinline__ void seg_store(double* __restrict__ col, int lo, int 
     }
 }
 
+// Levels lo .. lo+S-1 of one column of an array whose level k sits at
+// a[cs * k]: VEC uses 16-byte loads of level pairs (cs = 1, lo and nz even,
+// 16-byte aligned column). Levels for which want(k) is false (or k >= nz)
+// keep their values.
+template <int S, bool VEC, class W>
+__device__ __forceinline__ void col_load(const double* __restrict__ a, int64_t cs, int lo, int nz, W want,
+                                         double (&v)[S]) {
+    if constexpr (VEC) {
+#pragma unroll
+        for (int t = 0; t < S; t += 2) {
+            const int k = lo + t;
+            if (k < nz && (want(k) || want(k + 1))) {
+                const double2 w = *reinterpret_cast<const double2*>(a + k);
+                v[t] = w.x;
+                v[t + 1] = w.y;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < S; ++t) {
+            const int k = lo + t;
+            if (k < nz && want(k)) v[t] = a[cs * k];
+        }
+    }
+}
+
+// Stores levels lo .. lo+S-1 (k < nz) of one column (layout as col_load).
+template <int S, bool VEC>
+__device__ __forceinline__ void col_store(double* __restrict__ a, int64_t cs, int lo, int nz, const double (&v)[S]) {
+    if constexpr (VEC) {
+#pragma unroll
+        for (int t = 0; t < S; t += 2) {
+            if (lo + t < nz) *reinterpret_cast<double2*>(a + lo + t) = make_double2(v[t], v[t + 1]);
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < S; ++t) {
+            if (lo + t < nz) a[cs * (lo + t)] = v[t];
+        }
+    }
+}
+
+// Stores the levels with st[t] set; with VEC a pair is stored when either
+// level is, so the other level's v must then hold the value already there.
+template <int S, bool VEC>
+__device__ __forceinline__ void col_store_if(double* __restrict__ a, int64_t cs, int lo, int nz, const double (&v)[S],
+                                             const bool (&st)[S]) {
+    if constexpr (VEC) {
+#pragma unroll
+        for (int t = 0; t < S; t += 2) {
+            if (lo + t < nz && (st[t] || st[t + 1]))
+                *reinterpret_cast<double2*>(a + lo + t) = make_double2(v[t], v[t + 1]);
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < S; ++t) {
+            if (lo + t < nz && st[t]) a[cs * (lo + t)] = v[t];
+        }
+    }
+}
+
 template <int S, bool VEC>
 __device__ __forceinline__ void seg_rows(const TdmaArgs& p, const SegLane& L, double (&va)[S], double (&vb)[S],
                                          double (&vc)[S], double (&vd)[S]) {

@@ -92,23 +92,52 @@ __device__ __forceinline__ double lateral_div(const TurbGrid& g, int i, int j, i
     return (fe - fw) / (cost * g.dxt[i]) + (fn - fs) / (cost * g.dyt[j]);
 }
 
-// levels lo-1 .. lo+S of a (X, Y, nz) array at column `base` (0 outside [0, nz))
-template <int S>
-__device__ __forceinline__ void load_halo(const double* __restrict__ p, int64_t base, int lo, int nz,
+// levels lo-1 .. lo+S of a (X, Y, nz) array at column `base` (0 outside
+// [0, nz)); with VEC the lane loads its own levels in pairs and takes lo-1
+// and lo+S from the lanes below and above (every lane must call it).
+template <int S, bool VEC>
+__device__ __forceinline__ void load_halo(const double* __restrict__ p, int64_t base, int s, int lo, int nz,
                                           double (&v)[S + 2]) {
+    if constexpr (VEC) {
+        double w[S];
 #pragma unroll
-    for (int t = 0; t < S + 2; ++t) {
-        const int k = lo - 1 + t;
-        v[t] = (k >= 0 && k < nz) ? p[base + k] : 0.0;
+        for (int t = 0; t < S; ++t) w[t] = 0.0;
+        col_load<S, true>(p + base, 1, lo, nz, [](int) { return true; }, w);
+        v[0] = grp_prev(s, w[S - 1]);
+        v[S + 1] = grp_next(s, w[0]);
+#pragma unroll
+        for (int t = 0; t < S; ++t) v[t + 1] = w[t];
+    } else {
+#pragma unroll
+        for (int t = 0; t < S + 2; ++t) {
+            const int k = lo - 1 + t;
+            v[t] = (k >= 0 && k < nz) ? p[base + k] : 0.0;
+        }
     }
 }
 
-template <int S>
+// levels lo .. lo+S-1 (0 from nz on)
+template <int S, bool VEC>
 __device__ __forceinline__ void load_lv(const double* __restrict__ p, int64_t base, int lo, int nz, double (&v)[S]) {
 #pragma unroll
-    for (int t = 0; t < S; ++t) {
-        const int k = lo + t;
-        v[t] = k < nz ? p[base + k] : 0.0;
+    for (int t = 0; t < S; ++t) v[t] = 0.0;
+    col_load<S, VEC>(p + base, 1, lo, nz, [](int) { return true; }, v);
+}
+
+// sum of the forcing terms at levels lo .. lo+S-1 where want(k), summed
+// left to right as sum_terms (0 elsewhere)
+template <int S, bool VEC, class W>
+__device__ __forceinline__ void load_terms(const Terms& T, int64_t base, int lo, int nz, W want, double (&f)[S]) {
+#pragma unroll
+    for (int t = 0; t < S; ++t) f[t] = 0.0;
+    for (int q = 0; q < T.n; ++q) {
+        double v[S];
+#pragma unroll
+        for (int t = 0; t < S; ++t) v[t] = 0.0;
+        col_load<S, VEC>(T.p[q] + base, 1, lo, nz, want, v);
+        const double sg = T.sgn[q];
+#pragma unroll
+        for (int t = 0; t < S; ++t) f[t] = q == 0 ? sg * v[t] : f[t] + sg * v[t];
     }
 }
 
@@ -126,36 +155,44 @@ struct TkeArgs {
     int hor;
 };
 
-template <int S>
+// VEC (all three kernels): levels outermost, nz even and every array
+// 16-byte aligned; the lanes then move their levels in 16-byte pairs.
+template <int S, bool VEC>
 __global__ __launch_bounds__(256) void tke_kernel(TkeArgs a) {
     const TurbGrid& g = a.g;
     const TurbCol tc = turb_col(g, S);
-    const int nz = g.nz;
+    const int nz = g.nz, lo = tc.lo, k0 = tc.k0;
     const int64_t base = tc.col * nz;
     double km[S + 2];
-    load_halo<S>(a.kappaM, base, tc.lo, nz, km);
+    load_halo<S, VEC>(a.kappaM, base, tc.s, lo, nz, km);
     double sq[S], ml[S];
-    load_lv<S>(a.sqrttke, base, tc.lo, nz, sq);
-    load_lv<S>(a.mxl, base, tc.lo, nz, ml);
+    load_lv<S, VEC>(a.sqrttke, base, lo, nz, sq);
+    load_lv<S, VEC>(a.mxl, base, lo, nz, ml);
     const double* __restrict__ E0 = a.tke + a.tau * g.ls;
     double* __restrict__ E1 = a.tke + a.taup1 * g.ls;
+    // tau on the wet levels, taup1 on the others (kept there and read by the diagnostic)
+    double e0[S], e1[S], fo[S];
+#pragma unroll
+    for (int t = 0; t < S; ++t) e0[t] = e1[t] = 0.0;
+    col_load<S, VEC>(E0 + base * g.cs, g.cs, lo, nz, [&](int k) { return k >= k0; }, e0);
+    col_load<S, VEC>(E1 + base * g.cs, g.cs, lo, nz, [&](int k) { return tc.valid && k < k0; }, e1);
+    load_terms<S, VEC>(a.forc, base, lo, nz, [&](int k) { return k >= k0; }, fo);
     const double dt = a.dt;
     double va[S], vb[S], vc[S], vd[S];
     auto delta = [&](int t) {  // delta at level lo - 1 + t
-        const int k = tc.lo - 1 + t;
+        const int k = lo - 1 + t;
         return (k >= 0 && k < nz - 1) ? dt / g.dzt[k + 1] * a.alpha * 0.5 * (km[t] + km[t + 1]) : 0.0;
     };
 #pragma unroll
     for (int t = 0; t < S; ++t) {
-        const int k = tc.lo + t;
-        const bool w = tc.valid && k < nz && k >= tc.k0;
+        const int k = lo + t;
+        const bool w = tc.valid && k < nz && k >= k0;
         double A = 0.0, B = 1.0, C = 0.0, D = 0.0;
         if (w) {
             const double dk = delta(t + 1), dm = delta(t);
-            const double forc = sum_terms(a.forc, base + k);
             const double dzw = g.dzw[k];
-            D = E0[(base + k) * g.cs] + dt * forc;
-            if (k == tc.k0) {
+            D = e0[t] + dt * fo[t];
+            if (k == k0) {
                 A = 0.0;
                 B = 1 + dk / dzw + dt * a.c_eps / ml[t] * sq[t];
             } else if (k == nz - 1) {
@@ -175,28 +212,27 @@ __global__ __launch_bounds__(256) void tke_kernel(TkeArgs a) {
     }
     SegLane L;
     L.s = tc.s;
-    L.lo = tc.lo;
-    L.k0 = tc.k0;
+    L.lo = lo;
+    L.k0 = k0;
     L.valid = tc.valid;
     L.base = base;
     seg_solve_f64<S>(L, va, vb, vc, vd);
     if (!tc.valid) return;
+    double dg[S];
+    bool st[S];
 #pragma unroll
     for (int t = 0; t < S; ++t) {
-        const int k = tc.lo + t;
-        if (k >= nz) continue;
-        const int64_t c = base + k;
-        const bool w = k >= tc.k0;
-        double v = w ? vd[t] : E1[c * g.cs];
-        a.diss[c] = a.c_eps / ml[t] * sq[t] * v;  // tke_diss over the whole array (:230)
-        if (!tc.interior) continue;
-        bool store = w;
-        if (k == nz - 1) {  // surface correction and clamp of every interior column (:235-244)
+        const int k = lo + t;
+        const bool w = k >= k0;
+        double v = w ? vd[t] : e1[t];
+        dg[t] = a.c_eps / ml[t] * sq[t] * v;  // tke_diss over the whole array (:230)
+        bool store = tc.interior && w;
+        if (tc.interior && k == nz - 1) {  // surface correction and clamp of every interior column (:235-244)
             a.surf_corr[tc.col] = v < 0.0 ? -v * 0.5 * g.dzw[k] / dt : 0.0;
             v = fmax(0.0, v);
             store = true;
         }
-        if (a.hor && wet_at(tc.kb, k)) {
+        if (tc.interior && k < nz && a.hor && wet_at(tc.kb, k)) {
             const double Kh = a.K_h;
             v += dt * 1.0 *
                  lateral_div(
@@ -204,8 +240,11 @@ __global__ __launch_bounds__(256) void tke_kernel(TkeArgs a) {
                      [&](int64_t, int64_t) { return Kh; }, [&](int64_t, int64_t) { return Kh; });
             store = true;
         }
-        if (store) E1[c * g.cs] = v;
+        e1[t] = v;
+        st[t] = store;
     }
+    col_store<S, VEC>(a.diss + base, 1, lo, nz, dg);
+    col_store_if<S, VEC>(E1 + base * g.cs, g.cs, lo, nz, e1, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -221,38 +260,44 @@ struct EkeArgs {
     double dt, alpha, c_eps;
 };
 
-template <int S>
+template <int S, bool VEC>
 __global__ __launch_bounds__(256) void eke_kernel(EkeArgs a) {
     const TurbGrid& g = a.g;
     const TurbCol tc = turb_col(g, S);
-    const int nz = g.nz;
+    const int nz = g.nz, lo = tc.lo, k0 = tc.k0;
     const int64_t base = tc.col * nz;
     double km[S + 2];
-    load_halo<S>(a.kappaM, base, tc.lo, nz, km);
+    load_halo<S, VEC>(a.kappaM, base, tc.s, lo, nz, km);
     double sq[S], el[S];
-    load_lv<S>(a.sqrteke, base, tc.lo, nz, sq);
-    load_lv<S>(a.eke_len, base, tc.lo, nz, el);
+    load_lv<S, VEC>(a.sqrteke, base, lo, nz, sq);
+    load_lv<S, VEC>(a.eke_len, base, lo, nz, el);
     double cint[S];  // eke_c_eps * sqrteke / eke_len * maskW (:121)
 #pragma unroll
-    for (int t = 0; t < S; ++t) cint[t] = a.c_eps * sq[t] / el[t] * (wet_at(tc.kb, tc.lo + t) ? 1.0 : 0.0);
+    for (int t = 0; t < S; ++t) cint[t] = a.c_eps * sq[t] / el[t] * (wet_at(tc.kb, lo + t) ? 1.0 : 0.0);
     const double* __restrict__ E0 = a.eke + a.tau * g.ls;
     double* __restrict__ E1 = a.eke + a.taup1 * g.ls;
+    double e0[S], e1[S], fo[S];
+#pragma unroll
+    for (int t = 0; t < S; ++t) e0[t] = e1[t] = 0.0;
+    col_load<S, VEC>(E0 + base * g.cs, g.cs, lo, nz, [&](int k) { return k >= k0; }, e0);
+    col_load<S, VEC>(E1 + base * g.cs, g.cs, lo, nz, [&](int k) { return tc.valid && k < k0; }, e1);
+    load_terms<S, VEC>(a.forc, base, lo, nz, [&](int k) { return k >= k0; }, fo);
     const double dt = a.dt;
     double va[S], vb[S], vc[S], vd[S];
     auto delta = [&](int t) {
-        const int k = tc.lo - 1 + t;
+        const int k = lo - 1 + t;
         return (k >= 0 && k < nz - 1) ? dt / g.dzt[k + 1] * 0.5 * (km[t] + km[t + 1]) * a.alpha : 0.0;
     };
 #pragma unroll
     for (int t = 0; t < S; ++t) {
-        const int k = tc.lo + t;
-        const bool w = tc.valid && k < nz && k >= tc.k0;
+        const int k = lo + t;
+        const bool w = tc.valid && k < nz && k >= k0;
         double A = 0.0, B = 1.0, C = 0.0, D = 0.0;
         if (w) {
             const double dk = delta(t + 1), dm = delta(t);
             const double dzw = g.dzw[k];
-            D = E0[(base + k) * g.cs] + dt * sum_terms(a.forc, base + k);
-            if (k == tc.k0) {
+            D = e0[t] + dt * fo[t];
+            if (k == k0) {
                 B = 1 + dk / dzw + dt * cint[t];
             } else if (k == nz - 1) {
                 A = -dm / (0.5 * dzw);
@@ -270,30 +315,35 @@ __global__ __launch_bounds__(256) void eke_kernel(EkeArgs a) {
     }
     SegLane L;
     L.s = tc.s;
-    L.lo = tc.lo;
-    L.k0 = tc.k0;
+    L.lo = lo;
+    L.k0 = k0;
     L.valid = tc.valid;
     L.base = base;
     seg_solve_f64<S>(L, va, vb, vc, vd);
     if (!tc.valid) return;
+    double di[S], zero[S];
+    bool st[S];
 #pragma unroll
     for (int t = 0; t < S; ++t) {
-        const int k = tc.lo + t;
-        if (k >= nz) continue;
-        const int64_t c = base + k;
-        const bool w = k >= tc.k0;
-        double v = w ? vd[t] : E1[c * g.cs];
-        a.diss_iw[c] = cint[t] * v;  // (:151)
-        a.diss_tke[c] = 0.0;         // (:152)
-        if (!tc.interior || !wet_at(tc.kb, k)) continue;
-        const double* __restrict__ K = a.K_gm;
-        v += dt * 1.0 *
-             lateral_div(
-                 g, tc.i, tc.j, k, [&](int64_t cc) { return E0[cc * g.cs]; },
-                 [&](int64_t c0, int64_t c1) { return 0.5 * fmax(500.0, K[c0] + K[c1]); },
-                 [&](int64_t c0, int64_t c1) { return 0.5 * fmax(500.0, K[c0] + K[c1]); });
-        E1[c * g.cs] = v;
+        const int k = lo + t;
+        double v = k >= k0 ? vd[t] : e1[t];
+        di[t] = cint[t] * v;  // (:151)
+        zero[t] = 0.0;        // (:152)
+        st[t] = false;
+        if (tc.interior && k < nz && wet_at(tc.kb, k)) {
+            const double* __restrict__ K = a.K_gm;
+            v += dt * 1.0 *
+                 lateral_div(
+                     g, tc.i, tc.j, k, [&](int64_t cc) { return E0[cc * g.cs]; },
+                     [&](int64_t c0, int64_t c1) { return 0.5 * fmax(500.0, K[c0] + K[c1]); },
+                     [&](int64_t c0, int64_t c1) { return 0.5 * fmax(500.0, K[c0] + K[c1]); });
+            st[t] = true;
+        }
+        e1[t] = v;
     }
+    col_store<S, VEC>(a.diss_iw + base, 1, lo, nz, di);
+    col_store<S, VEC>(a.diss_tke + base, 1, lo, nz, zero);
+    col_store_if<S, VEC>(E1 + base * g.cs, g.cs, lo, nz, e1, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -312,33 +362,33 @@ struct IdemixArgs {
     int diss_mode;  // 0: as is; 1: column integral at the bottom cell; 2: split bottom / surface
 };
 
-template <int S>
+template <int S, bool VEC>
 __global__ __launch_bounds__(256) void idemix_kernel(IdemixArgs a) {
     const TurbGrid& g = a.g;
     const TurbCol tc = turb_col(g, S);
     const int nz = g.nz;
     const int64_t base = tc.col * nz;
     double c0[S + 2];
-    load_halo<S>(a.c0, base, tc.lo, nz, c0);
+    load_halo<S, VEC>(a.c0, base, tc.s, tc.lo, nz, c0);
     double al[S];
-    load_lv<S>(a.alpha_c, base, tc.lo, nz, al);
+    load_lv<S, VEC>(a.alpha_c, base, tc.lo, nz, al);
     const double* __restrict__ E0 = a.E + a.tau * g.ls;
     double* __restrict__ E1 = a.E + a.taup1 * g.ls;
     const double dt = a.dt;
-    double mx[S];  // max(0, E_iw[tau]) (:118)
+    // E_iw at tau on every level, at taup1 off the solve (kept there and read by the diagnostic)
+    double e0[S], e1[S], mx[S];
 #pragma unroll
-    for (int t = 0; t < S; ++t) {
-        const int k = tc.lo + t;
-        mx[t] = (tc.valid && k < nz) ? fmax(0.0, E0[(base + k) * g.cs]) : 0.0;
-    }
+    for (int t = 0; t < S; ++t) e0[t] = e1[t] = 0.0;
+    col_load<S, VEC>(E0 + base * g.cs, g.cs, tc.lo, nz, [&](int) { return tc.valid; }, e0);
+    col_load<S, VEC>(E1 + base * g.cs, g.cs, tc.lo, nz, [&](int k) { return tc.valid && k < tc.k0; }, e1);
+#pragma unroll
+    for (int t = 0; t < S; ++t) mx[t] = fmax(0.0, e0[t]);  // max(0, E_iw[tau]) (:118)
     // the forcing of the lane's levels (interior columns; wet levels feed the
     // solve, the column integral sums maskW-weighted levels)
     double fo[S];
+    load_terms<S, VEC>(a.forc, base, tc.lo, nz, [&](int k) { return tc.interior && wet_at(tc.kb, k); }, fo);
 #pragma unroll
-    for (int t = 0; t < S; ++t) {
-        const int k = tc.lo + t;
-        fo[t] = (tc.interior && k < nz && wet_at(tc.kb, k)) ? sum_terms(a.forc, base + k) : 0.0;
-    }
+    for (int t = 0; t < S; ++t) fo[t] = (tc.interior && wet_at(tc.kb, tc.lo + t)) ? fo[t] : 0.0;
     if (a.diss_mode != 0) {  // (:70-103)
         double part = 0.0;
 #pragma unroll
@@ -363,17 +413,15 @@ __global__ __launch_bounds__(256) void idemix_kernel(IdemixArgs a) {
         // the moved term applies to every interior column; rows of other
         // columns are never solved
     }
-    if (a.forc_post.n > 0) {
+    // terms added afterwards, on the levels whose rows are solved
+    for (int q = 0; q < a.forc_post.n; ++q) {
+        double v[S];
 #pragma unroll
-        for (int t = 0; t < S; ++t) {
-            const int k = tc.lo + t;
-            if (tc.interior && k < nz) {
-                const int64_t c = base + k;
-                double f = fo[t];
-                for (int q = 0; q < a.forc_post.n; ++q) f = f + a.forc_post.sgn[q] * a.forc_post.p[q][c];
-                fo[t] = f;
-            }
-        }
+        for (int t = 0; t < S; ++t) v[t] = 0.0;
+        col_load<S, VEC>(a.forc_post.p[q] + base, 1, tc.lo, nz, [&](int k) { return k >= tc.k0; }, v);
+        const double sg = a.forc_post.sgn[q];
+#pragma unroll
+        for (int t = 0; t < S; ++t) fo[t] = fo[t] + sg * v[t];
     }
     double va[S], vb[S], vc[S], vd[S];
     auto delta = [&](int t) {
@@ -390,7 +438,7 @@ __global__ __launch_bounds__(256) void idemix_kernel(IdemixArgs a) {
             const double dzw = g.dzw[k];
             const double diss = dt * al[t] * mx[t];
             const double cc = c0[t + 1];
-            D = E0[(base + k) * g.cs] + dt * fo[t];
+            D = e0[t] + dt * fo[t];
             if (k == tc.k0) {
                 B = 1 + dk / dzw * cc + diss;
                 D = D + dt * a.forc_bot[tc.col] / dzw;  // d_edge (:157-159)
@@ -417,17 +465,15 @@ __global__ __launch_bounds__(256) void idemix_kernel(IdemixArgs a) {
     L.base = base;
     seg_solve_f64<S>(L, va, vb, vc, vd);
     if (!tc.valid) return;
+    double di[S];
+    bool st[S];
 #pragma unroll
     for (int t = 0; t < S; ++t) {
         const int k = tc.lo + t;
-        if (k >= nz) continue;
-        const int64_t c = base + k;
-        const bool w = k >= tc.k0;
-        double v = w ? vd[t] : E1[c * g.cs];
-        a.iw_diss[c] = al[t] * mx[t] * v;  // (:174)
-        if (!tc.interior) continue;
-        bool store = w;
-        if (a.hor && wet_at(tc.kb, k)) {
+        double v = k >= tc.k0 ? vd[t] : e1[t];
+        di[t] = al[t] * mx[t] * v;  // (:174)
+        st[t] = tc.interior && k >= tc.k0;
+        if (tc.interior && k < nz && a.hor && wet_at(tc.kb, k)) {
             const double* __restrict__ V = a.v0;
             const double th = a.tau_h;
             v += dt * 1.0 *
@@ -435,10 +481,12 @@ __global__ __launch_bounds__(256) void idemix_kernel(IdemixArgs a) {
                      g, tc.i, tc.j, k, [&](int64_t cc) { return V[cc] * E0[cc * g.cs]; },
                      [&](int64_t c0_, int64_t c1) { return th * 0.5 * (V[c1] + V[c0_]); },
                      [&](int64_t c0_, int64_t c1) { return th * 0.5 * (V[c1] + V[c0_]); });
-            store = true;
+            st[t] = true;
         }
-        if (store) E1[c * g.cs] = v;
+        e1[t] = v;
     }
+    col_store<S, VEC>(a.iw_diss + base, 1, tc.lo, nz, di);
+    col_store_if<S, VEC>(E1 + base * g.cs, g.cs, tc.lo, nz, e1, st);
 }
 
 namespace {
@@ -481,13 +529,18 @@ int make_terms(const vx_terms* in, Terms& t, bool allow_empty, const char* who) 
     return VX_OK;
 }
 
-template <class Args, template <int> class K>
-int launch_turb(const Args& a, const TurbGrid& g, hipStream_t s, const char* who) {
+template <class Args, template <int, bool> class K>
+int launch_turb(const Args& a, const TurbGrid& g, bool vec, hipStream_t s, const char* who) {
     const int S = seg_levels(g.nz);
     const int64_t threads = (int64_t)g.X * g.Y * kSegP;
     const dim3 grid(blocks_mult8(threads, 256)), block(256);
-#define VX_TURB_CASE(N) \
-    case N: hipLaunchKernelGGL(K<N>::fn, grid, block, 0, s, a); break;
+#define VX_TURB_CASE(N)                                                     \
+    case N:                                                                 \
+        if (vec)                                                            \
+            hipLaunchKernelGGL((K<N, true>::fn), grid, block, 0, s, a);     \
+        else                                                                \
+            hipLaunchKernelGGL((K<N, false>::fn), grid, block, 0, s, a);    \
+        break;
     switch (S) {
         VX_TURB_CASE(2)
         VX_TURB_CASE(4)
@@ -503,18 +556,31 @@ int launch_turb(const Args& a, const TurbGrid& g, hipStream_t s, const char* who
     return check_launch(who);
 }
 
-template <int N>
+template <int N, bool V>
 struct TkeK {
-    static constexpr auto fn = tke_kernel<N>;
+    static constexpr auto fn = tke_kernel<N, V>;
 };
-template <int N>
+template <int N, bool V>
 struct EkeK {
-    static constexpr auto fn = eke_kernel<N>;
+    static constexpr auto fn = eke_kernel<N, V>;
 };
-template <int N>
+template <int N, bool V>
 struct IdemixK {
-    static constexpr auto fn = idemix_kernel<N>;
+    static constexpr auto fn = idemix_kernel<N, V>;
 };
+
+// the 16-byte pair variant: levels outermost, nz even, every array aligned
+bool turb_vec(const TurbGrid& g, std::initializer_list<const void*> arrays, const Terms& t1, const Terms* t2) {
+    if (g.cs != 1 || (g.nz & 1)) return false;
+    for (const void* p : arrays)
+        if (!aligned16(p)) return false;
+    for (int q = 0; q < t1.n; ++q)
+        if (!aligned16(t1.p[q])) return false;
+    if (t2)
+        for (int q = 0; q < t2->n; ++q)
+            if (!aligned16(t2->p[q])) return false;
+    return true;
+}
 
 bool levels_ok(int tau, int taup1) { return tau >= 0 && tau <= 2 && taup1 >= 0 && taup1 <= 2 && tau != taup1; }
 
@@ -541,7 +607,9 @@ extern "C" int vx_integrate_tke_f64(void* stream, const vx_grid* grid, const vx_
     a.surf_corr = tke_surf_corr, a.diss = tke_diss;
     a.dt = prm->dt_mom, a.alpha = prm->alpha_tke, a.c_eps = prm->c_eps, a.K_h = prm->K_h_tke;
     a.hor = prm->hor_diffusion != 0;
-    return launch_turb<TkeArgs, TkeK>(a, a.g, (hipStream_t)stream, who);
+    const bool vec = turb_vec(a.g, {tke, tke + tau * a.g.ls, tke + taup1 * a.g.ls, kappaM, sqrttke, mxl, tke_diss},
+                              a.forc, nullptr);
+    return launch_turb<TkeArgs, TkeK>(a, a.g, vec, (hipStream_t)stream, who);
 }
 
 extern "C" int vx_integrate_eke_f64(void* stream, const vx_grid* grid, const vx_eke_params* prm, double* eke,
@@ -563,7 +631,9 @@ extern "C" int vx_integrate_eke_f64(void* stream, const vx_grid* grid, const vx_
     a.kappaM = kappaM, a.sqrteke = sqrteke, a.eke_len = eke_len, a.K_gm = K_gm;
     a.diss_iw = eke_diss_iw, a.diss_tke = eke_diss_tke;
     a.dt = prm->dt_tracer, a.alpha = prm->alpha_eke, a.c_eps = prm->eke_c_eps;
-    return launch_turb<EkeArgs, EkeK>(a, a.g, (hipStream_t)stream, who);
+    const bool vec = turb_vec(a.g, {eke, eke + tau * a.g.ls, eke + taup1 * a.g.ls, kappaM, sqrteke, eke_len,
+                                    eke_diss_iw, eke_diss_tke}, a.forc, nullptr);
+    return launch_turb<EkeArgs, EkeK>(a, a.g, vec, (hipStream_t)stream, who);
 }
 
 extern "C" int vx_integrate_idemix_f64(void* stream, const vx_grid* grid, const vx_idemix_params* prm, double* E_iw,
@@ -591,5 +661,7 @@ extern "C" int vx_integrate_idemix_f64(void* stream, const vx_grid* grid, const 
     a.dt = prm->dt_tracer, a.tau_v = prm->tau_v, a.tau_h = prm->tau_h, a.frac = prm->eke_diss_surfbot_frac;
     a.hor = prm->hor_diffusion != 0;
     a.diss_mode = prm->eke_diss_mode;
-    return launch_turb<IdemixArgs, IdemixK>(a, a.g, (hipStream_t)stream, who);
+    const bool vec = turb_vec(a.g, {E_iw, E_iw + tau * a.g.ls, E_iw + taup1 * a.g.ls, c0, alpha_c, iw_diss},
+                              a.forc, &a.forc_post);
+    return launch_turb<IdemixArgs, IdemixK>(a, a.g, vec, (hipStream_t)stream, who);
 }
