@@ -32,12 +32,20 @@ STEP_KEYS = ("temp", "salt", "dtemp_iso", "dsalt_iso")
 WINDOWS = ((0, 100, 0, 84), (700, 800, 300, 384), (1344, 1444, 640, 724))
 
 
+# the dissipation terms the bench workload draws from four shared fields
+SHARED = ("K_diss_v", "P_diss_v", "P_diss_adv", "P_diss_nonlin", "eke_diss_tke", "K_diss_bot", "K_diss_gm",
+          "K_diss_h", "P_diss_skew", "P_diss_hmix")
+
+
 @pytest.fixture(scope="module")
 def workload():
     if str(REPO) not in sys.path:
         sys.path.insert(0, str(REPO))
     import bench
     st, td = bench.build_workload(1440, 720, 80)
+    vs = st.variables
+    for k in SHARED:  # independent arrays: the calls below write some of them
+        setattr(vs, k, getattr(vs, k).clone())
     torch.cuda.synchronize()
     yield st, td
     del st, td
@@ -102,3 +110,114 @@ def test_tdma_sampled_rows_against_oracle(workload, precision, refine):
     got = x.index_select(0, idx).cpu().numpy()
     assert rel_err(got, ref) <= TOL
     assert np.all(got[sub["kbot"] == 0] == 0)
+
+
+def _windows_check(st, run, oracle, keys, bar, levels_keys=()):
+    """Run ``run(st)`` on the whole grid and ``oracle(window)`` on each window
+    taken just before; compare the window cells [2:-2, 2:-2] of ``keys``."""
+    before = [W.window_state(st, *w) for w in WINDOWS]
+    run(st)
+    torch.cuda.synchronize()
+    worst = {}
+    for win, hs in zip(WINDOWS, before):
+        got = W.window_state(st, *win, masks=False).variables
+        ref = oracle(hs)
+        for k in keys:
+            g, r = getattr(got, k), getattr(ref, k)
+            if k in levels_keys:
+                g, r = g[..., hs.variables.taup1], r[..., hs.variables.taup1]
+            g, r = g[2:-2, 2:-2], r[2:-2, 2:-2]
+            e = rel_err(g, r)
+            worst[k] = max(worst.get(k, 0.0), e)
+            assert e <= bar(hs, k, r), (win, k, e)
+    return worst
+
+
+def test_vertmix_windows_against_oracle(workload):
+    """vertmix_tempsalt on the whole 1440x720x80 grid; the tendencies are
+    measured in units of the tracer / dt (tests/test_gpu_vertmix.py)."""
+    from oracle import vertmix as OV
+    from veros_amd import thermodynamics
+    st, _ = workload
+    dt = st.settings.dt_tracer
+    scale = {}
+
+    def oracle(hs):
+        ref = OV.vertmix_tempsalt(hs)
+        for k, tr in (("dtemp_vmix", "temp"), ("dsalt_vmix", "salt")):
+            scale[k] = np.abs(getattr(ref, tr)[2:-2, 2:-2, :, hs.variables.taup1]).max() / dt
+        return ref
+
+    def bar(hs, k, r):
+        return TOL * (scale[k] / np.abs(r).max() if k in scale else 1.0)
+
+    worst = _windows_check(st, thermodynamics.vertmix_tempsalt, oracle,
+                           ("temp", "salt", "dtemp_vmix", "dsalt_vmix"), bar, ("temp", "salt"))
+    print("vertmix windows:", {k: f"{v:.2e}" for k, v in worst.items()})
+
+
+def test_friction_windows_against_oracle(workload):
+    """implicit_vert_friction on the whole 1440x720x80 grid; du_mix / dv_mix
+    measured in units of the velocity / dt (tests/test_gpu_friction.py)."""
+    from oracle import friction as OF
+    from veros_amd import friction
+    st, _ = workload
+    dt = st.settings.dt_mom
+    scale = {}
+
+    def oracle(hs):
+        ref = OF.implicit_vert_friction(hs)
+        for k, vel in (("du_mix", "u"), ("dv_mix", "v")):
+            scale[k] = np.abs(getattr(ref, vel)[2:-2, 2:-2, :, hs.variables.taup1]).max() / dt
+        return ref
+
+    def bar(hs, k, r):
+        return TOL * (scale[k] / np.abs(r).max() if k in scale else 1.0)
+
+    worst = _windows_check(st, friction.implicit_vert_friction, oracle, ("u", "v", "du_mix", "dv_mix", "K_diss_v"),
+                           bar, ("u", "v"))
+    print("friction windows:", {k: f"{v:.2e}" for k, v in worst.items()})
+
+
+@pytest.mark.parametrize("model", ["tke", "eke", "idemix"])
+def test_turbulence_windows_against_oracle(workload, model):
+    """TKE, EKE and IDEMIX on the whole 1440x720x80 grid with the settings
+    bench.py times them with; bar max(1e-12, 8 * e_order) as in
+    tests/test_gpu_turbulence.py (e_order: the oracle's change when only its
+    elimination order changes)."""
+    from types import SimpleNamespace
+    from oracle import turbulence as OT_
+    from veros_amd import turbulence
+    st, _ = workload
+    settings = {
+        "tke": W.turbulence_settings(nz=80, enable_eke=True, enable_idemix=True, enable_tke_hor_diffusion=True),
+        "eke": W.turbulence_settings(nz=80, enable_eke=True),
+        "idemix": W.turbulence_settings(nz=80, enable_eke=True, enable_eke_diss_bottom=True,
+                                        enable_idemix_hor_diffusion=True),
+    }[model]
+    fn = {"tke": turbulence.integrate_tke_kernel, "eke": turbulence.integrate_eke_kernel,
+          "idemix": turbulence.integrate_idemix_kernel}[model]
+    ofn = {"tke": OT_.integrate_tke_kernel, "eke": OT_.integrate_eke_kernel,
+           "idemix": OT_.integrate_idemix_kernel}[model]
+    keys, energy = {"tke": (("tke", "tke_surf_corr", "tke_diss"), "tke"),
+                    "eke": (("eke", "eke_diss_iw", "eke_diss_tke"), "eke"),
+                    "idemix": (("E_iw", "iw_diss"), "E_iw")}[model]
+    ts = SimpleNamespace(variables=st.variables, settings=settings)
+    thomas = {}
+
+    def oracle(hs):
+        hs = SimpleNamespace(variables=hs.variables, settings=settings)
+        ref = ofn(hs)
+        th = ofn(hs, solver="thomas")
+        for k in keys:
+            a, b = getattr(th, k), getattr(ref, k)
+            if k == energy:
+                a, b = a[..., hs.variables.taup1], b[..., hs.variables.taup1]
+            thomas[k] = rel_err(a[2:-2, 2:-2], b[2:-2, 2:-2])
+        return ref
+
+    def bar(hs, k, r):
+        return max(TOL, 8 * thomas[k])
+
+    worst = _windows_check(ts, fn, oracle, keys, bar, (energy,))
+    print(model, "windows:", {k: f"{v:.2e}" for k, v in worst.items()})
