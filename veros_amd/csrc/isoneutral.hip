@@ -19,7 +19,7 @@
 #include <map>
 #include <mutex>
 
-#include "vx_common.hpp"
+#include "tdma_seg.hpp"
 
 // Tuning knob (compile-time): minimum waves per SIMD requested for the pre
 // kernel (caps its VGPRs).
@@ -993,6 +993,80 @@ __global__ __launch_bounds__(kWetThreads) void wet_scan_columns_kernel(Grid g, W
     }
 }
 
+// Per-tracer LDS arrays of the tracer step: F (flux_top), D (lateral
+// dissipation), R (rhs -> alpha -> new), Q (c' -> gamma), Wk (spike -> beta).
+constexpr int kStepArrays = 5;
+
+// One column's K33 system (rows a, b, c of n levels in LDS, right-hand side
+// in r) solved in place into r by the kSegP lanes of a group, lane sg owning
+// the rows [sg * S, sg * S + S) with S = max(2, ceil(n / kSegP)); rows from n
+// on are identity rows. Each lane eliminates its interior rows with the spike
+// from the row below (q: c', then gamma; w: the spike, then beta; r: dp, then
+// alpha), the interface rows are solved across the group (iface_solve), and
+// each lane writes its rows of x = alpha + beta L_{s-1} + gamma L_s. Every
+// lane of the group must call it.
+__device__ __forceinline__ void seg_lds_column(const double* a, const double* b, const double* c, double* r,
+                                               double* q, double* w, int n, int sg) {
+    const int S = max(2, (n + kSegP - 1) / kSegP);
+    const int m0 = sg * S;
+    auto A = [&](int m) { return m < n ? a[m] : 0.0; };
+    auto B = [&](int m) { return m < n ? b[m] : 1.0; };
+    auto C = [&](int m) { return m < n ? c[m] : 0.0; };
+    auto D = [&](int m) { return m < n ? r[m] : 0.0; };
+    // forward sweep over the interior rows t = 0 .. S-2 (c', spike, dp to LDS)
+    double cp = 0.0, wp = 0.0, dp = 0.0;
+    for (int t = 0; t < S - 1; ++t) {
+        const int m = m0 + t;
+        const double at = A(m);
+        const double rr = seg_rcp(t == 0 ? B(m) : B(m) - at * cp);
+        cp = C(m) * rr;
+        wp = (t == 0 ? at : -at * wp) * rr;
+        dp = (t == 0 ? D(m) : D(m) - at * dp) * rr;
+        if (m < n) {
+            q[m] = cp;
+            w[m] = wp;
+            r[m] = dp;
+        }
+    }
+    // back substitution: alpha, beta, gamma of the interior rows (rows past
+    // the run have c' = spike = dp = 0)
+    const double alS = dp, beS = -wp, gaS = -cp;  // row S-2
+    if (m0 + S - 2 < n) {
+        r[m0 + S - 2] = alS;
+        w[m0 + S - 2] = beS;
+        q[m0 + S - 2] = gaS;
+    }
+    double al = alS, be = beS, ga = gaS;
+    for (int t = S - 3; t >= 0; --t) {
+        const int m = m0 + t;
+        const bool real = m < n;
+        const double cpt = real ? q[m] : 0.0, dpt = real ? r[m] : 0.0, wpt = real ? w[m] : 0.0;
+        al = dpt - cpt * al;
+        be = -wpt - cpt * be;
+        ga = -cpt * ga;
+        if (real) {
+            r[m] = al;
+            w[m] = be;
+            q[m] = ga;
+        }
+    }
+    // the interface row (row S-1) with the next segment's row 0 by DPP
+    const int mL = m0 + S - 1;
+    const double aL = A(mL), bL = B(mL), cL = C(mL), dL = D(mL);
+    const double n_al = grp_next(sg, al), n_be = grp_next(sg, be), n_ga = grp_next(sg, ga);
+    const double RA = aL * beS;
+    const double RB = bL + aL * gaS + cL * n_be;
+    const double RC = cL * n_ga;
+    const double RD[1] = {dL - aL * alS - cL * n_al};
+    double Ls[1], Lm[1];
+    iface_solve<1>(sg, RA, RB, RC, RD, Ls, Lm);
+    for (int t = 0; t < S - 1; ++t) {
+        const int m = m0 + t;
+        if (m < n) r[m] = r[m] + w[m] * Lm[0] + q[m] * Ls[0];
+    }
+    if (mL < n) r[mL] = Ls[0];
+}
+
 // Persistent grid (a multiple of 8 blocks): the blocks of XCD x (blockIdx % 8)
 // share the groups [x, x+1) * ngroups / 8, so neighbouring rows share an L2,
 // and take them one at a time from that XCD's counter.
@@ -1011,13 +1085,13 @@ __global__ __launch_bounds__(kWetThreads) __attribute__((amdgpu_waves_per_eu(VX_
     extern __shared__ double lds[];
     constexpr int T = kWetThreads;
     // LDS: rows A, B, C (shared by the tracers), then per tracer F (flux_top),
-    // D (lateral dissipation), R (rhs -> dp -> new), Q (cp), each T cells;
-    // then the group's column table
+    // D (lateral dissipation), R (rhs -> alpha -> new), Q (c' -> gamma), Wk
+    // (spike -> beta), each T cells; then the group's column table
     double* sA = lds;
     double* sB = lds + T;
     double* sC = lds + 2 * T;
     double* sT = lds + 3 * T;
-    int* sOff = reinterpret_cast<int*>(lds + (3 + 4 * NTR) * T);  // [T+1] first cell of each column
+    int* sOff = reinterpret_cast<int*>(lds + (3 + kStepArrays * NTR) * T);  // [T+1] first cell of each column
     int* sCol = sOff + (T + 1);                                   // [T]   its column index
     const int nz = g.nz;
     const int tid = (int)threadIdx.x;
@@ -1119,7 +1193,7 @@ __global__ __launch_bounds__(kWetThreads) __attribute__((amdgpu_waves_per_eu(VX_
         if (have) {
 #pragma unroll
             for (int t = 0; t < NTR; ++t) {
-                double* sTt = sT + t * 4 * T;
+                double* sTt = sT + t * kStepArrays * T;
                 sTt[lo] = ft_c[t];
                 if (energy) {
                     // compute_dissipation (veros/core/diffusion.py:13-33)
@@ -1149,7 +1223,7 @@ __global__ __launch_bounds__(kWetThreads) __attribute__((amdgpu_waves_per_eu(VX_
         for (int t = 0; t < NTR; ++t) {
             dtrk[t] = rhs[t] = 0.0;
             if (interior) {
-                double* sTt = sT + t * 4 * T;
+                double* sTt = sT + t * kStepArrays * T;
                 // below the lowest wet cell: with the fused pre's fluxes the
                 // top face there is zero (its maskW), else it was read above
                 const double ftm = (k >= kb) ? sTt[lo - 1] : ft_b[t];
@@ -1163,30 +1237,21 @@ __global__ __launch_bounds__(kWetThreads) __attribute__((amdgpu_waves_per_eu(VX_
         }
         __syncthreads();
 
-        // ---- phase 2: a lane per (interior column, tracer) sweeps in LDS
-        for (int lin = tid; IMPLICIT && lin < ncolg * NTR; lin += T) {
-            const int cq = lin % ncolg, t = lin / ncolg;
-            int ci, cj;
-            wet_col_ij(g, sCol[cq], ci, cj);
-            if (ci < 2 || ci > g.X - 3 || cj < 2 || cj > g.Y - 3) continue;
-            const int b0 = sOff[cq], b1 = sOff[cq + 1];
-            double* R = sT + t * 4 * T + 2 * T;
-            double* Q = sT + t * 4 * T + 3 * T;
-            double cpp = 0.0, dpp = 0.0;
-#pragma unroll 8
-            for (int kk = b0; kk < b1; ++kk) {
-                const double ak = sA[kk];
-                const double r = rcp_nr(sB[kk] - ak * cpp);
-                cpp = sC[kk] * r;
-                dpp = (R[kk] - ak * dpp) * r;
-                Q[kk] = cpp;
-                R[kk] = dpp;
-            }
-            double xn = 0.0;
-#pragma unroll 8
-            for (int kk = b1 - 1; kk >= b0; --kk) {
-                xn = R[kk] - Q[kk] * xn;
-                R[kk] = xn;
+        // ---- phase 2: a group of kSegP lanes per (interior column, tracer)
+        // solves the column's run in LDS by the segmented elimination
+        // (tdma_seg.hpp), segments of S = max(2, ceil(n / kSegP)) rows, rows
+        // past the run identity rows
+        if (IMPLICIT) {
+            const int sg = tid & (kSegP - 1);
+            for (int task = tid / kSegP; task < ncolg * NTR; task += T / kSegP) {
+                const int cq = task % ncolg, t = task / ncolg;
+                int ci, cj;
+                wet_col_ij(g, sCol[cq], ci, cj);
+                if (ci < 2 || ci > g.X - 3 || cj < 2 || cj > g.Y - 3) continue;  // uniform in the group
+                const int b0 = sOff[cq], n = sOff[cq + 1] - b0;
+                double* sTt = sT + t * kStepArrays * T;
+                seg_lds_column(sA + b0, sB + b0, sC + b0, sTt + 2 * T + b0, sTt + 3 * T + b0, sTt + 4 * T + b0, n,
+                               sg);
             }
         }
         __syncthreads();
@@ -1196,7 +1261,7 @@ __global__ __launch_bounds__(kWetThreads) __attribute__((amdgpu_waves_per_eu(VX_
             double Pk = P0;
 #pragma unroll
             for (int t = 0; t < NTR; ++t) {
-                const double* sTt = sT + t * 4 * T;
+                const double* sTt = sT + t * kStepArrays * T;
                 if (interior) {
                     const double newk = sTt[2 * T + lo];
                     a.t[t].tr[g.cs * c + (uint32_t)a.taup1 * g.ls] = newk;
@@ -1645,7 +1710,8 @@ int launch_wet_step(const Grid& g, const Prm& p, const StepArgs& a, const WetPla
                     const char* who) {
     const int ncu = device_cu_count();
     const unsigned nblk = (unsigned)(((ncu > 0 ? ncu : 256) * VX_WET_BPC + 7) / 8 * 8);
-    const size_t lds = (size_t)(3 + 4 * NTR) * kWetThreads * sizeof(double) + (2 * kWetThreads + 1) * sizeof(int);
+    const size_t lds = (size_t)(3 + kStepArrays * NTR) * kWetThreads * sizeof(double) +
+                       (2 * kWetThreads + 1) * sizeof(int);
     hipLaunchKernelGGL((iso_step_wet_kernel<NTR, IMPLICIT, WETFLUX>), dim3(nblk), dim3(kWetThreads), lds, s, g, p, a,
                        wp);
     return check_launch(who);

@@ -185,7 +185,7 @@ __device__ __forceinline__ T grp_prev(int s, T v) {
     return s == 0 ? T(0) : w;
 }
 
-// The float64 interface system: parallel cyclic reduction, then, in a wave
+// The float64 interface system: parallel cyclic reduction, then, in a group
 // where some interface row is not strongly diagonally dominant (|A| + |C| >
 // |B| / 4), one step of iterative refinement (the float64 residual of the
 // interface rows, neighbours by DPP, solved with the same factors and added).
@@ -194,7 +194,8 @@ __device__ __forceinline__ T grp_prev(int s, T v) {
 // with large viscosity: 4e-12 relative in u, against 3e-14 after the
 // refinement step, the accuracy of a sequential sweep of the interface rows);
 // on strongly dominant rows it is as accurate as the sweep. The branch is
-// wave-uniform, so every lane takes part in the moves. NR right-hand sides
+// taken by the whole wave when any of its groups needs the step, so every
+// lane takes part in the moves. NR right-hand sides
 // share the factors; Ls[q] is this lane's unknown, Lm[q] the one of the lane
 // below (0 in lane 0).
 template <int NR>
@@ -203,11 +204,19 @@ __device__ __forceinline__ void iface_solve(int s, double A, double B, double C,
     PcrFactors<double> F;
 #pragma unroll
     for (int q = 0; q < NR; ++q) Ls[q] = q == 0 ? pcr_solve<double>(s, A, B, C, R[0], &F) : pcr_resolve(s, R[q], F);
-    if (__any(fabs(A) + fabs(C) > 0.25 * fabs(B))) {
+    const bool weak = fabs(A) + fabs(C) > 0.25 * fabs(B);
+    if (__any(weak)) {
+        // the step is kept only in groups with such a row, so a column's
+        // result does not depend on the columns that share its wave
+        int gw = weak ? 1 : 0;
+        gw |= __shfl_xor(gw, 1, kSegP);
+        gw |= __shfl_xor(gw, 2, kSegP);
+        gw |= __shfl_xor(gw, 4, kSegP);
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
             const double res = R[q] - (A * grp_prev(s, Ls[q]) + B * Ls[q] + C * grp_next(s, Ls[q]));
-            Ls[q] = Ls[q] + pcr_resolve(s, res, F);
+            const double ref = Ls[q] + pcr_resolve(s, res, F);
+            Ls[q] = gw ? ref : Ls[q];
         }
     }
 #pragma unroll
