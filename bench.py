@@ -643,7 +643,11 @@ def main():
                 "measured_traffic": None if traffic is None else {
                     "bytes": traffic, "achieved": round(traffic / (t_dom * 1e-3) / 1e9, 1),
                     "frac": round(traffic / (t_dom * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}},
-            "copy_GBps": round(copy_gbps, 1), "frac_of_copy": round(gbs[dom] / copy_gbps, 4)}
+            "copy_GBps": round(copy_gbps, 1), "frac_of_copy": round(gbs[dom] / copy_gbps, 4),
+            "note": "achieved/frac count the per-cell bytes of pre + T + S as separate calls (the tensor written "
+                    "once and read back for each tracer); the one call never reads the tensor back, so they can "
+                    "exceed the peak: fused_minimum (the call's own minimum bytes) and measured_traffic (HBM "
+                    "counters) give the bytes it moves"}
 
     # the 360x160x60 ACC-like channel (FP64), isoneutral one call
     st1 = to_device_state(W.iso_config1())

@@ -1765,22 +1765,27 @@ int launch_pre(const Grid& g, const Prm& p, const Fields& f, const double* K_iso
     // without the helper stream (use_aux false) the zero fill runs on `s`
     const hipStream_t sd = use_aux ? fork_aux(s) : s;
     if (dry_stream) *dry_stream = sd;
-    // Blocks per CU of the zero fill (0: one wave per column). Beside the
-    // whole one call, one block per CU fills in the background at a modest
-    // store rate instead of flooding the memory system ahead of the wet-cell
-    // kernel (measured on 1440x720x80 with the nontemporal stores: 7.5 ms for
-    // the call, 8.2 ms with one wave per column). Pre alone is joined at the
-    // end of pre, so there the fill must not start late: one wave per column
-    // (4.6 ms; one block per CU gave 4.26 ms on one box and a 5.1 ms median on
-    // another, where the fill's blocks were dispatched behind the wet cells').
-#ifndef VX_DRY_BPC_PRE
-#define VX_DRY_BPC_PRE 0
+    // Blocks of the zero fill, in eighths of the CU count (0: one wave per
+    // column). Beside the whole one call, fewer blocks than CUs fill in the
+    // background at a modest store rate instead of flooding the memory system
+    // ahead of the wet-cell kernels (measured on 1440x720x80 with the
+    // nontemporal stores, two runs each: 96 blocks 6.34 / 5.61 ms, 128 blocks
+    // 5.95 / 5.79, 160 blocks 5.80 / 5.73, 192 blocks 5.80 / 6.38, one block
+    // per CU 6.90 / 6.61, two per CU 7.05, one wave per column 6.98). Pre
+    // alone is joined at the end of pre, so there the fill must not start
+    // late: one wave per column (4.6 ms; one block per CU gave 4.26 ms on one
+    // box and a 5.1 ms median on another, where the fill's blocks were
+    // dispatched behind the wet cells').
+#ifndef VX_DRY_EIGHTHS_PRE
+#define VX_DRY_EIGHTHS_PRE 0
 #endif
-#ifndef VX_DRY_BPC_FUSED
-#define VX_DRY_BPC_FUSED 1
+#ifndef VX_DRY_EIGHTHS_FUSED
+#define VX_DRY_EIGHTHS_FUSED 5
 #endif
-    const unsigned bpc = dry_stream ? VX_DRY_BPC_FUSED : VX_DRY_BPC_PRE;
-    const unsigned dry_full = blocks_for(ncol, 4), ndry = bpc * (unsigned)device_cu_count();
+    // (on the caller's stream, without the helper, the fill runs before pre:
+    // the whole grid of it)
+    const unsigned eighths = (dry_stream && use_aux) ? VX_DRY_EIGHTHS_FUSED : VX_DRY_EIGHTHS_PRE;
+    const unsigned dry_full = blocks_for(ncol, 4), ndry = eighths * (unsigned)device_cu_count() / 8;
     const unsigned dry_grid = ndry > 0 && ndry < dry_full ? ndry : dry_full;
     hipLaunchKernelGGL(iso_pre_dry_kernel, dim3(dry_grid), dim3(256), 0, sd, g, Ai_ez, Ai_nz, Ai_bx, Ai_by,
                        K_11, K_22, K_33);
