@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 # bytes = 1.0 x WRITE_SIZE, calibrated on a 1 GiB device copy), per layout of
 # the time-level variables
 # (the levels-innermost figure predates the segmented K33 solve of the step kernel)
-PRE_TS_TRAFFIC_BYTES = {(1440, 720, 80, "outer"): 21_476_350_976, (1440, 720, 80, "inner"): 27_204_216_832}
+PRE_TS_TRAFFIC_BYTES = {(1440, 720, 80, "outer"): 21_432_116_224, (1440, 720, 80, "inner"): 27_204_216_832}
 # the minimum bytes per interior cell of pre + the temperature + salinity step
 # in one call: K_iso read (8), Ai_* and K_11/22/33 written once and not read
 # back (128 + 24), P_diss read and written (16), and per tracer tr[tau] (8),
