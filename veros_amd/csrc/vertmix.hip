@@ -198,11 +198,13 @@ __global__ __launch_bounds__(kSegLdsBlock) void vertmix_segl_kernel(VmixArgs p) 
     const int nz = p.nz;
     double* __restrict__ T = p.tr[0] + p.taup1 * p.ls;
     double* __restrict__ Sa = p.tr[1] + p.taup1 * p.ls;
-    SegCopyPlan<S> wet;
-    seg_copy_plan<S>(L, threadIdx.x & 63, nz, true, wet);
-    seg_lds_copy<S>(p.kappaH, lds + 0 * IMG, wet);
-    seg_lds_copy<S>(T, lds + 1 * IMG, wet);
-    seg_lds_copy<S>(Sa, lds + 2 * IMG, wet);
+    {
+        SegCopyPlan<S> wet;
+        seg_copy_plan<S>(L, threadIdx.x & 63, nz, true, wet);
+        seg_lds_copy<S>(p.kappaH, lds + 0 * IMG, wet);
+        seg_lds_copy<S>(T, lds + 1 * IMG, wet);
+        seg_lds_copy<S>(Sa, lds + 2 * IMG, wet);
+    }
     const bool solve = L.k0 < nz;
     const double fT = solve ? p.forc[0][col] : 0.0, fS = solve ? p.forc[1][col] : 0.0;
     seg_wait_vmem();
@@ -228,6 +230,9 @@ __global__ __launch_bounds__(kSegLdsBlock) void vertmix_segl_kernel(VmixArgs p) 
         oT[t] = w ? (nT - oT[t]) / dt : 0.0;  // (new - old) / dt; 0 where unchanged
         oS[t] = w ? (nS - oS[t]) / dt : 0.0;
     }
+    // the copy plans are formed again here rather than kept through the solve
+    SegCopyPlan<S> wet;
+    seg_copy_plan<S>(L, threadIdx.x & 63, nz, true, wet);
     vmix_store_pairs<S>(lds + 0 * IMG, wet, T, L, z1);
     vmix_store_pairs<S>(lds + 1 * IMG, wet, Sa, L, z2);
     SegCopyPlan<S> all;
