@@ -1630,13 +1630,17 @@ int device_cu_count() {
 }
 
 #ifndef VX_PRE_CELLS
-#define VX_PRE_CELLS 1920
+#define VX_PRE_CELLS 3840
 #endif
 // Columns per block of the wet-cell pre kernel: about VX_PRE_CELLS cells
-// (24 columns at nz = 80), so that a block's wet cells fill its 256 threads
-// for a few rounds at typical land fractions (measured on 1440x720x80 with 31%
-// wet cells: 240 cells per block 11.7 ms for pre + the temperature/salinity
-// step, 640 10.1 ms, 1920 9.8 ms).
+// (48 columns at nz = 80), so that a block's wet cells fill its 256 threads
+// for several rounds at typical land fractions and the partly filled last
+// round wastes few lanes (on 1440x720x80 with 31% wet cells a block holds
+// about 1180 wet cells: 5 rounds at 92% of the lanes, against 3 at 77% with
+// 24 columns). Measured there, pre + the temperature/salinity step in one
+// call: 240 cells per block 11.7 ms, 640 10.1 ms, 1920 9.8 ms (an earlier
+// build); 1920 against 3840 cells, four runs each alternating on one box:
+// 6.49 / 6.23 / 5.63 / 6.39 against 6.17 / 5.66 / 5.99 / 5.50 ms.
 int pre_columns(int nz) {
     const int n = (VX_PRE_CELLS + nz - 1) / nz;
     return n < 1 ? 1 : (n > 64 ? 64 : n);
