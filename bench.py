@@ -582,7 +582,7 @@ def main():
     B_td, W_td = tdma_bytes(td["kbot"], nz)
     C = nx * ny
     cols_halo = X * Y
-    # SURVEY-style algorithmic bytes per interior cell: pre 176 (T, S, K_iso
+    # algorithmic bytes per interior cell with each call counted on its own: pre 176 (T, S, K_iso
     # read; Ai_* and K_11/22/33 written), each tracer's isoneutral_diffusion
     # 200 (tr[tau], tr[taup1], K_iso, Ai_*, K_*, dtr read; tr[taup1], dtr
     # written) + 24 with the energy branch; 4 B of kbot per column and kernel
