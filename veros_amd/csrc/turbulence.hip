@@ -92,6 +92,64 @@ __device__ __forceinline__ double lateral_div(const TurbGrid& g, int i, int j, i
     return (fe - fw) / (cost * g.dxt[i]) + (fn - fs) / (cost * g.dyt[j]);
 }
 
+__device__ __forceinline__ double2 ld2(const double* __restrict__ p) { return *reinterpret_cast<const double2*>(p); }
+
+// lateral_div at the levels k and k+1 together: F2 and the coefficient
+// functors return the two levels' values from 16-byte loads (levels
+// outermost, k even), the arithmetic of each level that of lateral_div.
+template <class FV2, class CE2, class CN2>
+__device__ __forceinline__ double2 lateral_div_pair(const TurbGrid& g, int i, int j, int k, FV2 F, CE2 ce, CN2 cn) {
+    const int64_t Y = g.Y, nz = g.nz;
+    const int64_t c = ((int64_t)i * Y + j) * nz + k;
+    const int64_t cE = c + Y * nz, cW = c - Y * nz, cN = c + nz, cS = c - nz;
+    const int kC = g.kbot[(int64_t)i * Y + j], kE = g.kbot[(int64_t)(i + 1) * Y + j];
+    const int kW = g.kbot[(int64_t)(i - 1) * Y + j], kN = g.kbot[(int64_t)i * Y + j + 1];
+    const int kS = g.kbot[(int64_t)i * Y + j - 1];
+    const double cost = g.cost[j], dxu = g.dxu[i], dxuw = g.dxu[i - 1], dyu = g.dyu[j], dyus = g.dyu[j - 1];
+    const double cosu = g.cosu[j], cosus = g.cosu[j - 1], dxt = g.dxt[i], dyt = g.dyt[j];
+    const double2 Fc = F(c), Fe = F(cE), Fw = F(cW), Fn = F(cN), Fs = F(cS);
+    const double2 ceE = ce(c, cE), ceW = ce(cW, c), cnN = cn(c, cN), cnS = cn(cS, c);
+    double r[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int kk = k + h;
+        const bool mC = wet_at(kC, kk), mE = wet_at(kE, kk), mW = wet_at(kW, kk);
+        const bool mN = wet_at(kN, kk), mS = wet_at(kS, kk);
+        const double fc = h ? Fc.y : Fc.x, fe_ = h ? Fe.y : Fe.x, fw_ = h ? Fw.y : Fw.x;
+        const double fn_ = h ? Fn.y : Fn.x, fs_ = h ? Fs.y : Fs.x;
+        const double fe = (h ? ceE.y : ceE.x) * (fe_ - fc) / (cost * dxu) * ((mC && mE) ? 1.0 : 0.0);
+        const double fw = (h ? ceW.y : ceW.x) * (fc - fw_) / (cost * dxuw) * ((mW && mC) ? 1.0 : 0.0);
+        const double fn = (h ? cnN.y : cnN.x) * (fn_ - fc) / dyu * ((mC && mN) ? 1.0 : 0.0) * cosu;
+        const double fs = (h ? cnS.y : cnS.x) * (fc - fs_) / dyus * ((mS && mC) ? 1.0 : 0.0) * cosus;
+        r[h] = (fe - fw) / (cost * dxt) + (fn - fs) / (cost * dyt);
+    }
+    return make_double2(r[0], r[1]);
+}
+
+// The lateral term of the lane's levels where need(k) (0 elsewhere): per
+// level with lateral_div, or with VEC per level pair with lateral_div_pair.
+template <int S, bool VEC, class NEED, class L1, class L2>
+__device__ __forceinline__ void lateral_levels(int lo, NEED need, L1 one, L2 pair, double (&lat)[S]) {
+#pragma unroll
+    for (int t = 0; t < S; ++t) lat[t] = 0.0;
+    if constexpr (VEC) {
+#pragma unroll
+        for (int t = 0; t < S; t += 2) {
+            const int k = lo + t;
+            if (need(k) || need(k + 1)) {
+                const double2 v = pair(k);
+                lat[t] = v.x;
+                lat[t + 1] = v.y;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < S; ++t) {
+            if (need(lo + t)) lat[t] = one(lo + t);
+        }
+    }
+}
+
 // levels lo-1 .. lo+S of a (X, Y, nz) array at column `base` (0 outside
 // [0, nz)); with VEC the lane loads its own levels in pairs and takes lo-1
 // and lo+S from the lanes below and above (every lane must call it).
@@ -218,6 +276,25 @@ __global__ __launch_bounds__(256) void tke_kernel(TkeArgs a) {
     L.base = base;
     seg_solve_f64<S>(L, va, vb, vc, vd);
     if (!tc.valid) return;
+    // the lateral diffusion of the tau level (:250-276)
+    double lat[S];
+    {
+        const double Kh = a.K_h;
+        lateral_levels<S, VEC>(
+            lo, [&](int k) { return tc.interior && a.hor && k < nz && wet_at(tc.kb, k); },
+            [&](int k) {
+                return lateral_div(
+                    g, tc.i, tc.j, k, [&](int64_t cc) { return E0[cc * g.cs]; },
+                    [&](int64_t, int64_t) { return Kh; }, [&](int64_t, int64_t) { return Kh; });
+            },
+            [&](int k) {
+                return lateral_div_pair(
+                    g, tc.i, tc.j, k, [&](int64_t cc) { return ld2(E0 + cc); },
+                    [&](int64_t, int64_t) { return make_double2(Kh, Kh); },
+                    [&](int64_t, int64_t) { return make_double2(Kh, Kh); });
+            },
+            lat);
+    }
     double dg[S];
     bool st[S];
 #pragma unroll
@@ -233,11 +310,7 @@ __global__ __launch_bounds__(256) void tke_kernel(TkeArgs a) {
             store = true;
         }
         if (tc.interior && k < nz && a.hor && wet_at(tc.kb, k)) {
-            const double Kh = a.K_h;
-            v += dt * 1.0 *
-                 lateral_div(
-                     g, tc.i, tc.j, k, [&](int64_t cc) { return E0[cc * g.cs]; },
-                     [&](int64_t, int64_t) { return Kh; }, [&](int64_t, int64_t) { return Kh; });
+            v += dt * 1.0 * lat[t];
             store = true;
         }
         e1[t] = v;
@@ -321,6 +394,27 @@ __global__ __launch_bounds__(256) void eke_kernel(EkeArgs a) {
     L.base = base;
     seg_solve_f64<S>(L, va, vb, vc, vd);
     if (!tc.valid) return;
+    // the lateral diffusion of the tau level (:168-200)
+    double lat[S];
+    {
+        const double* __restrict__ K = a.K_gm;
+        auto ce2 = [&](int64_t c0, int64_t c1) {
+            const double2 p0 = ld2(K + c0), p1 = ld2(K + c1);
+            return make_double2(0.5 * fmax(500.0, p0.x + p1.x), 0.5 * fmax(500.0, p0.y + p1.y));
+        };
+        lateral_levels<S, VEC>(
+            lo, [&](int k) { return tc.interior && k < nz && wet_at(tc.kb, k); },
+            [&](int k) {
+                return lateral_div(
+                    g, tc.i, tc.j, k, [&](int64_t cc) { return E0[cc * g.cs]; },
+                    [&](int64_t c0, int64_t c1) { return 0.5 * fmax(500.0, K[c0] + K[c1]); },
+                    [&](int64_t c0, int64_t c1) { return 0.5 * fmax(500.0, K[c0] + K[c1]); });
+            },
+            [&](int k) {
+                return lateral_div_pair(g, tc.i, tc.j, k, [&](int64_t cc) { return ld2(E0 + cc); }, ce2, ce2);
+            },
+            lat);
+    }
     double di[S], zero[S];
     bool st[S];
 #pragma unroll
@@ -331,12 +425,7 @@ __global__ __launch_bounds__(256) void eke_kernel(EkeArgs a) {
         zero[t] = 0.0;        // (:152)
         st[t] = false;
         if (tc.interior && k < nz && wet_at(tc.kb, k)) {
-            const double* __restrict__ K = a.K_gm;
-            v += dt * 1.0 *
-                 lateral_div(
-                     g, tc.i, tc.j, k, [&](int64_t cc) { return E0[cc * g.cs]; },
-                     [&](int64_t c0, int64_t c1) { return 0.5 * fmax(500.0, K[c0] + K[c1]); },
-                     [&](int64_t c0, int64_t c1) { return 0.5 * fmax(500.0, K[c0] + K[c1]); });
+            v += dt * 1.0 * lat[t];
             st[t] = true;
         }
         e1[t] = v;
@@ -465,6 +554,34 @@ __global__ __launch_bounds__(256) void idemix_kernel(IdemixArgs a) {
     L.base = base;
     seg_solve_f64<S>(L, va, vb, vc, vd);
     if (!tc.valid) return;
+    // the lateral diffusion of the tau level (:186-218)
+    double lat[S];
+    {
+        const double* __restrict__ V = a.v0;
+        const double th = a.tau_h;
+        auto ce2 = [&](int64_t c0_, int64_t c1) {
+            const double2 p1 = ld2(V + c1), p0 = ld2(V + c0_);
+            return make_double2(th * 0.5 * (p1.x + p0.x), th * 0.5 * (p1.y + p0.y));
+        };
+        lateral_levels<S, VEC>(
+            tc.lo, [&](int k) { return tc.interior && k < nz && a.hor && wet_at(tc.kb, k); },
+            [&](int k) {
+                return lateral_div(
+                    g, tc.i, tc.j, k, [&](int64_t cc) { return V[cc] * E0[cc * g.cs]; },
+                    [&](int64_t c0_, int64_t c1) { return th * 0.5 * (V[c1] + V[c0_]); },
+                    [&](int64_t c0_, int64_t c1) { return th * 0.5 * (V[c1] + V[c0_]); });
+            },
+            [&](int k) {
+                return lateral_div_pair(
+                    g, tc.i, tc.j, k,
+                    [&](int64_t cc) {
+                        const double2 v = ld2(V + cc), e = ld2(E0 + cc);
+                        return make_double2(v.x * e.x, v.y * e.y);
+                    },
+                    ce2, ce2);
+            },
+            lat);
+    }
     double di[S];
     bool st[S];
 #pragma unroll
@@ -474,13 +591,7 @@ __global__ __launch_bounds__(256) void idemix_kernel(IdemixArgs a) {
         di[t] = al[t] * mx[t] * v;  // (:174)
         st[t] = tc.interior && k >= tc.k0;
         if (tc.interior && k < nz && a.hor && wet_at(tc.kb, k)) {
-            const double* __restrict__ V = a.v0;
-            const double th = a.tau_h;
-            v += dt * 1.0 *
-                 lateral_div(
-                     g, tc.i, tc.j, k, [&](int64_t cc) { return V[cc] * E0[cc * g.cs]; },
-                     [&](int64_t c0_, int64_t c1) { return th * 0.5 * (V[c1] + V[c0_]); },
-                     [&](int64_t c0_, int64_t c1) { return th * 0.5 * (V[c1] + V[c0_]); });
+            v += dt * 1.0 * lat[t];
             st[t] = true;
         }
         e1[t] = v;
@@ -632,7 +743,7 @@ extern "C" int vx_integrate_eke_f64(void* stream, const vx_grid* grid, const vx_
     a.diss_iw = eke_diss_iw, a.diss_tke = eke_diss_tke;
     a.dt = prm->dt_tracer, a.alpha = prm->alpha_eke, a.c_eps = prm->eke_c_eps;
     const bool vec = turb_vec(a.g, {eke, eke + tau * a.g.ls, eke + taup1 * a.g.ls, kappaM, sqrteke, eke_len,
-                                    eke_diss_iw, eke_diss_tke}, a.forc, nullptr);
+                                    K_gm, eke_diss_iw, eke_diss_tke}, a.forc, nullptr);
     return launch_turb<EkeArgs, EkeK>(a, a.g, vec, (hipStream_t)stream, who);
 }
 
@@ -661,7 +772,7 @@ extern "C" int vx_integrate_idemix_f64(void* stream, const vx_grid* grid, const 
     a.dt = prm->dt_tracer, a.tau_v = prm->tau_v, a.tau_h = prm->tau_h, a.frac = prm->eke_diss_surfbot_frac;
     a.hor = prm->hor_diffusion != 0;
     a.diss_mode = prm->eke_diss_mode;
-    const bool vec = turb_vec(a.g, {E_iw, E_iw + tau * a.g.ls, E_iw + taup1 * a.g.ls, c0, alpha_c, iw_diss},
+    const bool vec = turb_vec(a.g, {E_iw, E_iw + tau * a.g.ls, E_iw + taup1 * a.g.ls, c0, v0, alpha_c, iw_diss},
                               a.forc, &a.forc_post);
     return launch_turb<IdemixArgs, IdemixK>(a, a.g, vec, (hipStream_t)stream, who);
 }
