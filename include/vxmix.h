@@ -397,16 +397,13 @@ int vx_vertmix_tempsalt_f64(void* stream, const vx_grid* grid, double dt_tracer,
  * interpolated onto the T grid with dxu / dxt (u) and area_v / area_t (v)
  * (veros/core/numerics.py:313-336), is added to K_diss_v. grid->kbot, dzt,
  * dzw, dxt and dxu are read; the other grid arrays may be NULL. tau and taup1
- * must differ. `workspace` (device memory of
- * vx_implicit_vert_friction_workspace_bytes(grid) bytes, 16-byte aligned for
- * the fastest path) holds the faces' dissipation between the solve and the
- * interpolation; calls sharing one workspace must be ordered on one stream.
+ * must differ. The fastest path wants the levels outermost, nz even and
+ * every array 16-byte aligned.
  */
-int64_t vx_implicit_vert_friction_workspace_bytes(const vx_grid* grid);
 int vx_implicit_vert_friction_f64(void* stream, const vx_grid* grid, double dt_mom,
                                   double* u, double* v, int32_t tau, int32_t taup1,
                                   const double* kappaM, const double* area_t, const double* area_v,
-                                  double* du_mix, double* dv_mix, double* K_diss_v, void* workspace);
+                                  double* du_mix, double* dv_mix, double* K_diss_v);
 
 /*
  * isoneutral_skew_diffusion for one tracer (veros/core/isoneutral/diffusion.py:172-207,

@@ -139,8 +139,8 @@ def test_friction_argument_errors_without_gpu():
     g = _native.VxGrid()
     g.nx, g.ny, g.nz = 4, 4, 8
 
-    def call(grid=g, dt=3600.0, u=1, v=2, tau=0, taup1=1, kap=3, du=4, dv=5, kd=6, ws=9):
-        return L.vx_implicit_vert_friction_f64(None, grid, dt, u, v, tau, taup1, kap, 7, 8, du, dv, kd, ws)
+    def call(grid=g, dt=3600.0, u=1, v=2, tau=0, taup1=1, kap=3, du=4, dv=5, kd=6):
+        return L.vx_implicit_vert_friction_f64(None, grid, dt, u, v, tau, taup1, kap, 7, 8, du, dv, kd)
 
     assert call() == _native.VX_ERR_INVALID_ARGUMENT                # null kbot/dzt/dzw/dxt/dxu
     g.kbot, g.dzt, g.dzw, g.dxt, g.dxu = 16, 16, 16, 16, 16
@@ -150,9 +150,7 @@ def test_friction_argument_errors_without_gpu():
     assert call(v=1) == _native.VX_ERR_INVALID_ARGUMENT              # u and v alias
     assert call(kd=4) == _native.VX_ERR_INVALID_ARGUMENT             # K_diss_v aliases du_mix
     assert call(kap=None) == _native.VX_ERR_INVALID_ARGUMENT
-    assert call(ws=None) == _native.VX_ERR_INVALID_ARGUMENT          # no workspace
     assert call(dt=-1.0) == _native.VX_ERR_INVALID_ARGUMENT
-    assert L.vx_implicit_vert_friction_workspace_bytes(g) == 2 * ((8 * 8 * 8 * 8 + 255) // 256 * 256)
     g.nz = _native.max_nz() + 1
     assert call() == _native.VX_ERR_UNSUPPORTED
     assert "vx_implicit_vert_friction_f64" in L.vx_last_error().decode()

@@ -54,10 +54,17 @@ def _dev(st, levels="inner"):
                                                       ((12, 9, 2), True, 0.0, (2, 0)), ((8, 10, 80), True, 0.35, (0, 1)),
                                                       ((6, 5, 128), False, 0.2, (1, 0)), ((7, 6, 17), True, 1.0, (0, 2)),
                                                       ((10, 12, 100), True, 0.3, (2, 1)), ((5, 4, 3), False, 0.2, (0, 1)),
-                                                      ((11, 9, 40), False, 0.25, (1, 2))])
+                                                      ((11, 9, 40), False, 0.25, (1, 2)),
+                                                      ((70, 66, 20), True, 0.3, (0, 1)),
+                                                      ((24, 31, 14), False, 0.3, (2, 0)),
+                                                      ((47, 30, 9), True, 0.2, (1, 2))])
 @pytest.mark.parametrize("layout", ["inner", "outer"])
 def test_friction_random_against_oracle(shape, cyclic, land, levels, layout):
-    """Levels outermost with nz even runs the 16-byte load/store variant."""
+    """Levels outermost with nz even runs the 16-byte load/store variant. The
+    kernel's blocks own strips of 31 T columns and chunks of 24 rows (nx + 1
+    rows, ny + 1 columns): 70 x 66 spans three of each, ragged; 24 x 31 leaves
+    one row and one column to a last chunk and strip; 47 x 30 fills two chunks
+    and one strip exactly."""
     st = _state(sum(shape), *shape, cyclic=cyclic, land=land, tau=levels[0], taup1=levels[1])
     ref = F.implicit_vert_friction(st)
     ds = _dev(st, layout)

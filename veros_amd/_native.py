@@ -106,9 +106,8 @@ _SIGNATURES = {
                                   + [c_void_p, c_void_p, c_void_p, c_void_p]),
     "vx_vertmix_tempsalt_f64": (ctypes.c_int, [c_void_p, ctypes.POINTER(VxGrid), c_f64, c_void_p, c_void_p, c_i32,
                                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_i32]),
-    "vx_implicit_vert_friction_workspace_bytes": (c_i64, [ctypes.POINTER(VxGrid)]),
     "vx_implicit_vert_friction_f64": (ctypes.c_int, [c_void_p, ctypes.POINTER(VxGrid), c_f64, c_void_p, c_void_p,
-                                                     c_i32, c_i32] + [c_void_p] * 7),
+                                                     c_i32, c_i32] + [c_void_p] * 6),
     "vx_integrate_tke_f64": (ctypes.c_int, [c_void_p, ctypes.POINTER(VxGrid), ctypes.POINTER(VxTkeParams), c_void_p,
                                             c_i32, c_i32] + [c_void_p] * 4 + [ctypes.POINTER(VxTerms), c_void_p,
                                                                               c_void_p]),

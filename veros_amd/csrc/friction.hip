@@ -11,15 +11,15 @@
 // interpolated onto the T grid, to K_diss_v.
 //
 // Here a group of kSegP lanes owns a face column (the segmented solve of
-// tdma_seg.hpp, lane s holding levels [s*S, s*S + S)) and treats its u face,
-// then its v face: each lane forms its rows in registers from the face
-// viscosity, the group solves, and each lane writes the new velocity on the
-// wet levels, the tendency on every level, and the face's dissipation (from
-// the velocities still in registers; the level above comes from the next
-// lane) into a workspace. A second kernel adds the dissipation of the four
-// faces around each T cell to K_diss_v in the reference's order, its blocks
-// remapped so that each XCD walks a contiguous range of rows and the faces of
-// row i-1 are still in its L2.
+// tdma_seg.hpp, lane s holding levels [s*S, s*S + S)): each lane forms its
+// rows in registers from the face viscosity, the group solves, and each lane
+// writes the new velocity on the wet levels and the tendency on every level,
+// and forms the face's dissipation from the velocities still in registers
+// (the level above from the next lane). A T cell's dissipation needs the u
+// faces i-1 and i and the v faces j-1 and j, so a block owns a strip of T
+// columns and walks a chunk of rows in order: the u faces of the previous row
+// are kept in LDS, and the v faces of the row pass through LDS to the group
+// north of them. No face dissipation goes through memory.
 #include "tdma_seg.hpp"
 
 namespace vx {
@@ -36,7 +36,6 @@ struct FricArgs {
     const double* kappaM;     // (X, Y, nz)
     double* vel[2];           // u, v (X, Y, nz, 3)
     double* dmix[2];          // du_mix, dv_mix (X, Y, nz)
-    double* diss[2];          // workspace: dissipation of the u and v faces (X, Y, nz)
     double* K_diss_v;         // (X, Y, nz)
     int64_t X, Y;
     int32_t nz, tau, taup1;
@@ -68,14 +67,15 @@ __device__ inline Face face_of(int kb0, int kb1, int nz) {
 
 // One face column of component t (face range only when `live`): rows from
 // the face viscosity (veros/core/friction.py:111-126, 161-176), the solve,
-// the velocity at taup1 on the wet levels and the tendency on every level
-// (:127-130, 178-181), and the face's dissipation on every level (:137-153,
-// 189-201): fxa * (x1[k+1] - x1[k]) / dzw[k] * mask[k+1] * mask[k] times
-// (x0[k+1] - x0[k]) / dzw[k], x0 at tau and x1 at taup1, 0 on the top level.
-// The masks are 0 or 1, so the products are taken as selects.
+// with `store` the velocity at taup1 on the wet levels and the tendency on
+// every level (:127-130, 178-181), and in ds the face's dissipation on every
+// level (:137-153, 189-201): fxa * (x1[k+1] - x1[k]) / dzw[k] * mask[k+1] *
+// mask[k] times (x0[k+1] - x0[k]) / dzw[k], x0 at tau and x1 at taup1, 0 on
+// the top level. The masks are 0 or 1, so the products are taken as selects.
+// Every lane of the wave calls it (the solve moves data across lanes).
 template <int S, bool VEC>
 __device__ __forceinline__ void fric_face(const FricArgs& p, SegLane L, int t, int64_t col, int64_t nb, bool live,
-                                          bool both) {
+                                          bool both, bool store, double (&ds)[S]) {
     const int nz = p.nz;
     const int lo = L.lo;
     const int k0 = L.k0;
@@ -123,13 +123,6 @@ __device__ __forceinline__ void fric_face(const FricArgs& p, SegLane L, int t, i
     // level lo+S from the lane above (every lane takes part in the moves)
     const double x1n = grp_next(L.s, x1[0]);
     const double x0n = grp_next(L.s, x0[0]);
-    if (!live) return;
-#pragma unroll
-    for (int q = 0; q < S; ++q) {
-        const int k = lo + q;
-        if (k >= k0 && k < nz) u1p[p.cs * k] = x1[q];
-    }
-    double dm[S], ds[S];
 #pragma unroll
     for (int q = 0; q < S; ++q) {
         const int k = lo + q;
@@ -138,87 +131,140 @@ __device__ __forceinline__ void fric_face(const FricArgs& p, SegLane L, int t, i
         const int kc = k < nz - 1 ? k : 0;
         const double dzw = p.dzw[kc];
         ds[q] = (both && k >= k0 && k < nz - 1) ? ((a0 - x0[q]) * ((fxa[q] * (a1 - x1[q])) / dzw)) / dzw : 0.0;
+    }
+    if (!(live && store)) return;
+    bool st[S];
+    double dm[S];
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+        const int k = lo + q;
+        st[q] = k >= k0;
         dm[q] = (x1[q] - x0[q]) / dt;
     }
+    // the levels below the solve hold taup1's own values, so a pair store
+    // that covers one of them writes back what is there
+    col_store_if<S, VEC>(u1p, p.cs, lo, nz, x1, st);
     col_store<S, VEC>(p.dmix[t] + base, 1, lo, nz, dm);
-    col_store<S, VEC>(p.diss[t] + base, 1, lo, nz, ds);
 }
 
+// A block of 32 face-column groups owns a strip of 31 T columns j (group g at
+// column jb - 1 + g; group 0 solves the v face south of the strip for group
+// 1) and walks the rows of a chunk [ib, ie) in order, starting one row early
+// for the u faces west of the chunk. Per row: the v face, then the u face of
+// each column. The u-face dissipation of the previous row stays in LDS, each
+// lane reading back only what it wrote; the v-face dissipation of the row
+// goes through LDS to the group north of it (two buffers, one barrier per
+// row). Each face is stored by exactly one block. K_diss_v +=
+// ugrid_to_tgrid(diss_u), then += vgrid_to_tgrid(diss_v)
+// (veros/core/friction.py:154-155, 203-204; veros/core/numerics.py:313-336):
+// at T cell (i, j) the u faces i and i-1 weighted dxu / (2 dxt) for 2 <= i <
+// X-2 (face i alone for i = 1), the v faces j and j-1 weighted area_v / (2
+// area_t) for 2 <= j < Y-2 (face j alone for j = 1), formed as (K + u part)
+// + v part on the levels
+// below the top one (which dissipates nothing). Blocks are XCD-contiguous,
+// strips fastest. Two waves per SIMD: 256 VGPRs at nz = 80.
 // VEC: levels outermost, nz even and every array 16-byte aligned.
 template <int S, bool VEC>
-__global__ __launch_bounds__(kSegBlock) void friction_seg_kernel(FricArgs p) {
-    const int64_t gt = xcd_contiguous_block() * blockDim.x + threadIdx.x;
-    int64_t col = gt / kSegP;
+__global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(2))) void friction_walk_kernel(FricArgs p, int rows, int64_t nstrips,
+                                                                  int64_t nblocks) {
+    __shared__ double dsu[S * kSegBlock];
+    __shared__ double dsv[2][S * kSegBlock];
+    const int64_t b = xcd_contiguous_block();
+    if (b >= nblocks) return;
+    const int tid = threadIdx.x;
+    const int g = tid / kSegP;
+    const int64_t strip = b % nstrips, chunk = b / nstrips;
+    const int64_t j = 31 * strip + g;  // the strip's T columns are [31 strip + 1, 31 strip + 32)
+    const int64_t ib = 1 + (int64_t)rows * chunk;
+    const int64_t ie = ib + rows < p.X - 2 ? ib + rows : p.X - 2;
+    const bool jlive = j >= 1 && j <= p.Y - 3;
+    const bool jown = g >= 1 && jlive;
+    const int nz = p.nz;
     SegLane L;
-    L.s = (int)(gt % kSegP);
+    L.s = tid % kSegP;
     L.lo = L.s * S;
-    L.valid = col < p.X * p.Y;
-    if (!L.valid) col = 0;
-    const int64_t i = col / p.Y, j = col - (col / p.Y) * p.Y;
-    const bool live = L.valid && i >= 1 && i <= p.X - 3 && j >= 1 && j <= p.Y - 3;
-    const int kb = live ? p.kbot[col] : 0;
-    L.base = col * p.nz;
+    L.valid = true;
 #pragma unroll 1
-    for (int t = 0; t < 2; ++t) {
-        const int64_t nb = live ? (t == 0 ? col + p.Y : col + 1) : col;  // the T column across the face
-        const Face f = face_of(kb, live ? p.kbot[nb] : 0, p.nz);
-        L.k0 = f.k0;
-        fric_face<S, VEC>(p, L, t, col, nb, live, f.both);
+    for (int64_t i = ib >= 2 ? ib - 1 : ib; i < ie; ++i) {
+        // the grid spacings are loaded again in each face (kept across the
+        // row loop they would double the registers of the solve)
+        asm volatile("" ::: "memory");
+        const bool pre = i < ib;
+        const int64_t col = jlive ? i * p.Y + j : 0;
+        L.base = col * nz;
+        const int kb = jlive ? p.kbot[col] : 0;
+        const int buf = (int)(i & 1);
+        if (!pre) {
+            const int64_t nb = jlive ? col + 1 : col;
+            const Face f = face_of(kb, jlive ? p.kbot[nb] : 0, nz);
+            L.k0 = f.k0;
+            double ds[S];
+            fric_face<S, VEC>(p, L, 1, col, nb, jlive, f.both, jown, ds);
+#pragma unroll
+            for (int q = 0; q < S; ++q) dsv[buf][q * kSegBlock + tid] = ds[q];
+        }
+        double bu[S];
+        asm volatile("" ::: "memory");
+        {
+            const bool ulive = jown;
+            const int64_t nb = ulive ? col + p.Y : col;
+            const Face f = face_of(ulive ? kb : 0, ulive ? p.kbot[nb] : 0, nz);
+            L.k0 = f.k0;
+            double ds[S];
+            fric_face<S, VEC>(p, L, 0, col, nb, ulive, f.both, !pre, ds);
+            const double wi = p.dxu[i], wm = p.dxu[i - 1], den = 2 * p.dxt[i];
+#pragma unroll
+            for (int q = 0; q < S; ++q) {
+                bu[q] = i >= 2 ? (wi * ds[q] + wm * dsu[q * kSegBlock + tid]) / den : ds[q];
+                dsu[q * kSegBlock + tid] = ds[q];
+            }
+        }
+        if (pre) continue;
+        __syncthreads();
+        if (jown) {
+            double K[S];
+#pragma unroll
+            for (int q = 0; q < S; ++q) K[q] = 0.0;
+            double* Kp = p.K_diss_v + L.base;
+            col_load<S, VEC>(Kp, 1, L.lo, nz, [&](int k) { return k < nz - 1; }, K);
+            const int64_t ij = col;
+            const double wj = p.area_v[ij], wjm = p.area_v[ij - 1], dnj = 2 * p.area_t[ij];
+            bool st[S];
+#pragma unroll
+            for (int q = 0; q < S; ++q) {
+                const double dv = dsv[buf][q * kSegBlock + tid];
+                const double bv = j >= 2 ? (wj * dv + wjm * dsv[buf][q * kSegBlock + tid - kSegP]) / dnj : dv;
+                st[q] = L.lo + q < nz - 1;
+                K[q] = st[q] ? (K[q] + bu[q]) + bv : K[q];
+            }
+            col_store_if<S, VEC>(Kp, 1, L.lo, nz, K, st);
+        }
     }
 }
 
-// K_diss_v += ugrid_to_tgrid(diss_u), then += vgrid_to_tgrid(diss_v)
-// (veros/core/friction.py:154-155, 203-204; veros/core/numerics.py:313-336):
-// at T cell (i, j) the u faces i and i-1 weighted dxu / (2 dxt) for 2 <= i <
-// X-2 (face i alone elsewhere), the v faces j and j-1 weighted area_v / (2
-// area_t) for 2 <= j < Y-2. Thread per (T cell column, level) of [1, X-2) x
-// [1, Y-2); the top level dissipates nothing. The grid is a multiple of 8
-// blocks, remapped so that XCD x takes the x-th eighth of the cells.
-__global__ __launch_bounds__(256) void friction_gather_kernel(FricArgs p) {
-    const int nz1 = p.nz - 1;
-    const int64_t e = xcd_contiguous_block() * blockDim.x + threadIdx.x;
-    const int64_t ncols = (p.X - 3) * (p.Y - 3);  // T columns [1, X-2) x [1, Y-2)
-    if (e >= ncols * nz1) return;
-    const int64_t c = e / nz1;
-    const int k = (int)(e - c * nz1);
-    const int64_t i = 1 + c / (p.Y - 3), j = 1 + c % (p.Y - 3);
-    const int64_t ij = i * p.Y + j;
-    const int64_t cell = ij * p.nz + k;
-    const double du_i = p.diss[0][cell];
-    double bu = du_i;
-    if (i >= 2) bu = (p.dxu[i] * du_i + p.dxu[i - 1] * p.diss[0][cell - p.Y * p.nz]) / (2 * p.dxt[i]);
-    const double dv_j = p.diss[1][cell];
-    double bv = dv_j;
-    if (j >= 2) bv = (p.area_v[ij] * dv_j + p.area_v[ij - 1] * p.diss[1][cell - p.nz]) / (2 * p.area_t[ij]);
-    double* K = p.K_diss_v + cell;
-    *K = (*K + bu) + bv;
-}
+#ifndef VX_FRIC_ROWS
+#define VX_FRIC_ROWS 24  // rows per block of the walk kernel
+#endif
 
 template <int S>
-void launch_fric(const FricArgs& p, bool vec, hipStream_t s) {
-    const int64_t threads = p.X * p.Y * kSegP;
-    const dim3 grid(blocks_mult8(threads, kSegBlock));
+void launch_fric_walk(const FricArgs& p, bool vec, hipStream_t s) {
+    const int rows = VX_FRIC_ROWS;
+    const int64_t nstrips = (p.Y - 3 + 30) / 31, nchunks = (p.X - 3 + rows - 1) / rows;
+    const int64_t nblocks = nstrips * nchunks;
+    const dim3 grid((unsigned)((nblocks + 7) / 8 * 8));
     if (vec)
-        hipLaunchKernelGGL((friction_seg_kernel<S, true>), grid, dim3(kSegBlock), 0, s, p);
+        hipLaunchKernelGGL((friction_walk_kernel<S, true>), grid, dim3(kSegBlock), 0, s, p, rows, nstrips, nblocks);
     else
-        hipLaunchKernelGGL((friction_seg_kernel<S, false>), grid, dim3(kSegBlock), 0, s, p);
+        hipLaunchKernelGGL((friction_walk_kernel<S, false>), grid, dim3(kSegBlock), 0, s, p, rows, nstrips, nblocks);
 }
-
-int64_t fric_ws_elems(int64_t X, int64_t Y, int nz) { return X * Y * (int64_t)nz; }
 
 }  // namespace
 }  // namespace vx
 
-extern "C" int64_t vx_implicit_vert_friction_workspace_bytes(const vx_grid* grid) {
-    if (!grid || grid->nx < 1 || grid->ny < 1 || grid->nz < 1) return 0;
-    const int64_t n = vx::fric_ws_elems(grid->nx + 4, grid->ny + 4, grid->nz);
-    return 2 * ((n * 8 + 255) / 256 * 256);
-}
-
 extern "C" int vx_implicit_vert_friction_f64(void* stream, const vx_grid* grid, double dt_mom, double* u, double* v,
                                              int32_t tau, int32_t taup1, const double* kappaM, const double* area_t,
                                              const double* area_v, double* du_mix, double* dv_mix,
-                                             double* K_diss_v, void* workspace) {
+                                             double* K_diss_v) {
     using namespace vx;
     const char* who = "vx_implicit_vert_friction_f64";
     if (!grid) return fail(VX_ERR_INVALID_ARGUMENT, "%s: null grid", who);
@@ -230,7 +276,7 @@ extern "C" int vx_implicit_vert_friction_f64(void* stream, const vx_grid* grid, 
         return fail(VX_ERR_INVALID_ARGUMENT, "%s: null grid array", who);
     if (tau < 0 || tau > 2 || taup1 < 0 || taup1 > 2 || tau == taup1)
         return fail(VX_ERR_INVALID_ARGUMENT, "%s: tau and taup1 must be distinct levels in 0..2", who);
-    if (!u || !v || !kappaM || !area_t || !area_v || !du_mix || !dv_mix || !K_diss_v || !workspace)
+    if (!u || !v || !kappaM || !area_t || !area_v || !du_mix || !dv_mix || !K_diss_v)
         return fail(VX_ERR_INVALID_ARGUMENT, "%s: null array pointer", who);
     if (u == v || du_mix == dv_mix || (const double*)K_diss_v == kappaM || K_diss_v == du_mix || K_diss_v == dv_mix)
         return fail(VX_ERR_INVALID_ARGUMENT, "%s: u/v, du_mix/dv_mix and K_diss_v must be distinct arrays", who);
@@ -260,28 +306,18 @@ extern "C" int vx_implicit_vert_friction_f64(void* stream, const vx_grid* grid, 
     const bool outer = grid->time_layout == VX_LEVELS_OUTER;
     p.cs = outer ? 1 : 3;
     p.ls = outer ? p.X * p.Y * p.nz : 1;
-    const int64_t plane = (vx::fric_ws_elems(p.X, p.Y, p.nz) * 8 + 255) / 256 * 256;
-    p.diss[0] = (double*)workspace;
-    p.diss[1] = (double*)((char*)workspace + plane);
     const bool vec = outer && !(p.nz & 1) && aligned16(u) && aligned16(v) && aligned16(kappaM) &&
-                     aligned16(du_mix) && aligned16(dv_mix) && aligned16(workspace);
+                     aligned16(du_mix) && aligned16(dv_mix) && aligned16(K_diss_v);
     hipStream_t s = (hipStream_t)stream;
     switch (seg_levels(p.nz)) {
-        case 2: launch_fric<2>(p, vec, s); break;
-        case 4: launch_fric<4>(p, vec, s); break;
-        case 6: launch_fric<6>(p, vec, s); break;
-        case 8: launch_fric<8>(p, vec, s); break;
-        case 10: launch_fric<10>(p, vec, s); break;
-        case 12: launch_fric<12>(p, vec, s); break;
-        case 14: launch_fric<14>(p, vec, s); break;
-        default: launch_fric<16>(p, vec, s); break;
+        case 2: launch_fric_walk<2>(p, vec, s); break;
+        case 4: launch_fric_walk<4>(p, vec, s); break;
+        case 6: launch_fric_walk<6>(p, vec, s); break;
+        case 8: launch_fric_walk<8>(p, vec, s); break;
+        case 10: launch_fric_walk<10>(p, vec, s); break;
+        case 12: launch_fric_walk<12>(p, vec, s); break;
+        case 14: launch_fric_walk<14>(p, vec, s); break;
+        default: launch_fric_walk<16>(p, vec, s); break;
     }
-    int rc = check_launch(who);
-    if (rc) return rc;
-    if (p.nz > 1) {
-        const int64_t n = (p.X - 3) * (p.Y - 3) * (int64_t)(p.nz - 1);
-        hipLaunchKernelGGL(friction_gather_kernel, dim3(blocks_mult8(n, 256)), dim3(256), 0, s, p);
-        rc = check_launch(who);
-    }
-    return rc;
+    return check_launch(who);
 }

@@ -18,7 +18,7 @@ import ctypes
 import torch
 
 from . import _native
-from ._arrays import ptr, stream_handle, workspace
+from ._arrays import ptr, stream_handle
 from .isoneutral import KernelOutput, _check_array, _check_levels
 from .layout import common_layout
 
@@ -44,9 +44,8 @@ def implicit_vert_friction(state):
     g.nx, g.ny, g.nz = X - 4, Y - 4, nz
     g.time_layout = common_layout(u=u, v=v)
     L = _native.lib()
-    ws = workspace(L.vx_implicit_vert_friction_workspace_bytes(ctypes.byref(g)), u.device)
     rc = L.vx_implicit_vert_friction_f64(
         stream_handle(), ctypes.byref(g), float(st.dt_mom), ptr(u), ptr(v), int(vs.tau), int(vs.taup1), ptr(K),
-        ptr(at), ptr(av), ptr(du), ptr(dv), ptr(kd), ptr(ws))
+        ptr(at), ptr(av), ptr(du), ptr(dv), ptr(kd))
     _native.check(rc, "implicit_vert_friction")
     return KernelOutput(u=u, v=v, du_mix=du, dv_mix=dv, K_diss_v=kd)
