@@ -182,21 +182,44 @@ __device__ __forceinline__ void load_lv(const double* __restrict__ p, int64_t ba
     col_load<S, VEC>(p + base, 1, lo, nz, [](int) { return true; }, v);
 }
 
+#ifndef VX_TERMS_GROUP
+#define VX_TERMS_GROUP 3
+#endif
+// f (levels lo .. lo+S-1) plus the sum of the forcing terms where want(k),
+// summed left to right as sum_terms; with `first` the sum replaces f. The
+// terms are loaded VX_TERMS_GROUP at a time, all loads of a group before its
+// first use, so a group costs one memory round trip instead of one per term.
+template <int S, bool VEC, class W>
+__device__ __forceinline__ void add_terms(const Terms& T, int64_t base, int lo, int nz, W want, bool first,
+                                          double (&f)[S]) {
+    constexpr int G = VX_TERMS_GROUP;
+    for (int q0 = 0; q0 < T.n; q0 += G) {
+        double v[G][S];
+#pragma unroll
+        for (int r = 0; r < G; ++r) {
+#pragma unroll
+            for (int t = 0; t < S; ++t) v[r][t] = 0.0;
+            if (q0 + r < T.n) col_load<S, VEC>(T.p[q0 + r] + base, 1, lo, nz, want, v[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < G; ++r) {
+            if (q0 + r < T.n) {
+                const double sg = T.sgn[q0 + r];
+                const bool set = first && q0 + r == 0;
+#pragma unroll
+                for (int t = 0; t < S; ++t) f[t] = set ? sg * v[r][t] : f[t] + sg * v[r][t];
+            }
+        }
+    }
+}
+
 // sum of the forcing terms at levels lo .. lo+S-1 where want(k), summed
 // left to right as sum_terms (0 elsewhere)
 template <int S, bool VEC, class W>
 __device__ __forceinline__ void load_terms(const Terms& T, int64_t base, int lo, int nz, W want, double (&f)[S]) {
 #pragma unroll
     for (int t = 0; t < S; ++t) f[t] = 0.0;
-    for (int q = 0; q < T.n; ++q) {
-        double v[S];
-#pragma unroll
-        for (int t = 0; t < S; ++t) v[t] = 0.0;
-        col_load<S, VEC>(T.p[q] + base, 1, lo, nz, want, v);
-        const double sg = T.sgn[q];
-#pragma unroll
-        for (int t = 0; t < S; ++t) f[t] = q == 0 ? sg * v[t] : f[t] + sg * v[t];
-    }
+    add_terms<S, VEC>(T, base, lo, nz, want, true, f);
 }
 
 // ---------------------------------------------------------------------------
@@ -503,15 +526,7 @@ __global__ __launch_bounds__(256) void idemix_kernel(IdemixArgs a) {
         // columns are never solved
     }
     // terms added afterwards, on the levels whose rows are solved
-    for (int q = 0; q < a.forc_post.n; ++q) {
-        double v[S];
-#pragma unroll
-        for (int t = 0; t < S; ++t) v[t] = 0.0;
-        col_load<S, VEC>(a.forc_post.p[q] + base, 1, tc.lo, nz, [&](int k) { return k >= tc.k0; }, v);
-        const double sg = a.forc_post.sgn[q];
-#pragma unroll
-        for (int t = 0; t < S; ++t) fo[t] = fo[t] + sg * v[t];
-    }
+    add_terms<S, VEC>(a.forc_post, base, tc.lo, nz, [&](int k) { return k >= tc.k0; }, false, fo);
     double va[S], vb[S], vc[S], vd[S];
     auto delta = [&](int t) {
         const int k = tc.lo - 1 + t;
