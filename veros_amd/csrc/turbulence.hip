@@ -20,6 +20,9 @@
 namespace vx {
 
 constexpr int kMaxTerms = 12;
+// threads per block of the three kernels; a constant, since reading the
+// block size at run time costs a memory round trip at the start of every wave
+constexpr int kTurbBlock = 256;
 
 struct TurbGrid {
     const int32_t* kbot;
@@ -55,7 +58,7 @@ struct TurbCol {
 
 __device__ __forceinline__ TurbCol turb_col(const TurbGrid& g, int S) {
     TurbCol t;
-    const int64_t gt = xcd_contiguous_block() * blockDim.x + threadIdx.x;
+    const int64_t gt = xcd_contiguous_block() * kTurbBlock + threadIdx.x;
     const int64_t col = gt / kSegP;
     t.s = (int)(gt % kSegP);
     t.lo = t.s * S;
@@ -150,19 +153,32 @@ __device__ __forceinline__ void lateral_levels(int lo, NEED need, L1 one, L2 pai
     }
 }
 
+// dzt (s[0, nz)) and dzw (s[kMaxNz, kMaxNz + nz)) of the grid in LDS; every
+// thread of the block calls it. The row loops read the spacings of their
+// levels inside per-lane branches, where a global load is waited for at once:
+// from LDS that wait is short.
+__device__ __forceinline__ void stage_dz(const TurbGrid& g, double* s) {
+    for (int k = threadIdx.x; k < g.nz; k += kTurbBlock) {
+        s[k] = g.dzt[k];
+        s[kMaxNz + k] = g.dzw[k];
+    }
+    __syncthreads();
+}
+
 // levels lo-1 .. lo+S of a (X, Y, nz) array at column `base` (0 outside
-// [0, nz)); with VEC the lane loads its own levels in pairs and takes lo-1
-// and lo+S from the lanes below and above (every lane must call it).
+// [0, nz)), in two steps so that other loads can be issued between them:
+// load_halo_issue loads the lane's levels (VEC: its own, in pairs; otherwise
+// all S+2), and load_halo_finish takes lo-1 and lo+S from the lanes below
+// and above (VEC; every lane must call it).
 template <int S, bool VEC>
-__device__ __forceinline__ void load_halo(const double* __restrict__ p, int64_t base, int s, int lo, int nz,
-                                          double (&v)[S + 2]) {
+__device__ __forceinline__ void load_halo_issue(const double* __restrict__ p, int64_t base, int lo, int nz,
+                                                double (&v)[S + 2]) {
     if constexpr (VEC) {
         double w[S];
 #pragma unroll
         for (int t = 0; t < S; ++t) w[t] = 0.0;
         col_load<S, true>(p + base, 1, lo, nz, [](int) { return true; }, w);
-        v[0] = grp_prev(s, w[S - 1]);
-        v[S + 1] = grp_next(s, w[0]);
+        v[0] = v[S + 1] = 0.0;
 #pragma unroll
         for (int t = 0; t < S; ++t) v[t + 1] = w[t];
     } else {
@@ -171,6 +187,13 @@ __device__ __forceinline__ void load_halo(const double* __restrict__ p, int64_t 
             const int k = lo - 1 + t;
             v[t] = (k >= 0 && k < nz) ? p[base + k] : 0.0;
         }
+    }
+}
+template <int S, bool VEC>
+__device__ __forceinline__ void load_halo_finish(int s, double (&v)[S + 2]) {
+    if constexpr (VEC) {
+        v[0] = grp_prev(s, v[S]);
+        v[S + 1] = grp_next(s, v[1]);
     }
 }
 
@@ -239,13 +262,17 @@ struct TkeArgs {
 // VEC (all three kernels): levels outermost, nz even and every array
 // 16-byte aligned; the lanes then move their levels in 16-byte pairs.
 template <int S, bool VEC>
-__global__ __launch_bounds__(256) void tke_kernel(TkeArgs a) {
+__global__ __launch_bounds__(kTurbBlock) void tke_kernel(TkeArgs a) {
     const TurbGrid& g = a.g;
     const TurbCol tc = turb_col(g, S);
+    __shared__ double sdz[2 * kMaxNz];
+    stage_dz(g, sdz);
+    const double* __restrict__ zt = sdz;           // dzt
+    const double* __restrict__ zw = sdz + kMaxNz;  // dzw
     const int nz = g.nz, lo = tc.lo, k0 = tc.k0;
     const int64_t base = tc.col * nz;
     double km[S + 2];
-    load_halo<S, VEC>(a.kappaM, base, tc.s, lo, nz, km);
+    load_halo_issue<S, VEC>(a.kappaM, base, lo, nz, km);
     double sq[S], ml[S];
     load_lv<S, VEC>(a.sqrttke, base, lo, nz, sq);
     load_lv<S, VEC>(a.mxl, base, lo, nz, ml);
@@ -258,11 +285,13 @@ __global__ __launch_bounds__(256) void tke_kernel(TkeArgs a) {
     col_load<S, VEC>(E0 + base * g.cs, g.cs, lo, nz, [&](int k) { return k >= k0; }, e0);
     col_load<S, VEC>(E1 + base * g.cs, g.cs, lo, nz, [&](int k) { return tc.valid && k < k0; }, e1);
     load_terms<S, VEC>(a.forc, base, lo, nz, [&](int k) { return k >= k0; }, fo);
+    const double fsurf = a.forc_surf[tc.col];  // loaded with the rest, used by the top row
     const double dt = a.dt;
+    load_halo_finish<S, VEC>(tc.s, km);  // after the other loads are issued
     double va[S], vb[S], vc[S], vd[S];
     auto delta = [&](int t) {  // delta at level lo - 1 + t
         const int k = lo - 1 + t;
-        return (k >= 0 && k < nz - 1) ? dt / g.dzt[k + 1] * a.alpha * 0.5 * (km[t] + km[t + 1]) : 0.0;
+        return (k >= 0 && k < nz - 1) ? dt / zt[k + 1] * a.alpha * 0.5 * (km[t] + km[t + 1]) : 0.0;
     };
 #pragma unroll
     for (int t = 0; t < S; ++t) {
@@ -271,7 +300,7 @@ __global__ __launch_bounds__(256) void tke_kernel(TkeArgs a) {
         double A = 0.0, B = 1.0, C = 0.0, D = 0.0;
         if (w) {
             const double dk = delta(t + 1), dm = delta(t);
-            const double dzw = g.dzw[k];
+            const double dzw = zw[k];
             D = e0[t] + dt * fo[t];
             if (k == k0) {
                 A = 0.0;
@@ -284,7 +313,7 @@ __global__ __launch_bounds__(256) void tke_kernel(TkeArgs a) {
                 B = 1 + (dk + dm) / dzw + dt * a.c_eps * sq[t] / ml[t];
             }
             C = k < nz - 1 ? -dk / dzw : 0.0;
-            if (k == nz - 1) D += dt * a.forc_surf[tc.col] / (0.5 * dzw);
+            if (k == nz - 1) D += dt * fsurf / (0.5 * dzw);
         }
         va[t] = A;
         vb[t] = B;
@@ -328,7 +357,7 @@ __global__ __launch_bounds__(256) void tke_kernel(TkeArgs a) {
         dg[t] = a.c_eps / ml[t] * sq[t] * v;  // tke_diss over the whole array (:230)
         bool store = tc.interior && w;
         if (tc.interior && k == nz - 1) {  // surface correction and clamp of every interior column (:235-244)
-            a.surf_corr[tc.col] = v < 0.0 ? -v * 0.5 * g.dzw[k] / dt : 0.0;
+            a.surf_corr[tc.col] = v < 0.0 ? -v * 0.5 * zw[k] / dt : 0.0;
             v = fmax(0.0, v);
             store = true;
         }
@@ -357,13 +386,17 @@ struct EkeArgs {
 };
 
 template <int S, bool VEC>
-__global__ __launch_bounds__(256) void eke_kernel(EkeArgs a) {
+__global__ __launch_bounds__(kTurbBlock) void eke_kernel(EkeArgs a) {
     const TurbGrid& g = a.g;
     const TurbCol tc = turb_col(g, S);
+    __shared__ double sdz[2 * kMaxNz];
+    stage_dz(g, sdz);
+    const double* __restrict__ zt = sdz;           // dzt
+    const double* __restrict__ zw = sdz + kMaxNz;  // dzw
     const int nz = g.nz, lo = tc.lo, k0 = tc.k0;
     const int64_t base = tc.col * nz;
     double km[S + 2];
-    load_halo<S, VEC>(a.kappaM, base, tc.s, lo, nz, km);
+    load_halo_issue<S, VEC>(a.kappaM, base, lo, nz, km);
     double sq[S], el[S];
     load_lv<S, VEC>(a.sqrteke, base, lo, nz, sq);
     load_lv<S, VEC>(a.eke_len, base, lo, nz, el);
@@ -379,10 +412,11 @@ __global__ __launch_bounds__(256) void eke_kernel(EkeArgs a) {
     col_load<S, VEC>(E1 + base * g.cs, g.cs, lo, nz, [&](int k) { return tc.valid && k < k0; }, e1);
     load_terms<S, VEC>(a.forc, base, lo, nz, [&](int k) { return k >= k0; }, fo);
     const double dt = a.dt;
+    load_halo_finish<S, VEC>(tc.s, km);  // after the other loads are issued
     double va[S], vb[S], vc[S], vd[S];
     auto delta = [&](int t) {
         const int k = lo - 1 + t;
-        return (k >= 0 && k < nz - 1) ? dt / g.dzt[k + 1] * 0.5 * (km[t] + km[t + 1]) * a.alpha : 0.0;
+        return (k >= 0 && k < nz - 1) ? dt / zt[k + 1] * 0.5 * (km[t] + km[t + 1]) * a.alpha : 0.0;
     };
 #pragma unroll
     for (int t = 0; t < S; ++t) {
@@ -391,7 +425,7 @@ __global__ __launch_bounds__(256) void eke_kernel(EkeArgs a) {
         double A = 0.0, B = 1.0, C = 0.0, D = 0.0;
         if (w) {
             const double dk = delta(t + 1), dm = delta(t);
-            const double dzw = g.dzw[k];
+            const double dzw = zw[k];
             D = e0[t] + dt * fo[t];
             if (k == k0) {
                 B = 1 + dk / dzw + dt * cint[t];
@@ -475,18 +509,24 @@ struct IdemixArgs {
 };
 
 template <int S, bool VEC>
-__global__ __launch_bounds__(256) void idemix_kernel(IdemixArgs a) {
+__global__ __launch_bounds__(kTurbBlock) void idemix_kernel(IdemixArgs a) {
     const TurbGrid& g = a.g;
     const TurbCol tc = turb_col(g, S);
+    __shared__ double sdz[2 * kMaxNz];
+    stage_dz(g, sdz);
+    const double* __restrict__ zt = sdz;           // dzt
+    const double* __restrict__ zw = sdz + kMaxNz;  // dzw
     const int nz = g.nz;
     const int64_t base = tc.col * nz;
     double c0[S + 2];
-    load_halo<S, VEC>(a.c0, base, tc.s, tc.lo, nz, c0);
+    load_halo_issue<S, VEC>(a.c0, base, tc.lo, nz, c0);
     double al[S];
     load_lv<S, VEC>(a.alpha_c, base, tc.lo, nz, al);
     const double* __restrict__ E0 = a.E + a.tau * g.ls;
     double* __restrict__ E1 = a.E + a.taup1 * g.ls;
     const double dt = a.dt;
+    // the bottom and surface forcing of the column, used by the edge rows
+    const double fbot = a.forc_bot[tc.col], fsurf = a.forc_surf[tc.col];
     // E_iw at tau on every level, at taup1 off the solve (kept there and read by the diagnostic)
     double e0[S], e1[S], mx[S];
 #pragma unroll
@@ -506,8 +546,8 @@ __global__ __launch_bounds__(256) void idemix_kernel(IdemixArgs a) {
 #pragma unroll
         for (int t = 0; t < S; ++t) {
             const int k = tc.lo + t;
-            if (k < nz - 1) part += g.dzw[k] * fo[t];
-            else if (k == nz - 1) part += 0.5 * fo[t] * g.dzw[k];
+            if (k < nz - 1) part += zw[k] * fo[t];
+            else if (k == nz - 1) part += 0.5 * fo[t] * zw[k];
         }
         part += __shfl_xor(part, 1, kSegP);
         part += __shfl_xor(part, 2, kSegP);
@@ -518,8 +558,8 @@ __global__ __launch_bounds__(256) void idemix_kernel(IdemixArgs a) {
         for (int t = 0; t < S; ++t) {
             const int k = tc.lo + t;
             double f = 0.0;
-            if (k == ks) f = (a.diss_mode == 1 ? 1.0 : a.frac) * a_loc / g.dzw[k];
-            if (a.diss_mode == 2 && k == nz - 1) f = (1.0 - a.frac) * a_loc / (0.5 * g.dzw[k]);
+            if (k == ks) f = (a.diss_mode == 1 ? 1.0 : a.frac) * a_loc / zw[k];
+            if (a.diss_mode == 2 && k == nz - 1) f = (1.0 - a.frac) * a_loc / (0.5 * zw[k]);
             fo[t] = f;
         }
         // the moved term applies to every interior column; rows of other
@@ -527,10 +567,11 @@ __global__ __launch_bounds__(256) void idemix_kernel(IdemixArgs a) {
     }
     // terms added afterwards, on the levels whose rows are solved
     add_terms<S, VEC>(a.forc_post, base, tc.lo, nz, [&](int k) { return k >= tc.k0; }, false, fo);
+    load_halo_finish<S, VEC>(tc.s, c0);  // after the other loads are issued
     double va[S], vb[S], vc[S], vd[S];
     auto delta = [&](int t) {
         const int k = tc.lo - 1 + t;
-        return (k >= 0 && k < nz - 1) ? dt * a.tau_v / g.dzt[k + 1] * 0.5 * (c0[t] + c0[t + 1]) : 0.0;
+        return (k >= 0 && k < nz - 1) ? dt * a.tau_v / zt[k + 1] * 0.5 * (c0[t] + c0[t + 1]) : 0.0;
     };
 #pragma unroll
     for (int t = 0; t < S; ++t) {
@@ -539,17 +580,17 @@ __global__ __launch_bounds__(256) void idemix_kernel(IdemixArgs a) {
         double A = 0.0, B = 1.0, C = 0.0, D = 0.0;
         if (w) {
             const double dk = delta(t + 1), dm = delta(t);
-            const double dzw = g.dzw[k];
+            const double dzw = zw[k];
             const double diss = dt * al[t] * mx[t];
             const double cc = c0[t + 1];
             D = e0[t] + dt * fo[t];
             if (k == tc.k0) {
                 B = 1 + dk / dzw * cc + diss;
-                D = D + dt * a.forc_bot[tc.col] / dzw;  // d_edge (:157-159)
+                D = D + dt * fbot / dzw;  // d_edge (:157-159)
             } else if (k == nz - 1) {
                 A = -dm / (0.5 * dzw) * c0[t];
                 B = 1 + dm / (0.5 * dzw) * cc + diss;
-                D += dt * a.forc_surf[tc.col] / (0.5 * dzw);
+                D += dt * fsurf / (0.5 * dzw);
             } else {
                 A = -dm * c0[t] / dzw;
                 B = 1 + dk * cc / dzw + dm * cc / dzw + diss;
@@ -659,7 +700,7 @@ template <class Args, template <int, bool> class K>
 int launch_turb(const Args& a, const TurbGrid& g, bool vec, hipStream_t s, const char* who) {
     const int S = seg_levels(g.nz);
     const int64_t threads = (int64_t)g.X * g.Y * kSegP;
-    const dim3 grid(blocks_mult8(threads, 256)), block(256);
+    const dim3 grid(blocks_mult8(threads, kTurbBlock)), block(kTurbBlock);
 #define VX_TURB_CASE(N)                                                     \
     case N:                                                                 \
         if (vec)                                                            \
