@@ -72,10 +72,13 @@ __device__ inline Face face_of(int kb0, int kb1, int nz) {
 // level (:137-153, 189-201): fxa * (x1[k+1] - x1[k]) / dzw[k] * mask[k+1] *
 // mask[k] times (x0[k+1] - x0[k]) / dzw[k], x0 at tau and x1 at taup1, 0 on
 // the top level. The masks are 0 or 1, so the products are taken as selects.
-// Every lane of the wave calls it (the solve moves data across lanes).
+// Every lane of the wave calls it (the solve moves data across lanes). zt and
+// zw hold dzt and dzw in LDS: read inside per-level selects, a global load
+// would be waited for at once, level after level.
 template <int S, bool VEC>
-__device__ __forceinline__ void fric_face(const FricArgs& p, SegLane L, int t, int64_t col, int64_t nb, bool live,
-                                          bool both, bool store, double (&ds)[S]) {
+__device__ __forceinline__ void fric_face(const FricArgs& p, const double* __restrict__ zt,
+                                          const double* __restrict__ zw, SegLane L, int t, int64_t col, int64_t nb,
+                                          bool live, bool both, bool store, double (&ds)[S]) {
     const int nz = p.nz;
     const int lo = L.lo;
     const int k0 = L.k0;
@@ -99,14 +102,14 @@ __device__ __forceinline__ void fric_face(const FricArgs& p, SegLane L, int t, i
     double va[S], vb[S], vc[S], z[S];
     const double fxm = grp_prev(L.s, fxa[S - 1]);
     const int kp = lo - 1;
-    double dlm = (both && kp >= k0 && kp < nz - 1) ? dt / p.dzw[kp] * fxm : 0.0;
+    double dlm = (both && kp >= k0 && kp < nz - 1) ? dt / zw[kp] * fxm : 0.0;
 #pragma unroll
     for (int q = 0; q < S; ++q) {
         const int k = lo + q;
         const bool wet = k >= k0 && k < nz;
         const int kc = k < nz ? k : nz - 1;
-        const double dz = p.dzt[kc];
-        const double dl = (both && wet && k < nz - 1) ? dt / p.dzw[kc] * fxa[q] : 0.0;
+        const double dz = zt[kc];
+        const double dl = (both && wet && k < nz - 1) ? dt / zw[kc] * fxa[q] : 0.0;
         const bool edge = k == k0;
         va[q] = (wet && !edge) ? -dlm / dz : 0.0;
         vb[q] = wet ? (edge ? 1 + dl / dz : (1 + dlm / dz) + dl / dz) : 1.0;
@@ -129,7 +132,7 @@ __device__ __forceinline__ void fric_face(const FricArgs& p, SegLane L, int t, i
         const double a1 = q < S - 1 ? x1[q < S - 1 ? q + 1 : 0] : x1n;
         const double a0 = q < S - 1 ? x0[q < S - 1 ? q + 1 : 0] : x0n;
         const int kc = k < nz - 1 ? k : 0;
-        const double dzw = p.dzw[kc];
+        const double dzw = zw[kc];
         ds[q] = (both && k >= k0 && k < nz - 1) ? ((a0 - x0[q]) * ((fxa[q] * (a1 - x1[q])) / dzw)) / dzw : 0.0;
     }
     if (!(live && store)) return;
@@ -172,6 +175,14 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(2))) 
     const int64_t b = xcd_contiguous_block();
     if (b >= nblocks) return;
     const int tid = threadIdx.x;
+    __shared__ double sdz[2 * kMaxNz];  // dzt, then dzw
+    for (int k = tid; k < p.nz; k += kSegBlock) {
+        sdz[k] = p.dzt[k];
+        sdz[kMaxNz + k] = p.dzw[k];
+    }
+    __syncthreads();
+    const double* zt = sdz;
+    const double* zw = sdz + kMaxNz;
     const int g = tid / kSegP;
     const int64_t strip = b % nstrips, chunk = b / nstrips;
     const int64_t j = 31 * strip + g;  // the strip's T columns are [31 strip + 1, 31 strip + 32)
@@ -199,7 +210,7 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(2))) 
             const Face f = face_of(kb, jlive ? p.kbot[nb] : 0, nz);
             L.k0 = f.k0;
             double ds[S];
-            fric_face<S, VEC>(p, L, 1, col, nb, jlive, f.both, jown, ds);
+            fric_face<S, VEC>(p, zt, zw, L, 1, col, nb, jlive, f.both, jown, ds);
 #pragma unroll
             for (int q = 0; q < S; ++q) dsv[buf][q * kSegBlock + tid] = ds[q];
         }
@@ -211,7 +222,7 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(2))) 
             const Face f = face_of(ulive ? kb : 0, ulive ? p.kbot[nb] : 0, nz);
             L.k0 = f.k0;
             double ds[S];
-            fric_face<S, VEC>(p, L, 0, col, nb, ulive, f.both, !pre, ds);
+            fric_face<S, VEC>(p, zt, zw, L, 0, col, nb, ulive, f.both, !pre, ds);
             const double wi = p.dxu[i], wm = p.dxu[i - 1], den = 2 * p.dxt[i];
 #pragma unroll
             for (int q = 0; q < S; ++q) {

@@ -58,25 +58,37 @@ __device__ __forceinline__ SegLane vmix_lane(const VmixArgs& p, int S, int64_t& 
     return L;
 }
 
+// dzt (s[0, nz)) and dzw (s[kMaxNz, kMaxNz + nz)) in LDS; every thread of
+// the block calls it. The rows read the spacings inside per-level selects,
+// where a global load is waited for at once, level after level.
+__device__ __forceinline__ void vmix_stage_dz(const VmixArgs& p, double* s) {
+    for (int k = threadIdx.x; k < p.nz; k += blockDim.x) {
+        s[k] = p.dzt[k];
+        s[kMaxNz + k] = p.dzw[k];
+    }
+    __syncthreads();
+}
+
 // The rows of the lane's levels (veros/core/thermodynamics.py:267-277),
 // identity rows off the wet suffix; z1, z2 get the surface forcing on the top
 // level. KM(k) is kappaH at level k of the column, for k in [lo-1, lo+S); its
-// value matters for k0 <= k < nz-1 only.
+// value matters for k0 <= k < nz-1 only. zt, zw: dzt and dzw (in LDS).
 template <int S, class KMF>
-__device__ __forceinline__ void vmix_rows(const VmixArgs& p, const SegLane& L, KMF KM, double fT, double fS,
+__device__ __forceinline__ void vmix_rows(const VmixArgs& p, const double* __restrict__ zt,
+                                          const double* __restrict__ zw, const SegLane& L, KMF KM, double fT, double fS,
                                           double (&va)[S], double (&vb)[S], double (&vc)[S], double (&z1)[S],
                                           double (&z2)[S]) {
     const int nz = p.nz;
     const double dt = p.dt;
     const int kp = L.lo - 1;  // delta at the level below the segment
-    double dlm = (kp >= L.k0 && kp < nz - 1) ? dt / p.dzw[kp > 0 ? kp : 0] * KM(kp) : 0.0;
+    double dlm = (kp >= L.k0 && kp < nz - 1) ? dt / zw[kp > 0 ? kp : 0] * KM(kp) : 0.0;
 #pragma unroll
     for (int t = 0; t < S; ++t) {
         const int k = L.lo + t;
         const bool wet = k >= L.k0 && k < nz;
         const int kc = k < nz ? k : nz - 1;
-        const double dz = p.dzt[kc];
-        const double dl = (wet && k < nz - 1) ? dt / p.dzw[kc] * KM(k) : 0.0;
+        const double dz = zt[kc];
+        const double dl = (wet && k < nz - 1) ? dt / zw[kc] * KM(k) : 0.0;
         const bool edge = k == L.k0;
         va[t] = (wet && !edge) ? -dlm / dz : 0.0;
         vb[t] = wet ? (edge ? 1 + dl / dz : 1 + (dl + dlm) / dz) : 1.0;
@@ -96,6 +108,8 @@ __device__ __forceinline__ void vmix_rows(const VmixArgs& p, const SegLane& L, K
 // arrays 16-byte aligned).
 template <int S, bool VEC>
 __global__ __launch_bounds__(kSegBlock) void vertmix_seg_kernel(VmixArgs p) {
+    __shared__ double sdz[2 * kMaxNz];
+    vmix_stage_dz(p, sdz);
     int64_t col;
     const SegLane L = vmix_lane(p, S, col);
     const int nz = p.nz;
@@ -123,7 +137,7 @@ __global__ __launch_bounds__(kSegBlock) void vertmix_seg_kernel(VmixArgs p) {
         z1[t] = oT[t];
         z2[t] = oS[t];
     }
-    vmix_rows<S>(p, L, [&](int k) { return km[k - L.lo + 1]; }, fT, fS, va, vb, vc, z1, z2);
+    vmix_rows<S>(p, sdz, sdz + kMaxNz, L, [&](int k) { return km[k - L.lo + 1]; }, fT, fS, va, vb, vc, z1, z2);
     seg_solve2_f64<S>(L, va, vb, vc, z1, z2);
     if (!L.valid) return;
     const double dt = p.dt;
@@ -191,6 +205,8 @@ __host__ __device__ constexpr int vmix_lds_wave_bytes(int S) { return 3 * kSegCo
 template <int S>
 __global__ __launch_bounds__(kSegLdsBlock) void vertmix_segl_kernel(VmixArgs p) {
     extern __shared__ double vmix_lds[];
+    __shared__ double sdz[2 * kMaxNz];
+    vmix_stage_dz(p, sdz);
     constexpr int IMG = kSegCols * kSegP * S;
     double* lds = vmix_lds + (threadIdx.x / 64) * 3 * IMG;
     int64_t col;
@@ -212,7 +228,7 @@ __global__ __launch_bounds__(kSegLdsBlock) void vertmix_segl_kernel(VmixArgs p) 
     double va[S], vb[S], vc[S], z1[S], z2[S];
     seg_lds_read<S>(lds + 1 * IMG, L, z1);
     seg_lds_read<S>(lds + 2 * IMG, L, z2);
-    vmix_rows<S>(p, L, [&](int k) { return km[k > 0 ? k : 0]; }, fT, fS, va, vb, vc, z1, z2);
+    vmix_rows<S>(p, sdz, sdz + kMaxNz, L, [&](int k) { return km[k > 0 ? k : 0]; }, fT, fS, va, vb, vc, z1, z2);
     seg_solve2_f64<S>(L, va, vb, vc, z1, z2);
     // tracers: new values on the wet levels, the old ones (still in the
     // images) elsewhere
