@@ -165,7 +165,7 @@ __device__ __forceinline__ void fric_face(const FricArgs& p, const double* __res
 // area_t) for 2 <= j < Y-2 (face j alone for j = 1), formed as (K + u part)
 // + v part on the levels
 // below the top one (which dissipates nothing). Blocks are XCD-contiguous,
-// strips fastest. Two waves per SIMD: 256 VGPRs at nz = 80.
+// strips fastest. Two waves per SIMD: 218 VGPRs at nz = 80.
 // VEC: levels outermost, nz even and every array 16-byte aligned.
 template <int S, bool VEC>
 __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(2))) void friction_walk_kernel(FricArgs p, int rows, int64_t nstrips,
