@@ -357,10 +357,15 @@ __device__ __forceinline__ void pre_wet_cell(const Grid& g, const Prm& p, const 
     const int kb_c = g.kbot[i * g.Y + j], kb_e = g.kbot[(i + 1) * g.Y + j], kb_w = g.kbot[(i - 1) * g.Y + j];
     const int kb_n = g.kbot[i * g.Y + j + 1], kb_s = g.kbot[i * g.Y + j - 1];
 
-    // tracer values: c(enter), e(ast), w(est), n(orth), s(outh); p = k+1, m = k-1
+    // tracer values: c(enter), e(ast), w(est), n(orth), s(outh); p = k+1, m = k-1.
+    // Loaded whether or not they are used (k+1 at the top and k-1 at the
+    // bottom are cells of the next and previous columns, in bounds for every
+    // cell pre visits, i in [1, X-3], j in [1, Y-3]): a load under a per-lane
+    // condition is waited for at once.
     auto L2 = [&](uint32_t o, bool ok, double& t, double& sv) {
-        t = ok ? T[g.cs * o] : 0.0;
-        sv = ok ? S[g.cs * o] : 0.0;
+        const double tv = T[g.cs * o], svv = S[g.cs * o];
+        t = ok ? tv : 0.0;
+        sv = ok ? svv : 0.0;
     };
     double Tc, Sc, Tcp, Scp, Tcm, Scm, Te, Se, Tep, Sep, Tem, Sem, Tw, Sw, Twp, Swp;
     double Tn, Sn, Tnp, Snp, Tnm, Snm, Ts, Ss, Tsp, Ssp;
@@ -380,12 +385,13 @@ __device__ __forceinline__ void pre_wet_cell(const Grid& g, const Prm& p, const 
 
     // Every other global read up front: reads placed inside the divergent
     // face branches below would each cost a separate memory round trip.
-    const double dzw_k = g.dzw[k], dzw_m = hm ? g.dzw[k - 1] : 0.0, dzt_k = g.dzt[k];
+    const double dzw_k = g.dzw[k], dzw_l = g.dzw[hm ? k - 1 : k], dzt_k = g.dzt[k];
+    const double dzw_m = hm ? dzw_l : 0.0;
     const double dxu_i = g.dxu[i], dxu_w = g.dxu[i - 1], dxt_i = g.dxt[i];
     const double cost_j = g.cost[j], dyu_j = g.dyu[j], dyu_s = g.dyu[j - 1], dyt_j = g.dyt[j];
     const double cosu_j = g.cosu[j], cosu_s = g.cosu[j - 1];
-    const double Kc = K[c], Kcm = hm ? K[c - 1] : 0.0, Ke = K[ce], Kem = hm ? K[ce - 1] : 0.0;
-    const double Kn = K[cn], Knm = hm ? K[cn - 1] : 0.0;
+    const double Kc = K[c], Kcl = K[c - 1], Ke = K[ce], Kel = K[ce - 1], Kn = K[cn], Knl = K[cn - 1];
+    const double Kcm = hm ? Kcl : 0.0, Kem = hm ? Kel : 0.0, Knm = hm ? Knl : 0.0;
 
     // reciprocal metric factors
     const double rz_c = hp ? rcp_nr(dzw_k) : 0.0;
