@@ -30,8 +30,9 @@ HBM_PEAK_GBS = 8000.0
 # profiles/mixing_step_1440x720x80.md (read bytes = 2.0 x FETCH_SIZE, write
 # bytes = 1.0 x WRITE_SIZE, calibrated on a 1 GiB device copy), per layout of
 # the time-level variables
-# (the levels-innermost figure predates the segmented K33 solve of the step kernel)
-PRE_TS_TRAFFIC_BYTES = {(1440, 720, 80, "outer"): 21_432_116_224, (1440, 720, 80, "inner"): 27_204_216_832}
+# (the levels-innermost figure predates the segmented K33 solve of the step
+# kernel and pre's unconditional neighbour loads)
+PRE_TS_TRAFFIC_BYTES = {(1440, 720, 80, "outer"): 21_844_174_848, (1440, 720, 80, "inner"): 27_204_216_832}
 # the minimum bytes per interior cell of pre + the temperature + salinity step
 # in one call: K_iso read (8), Ai_* and K_11/22/33 written once and not read
 # back (128 + 24), P_diss read and written (16), and per tracer tr[tau] (8),
