@@ -231,15 +231,19 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(2))) 
             }
         }
         if (pre) continue;
+        // K_diss_v and the weights are loaded before the barrier, which their
+        // latency then overlaps
+        double K[S];
+#pragma unroll
+        for (int q = 0; q < S; ++q) K[q] = 0.0;
+        double* Kp = p.K_diss_v + L.base;
+        double wj = 0.0, wjm = 0.0, dnj = 1.0;
+        if (jown) {
+            col_load<S, VEC>(Kp, 1, L.lo, nz, [&](int k) { return k < nz - 1; }, K);
+            wj = p.area_v[col], wjm = p.area_v[col - 1], dnj = 2 * p.area_t[col];
+        }
         __syncthreads();
         if (jown) {
-            double K[S];
-#pragma unroll
-            for (int q = 0; q < S; ++q) K[q] = 0.0;
-            double* Kp = p.K_diss_v + L.base;
-            col_load<S, VEC>(Kp, 1, L.lo, nz, [&](int k) { return k < nz - 1; }, K);
-            const int64_t ij = col;
-            const double wj = p.area_v[ij], wjm = p.area_v[ij - 1], dnj = 2 * p.area_t[ij];
             bool st[S];
 #pragma unroll
             for (int q = 0; q < S; ++q) {
