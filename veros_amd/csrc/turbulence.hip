@@ -271,18 +271,61 @@ __global__ __launch_bounds__(kTurbBlock) void tke_kernel(TkeArgs a) {
     const double* __restrict__ zw = sdz + kMaxNz;  // dzw
     const int nz = g.nz, lo = tc.lo, k0 = tc.k0;
     const int64_t base = tc.col * nz;
+    const double* __restrict__ E0 = a.tke + a.tau * g.ls;
+    double* __restrict__ E1 = a.tke + a.taup1 * g.ls;
+    // VEC: the lateral diffusion's east and west neighbour columns (and, for
+    // the wave's first and last groups, the south and north ones outside the
+    // wave) are copied into the wave's LDS now, asynchronously and without
+    // registers; the other north and south neighbours are the adjacent groups'
+    // own columns, taken from their registers after the solve. Lane l's pair
+    // of region r lands at lat_stage[wave][r][l], read back by lane l only.
+    constexpr int NP = S / 2;
+    __shared__ double lat_stage[VEC ? (kTurbBlock / 64) * 3 * NP * 128 : 1];
+    double* const stg = lat_stage + (VEC ? (threadIdx.x / 64) * 3 * NP * 128 : 0);
+    const int g8 = (int)((threadIdx.x & 63) / kSegP);  // group in the wave
+    const bool lat_on = a.hor && tc.interior;
+    int kE = 0, kW = 0, kX = 0;
+    double m_cost = 1.0, m_dxu = 1.0, m_dxuw = 1.0, m_dyu = 1.0, m_dyus = 1.0, m_cosu = 0.0, m_cosus = 0.0,
+           m_dxt = 1.0, m_dyt = 1.0;
+    if constexpr (VEC) {
+        if (a.hor) {
+            const int64_t Y = g.Y;
+            const int64_t dE = lat_on ? Y * nz : 0;
+            const int64_t dX = lat_on ? (g8 == 0 ? -(int64_t)nz : (g8 == kSegCols - 1 ? (int64_t)nz : 0)) : 0;
+            const double* pc = E0 + base;
+#pragma unroll
+            for (int t = 0; t < S; t += 2) {
+                const int k = lo + t < nz - 2 ? lo + t : nz - 2;  // levels past the column: discarded
+                __builtin_amdgcn_global_load_lds((const void*)(pc + dE + k), (lds_void_ptr)(stg + (0 * NP + t / 2) * 128),
+                                                 16, 0, 0);
+                __builtin_amdgcn_global_load_lds((const void*)(pc - dE + k), (lds_void_ptr)(stg + (1 * NP + t / 2) * 128),
+                                                 16, 0, 0);
+                __builtin_amdgcn_global_load_lds((const void*)(pc + dX + k), (lds_void_ptr)(stg + (2 * NP + t / 2) * 128),
+                                                 16, 0, 0);
+            }
+            const int64_t cj = lat_on ? tc.col : (int64_t)Y + 1;  // column (1, 1) when unused
+            const int i = (int)(cj / Y), j = (int)(cj - (cj / Y) * Y);
+            kE = g.kbot[cj + Y], kW = g.kbot[cj - Y];
+            kX = g.kbot[cj + (g8 == 0 ? -1 : 1)];
+            m_cost = g.cost[j], m_dxu = g.dxu[i], m_dxuw = g.dxu[i - 1], m_dyu = g.dyu[j], m_dyus = g.dyu[j - 1];
+            m_cosu = g.cosu[j], m_cosus = g.cosu[j - 1], m_dxt = g.dxt[i], m_dyt = g.dyt[j];
+        }
+    }
     double km[S + 2];
     load_halo_issue<S, VEC>(a.kappaM, base, lo, nz, km);
     double sq[S], ml[S];
     load_lv<S, VEC>(a.sqrttke, base, lo, nz, sq);
     load_lv<S, VEC>(a.mxl, base, lo, nz, ml);
-    const double* __restrict__ E0 = a.tke + a.tau * g.ls;
-    double* __restrict__ E1 = a.tke + a.taup1 * g.ls;
-    // tau on the wet levels, taup1 on the others (kept there and read by the diagnostic)
+    // tau on the wet levels, taup1 on the others (kept there and read by the
+    // diagnostic); VEC with lateral diffusion: tau on every column's wet
+    // levels, halo columns included, for the neighbours' exchange
     double e0[S], e1[S], fo[S];
 #pragma unroll
     for (int t = 0; t < S; ++t) e0[t] = e1[t] = 0.0;
-    col_load<S, VEC>(E0 + base * g.cs, g.cs, lo, nz, [&](int k) { return k >= k0; }, e0);
+    if (VEC && a.hor)
+        col_load<S, VEC>(E0 + base * g.cs, g.cs, lo, nz, [&](int k) { return tc.valid && wet_at(tc.kb, k); }, e0);
+    else
+        col_load<S, VEC>(E0 + base * g.cs, g.cs, lo, nz, [&](int k) { return k >= k0; }, e0);
     col_load<S, VEC>(E1 + base * g.cs, g.cs, lo, nz, [&](int k) { return tc.valid && k < k0; }, e1);
     load_terms<S, VEC>(a.forc, base, lo, nz, [&](int k) { return k >= k0; }, fo);
     const double fsurf = a.forc_surf[tc.col];  // loaded with the rest, used by the top row
@@ -327,11 +370,49 @@ __global__ __launch_bounds__(kTurbBlock) void tke_kernel(TkeArgs a) {
     L.valid = tc.valid;
     L.base = base;
     seg_solve_f64<S>(L, va, vb, vc, vd);
-    if (!tc.valid) return;
     // the lateral diffusion of the tau level (:250-276)
     double lat[S];
-    {
-        const double Kh = a.K_h;
+    const double Kh = a.K_h;
+    if constexpr (VEC) {
+#pragma unroll
+        for (int t = 0; t < S; ++t) lat[t] = 0.0;
+        if (a.hor) {
+            // the adjacent groups' columns (j-1 and j+1 inside the wave): every
+            // lane takes part in the exchange
+            const int lane = (int)(threadIdx.x & 63);
+            const int kN_in = __shfl(tc.kb, (lane + kSegP) & 63), kS_in = __shfl(tc.kb, (lane - kSegP) & 63);
+            double eN[S], eS[S];
+#pragma unroll
+            for (int t = 0; t < S; ++t) {
+                eN[t] = __shfl(e0[t], (lane + kSegP) & 63);
+                eS[t] = __shfl(e0[t], (lane - kSegP) & 63);
+            }
+            seg_wait_vmem();  // the staged neighbours have landed
+            if (lat_on) {
+                const int kC = tc.kb;
+                const int kN = g8 == kSegCols - 1 ? kX : kN_in, kS = g8 == 0 ? kX : kS_in;
+#pragma unroll
+                for (int t = 0; t < S; ++t) {
+                    const int kk = lo + t, h = t & 1;
+                    const double* sp = stg + 2 * lane + h;
+                    const double fe_ = sp[(0 * NP + t / 2) * 128], fw_ = sp[(1 * NP + t / 2) * 128];
+                    const double fx_ = sp[(2 * NP + t / 2) * 128];
+                    const double fn_ = g8 == kSegCols - 1 ? fx_ : eN[t], fs_ = g8 == 0 ? fx_ : eS[t];
+                    const double fc = e0[t];
+                    const bool mC = wet_at(kC, kk), mE = wet_at(kE, kk), mW = wet_at(kW, kk);
+                    const bool mN = wet_at(kN, kk), mS = wet_at(kS, kk);
+                    const double fe = Kh * (fe_ - fc) / (m_cost * m_dxu) * ((mC && mE) ? 1.0 : 0.0);
+                    const double fw = Kh * (fc - fw_) / (m_cost * m_dxuw) * ((mW && mC) ? 1.0 : 0.0);
+                    const double fn = Kh * (fn_ - fc) / m_dyu * ((mC && mN) ? 1.0 : 0.0) * m_cosu;
+                    const double fs = Kh * (fc - fs_) / m_dyus * ((mS && mC) ? 1.0 : 0.0) * m_cosus;
+                    const double r = (fe - fw) / (m_cost * m_dxt) + (fn - fs) / (m_cost * m_dyt);
+                    lat[t] = (kk < nz && mC) ? r : 0.0;
+                }
+            }
+        }
+    }
+    if (!tc.valid) return;
+    if constexpr (!VEC) {
         lateral_levels<S, VEC>(
             lo, [&](int k) { return tc.interior && a.hor && k < nz && wet_at(tc.kb, k); },
             [&](int k) {
