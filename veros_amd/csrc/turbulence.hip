@@ -293,15 +293,20 @@ __global__ __launch_bounds__(kTurbBlock) void tke_kernel(TkeArgs a) {
             const int64_t dE = lat_on ? Y * nz : 0;
             const int64_t dX = lat_on ? (g8 == 0 ? -(int64_t)nz : (g8 == kSegCols - 1 ? (int64_t)nz : 0)) : 0;
             const double* pc = E0 + base;
+            // only the pairs that touch the column's wet suffix (the term is
+            // taken on wet levels only); the other lanes' slots are not read
+            const int kw = tc.kb >= 1 && tc.kb <= nz ? tc.kb - 1 : nz;
 #pragma unroll
             for (int t = 0; t < S; t += 2) {
-                const int k = lo + t < nz - 2 ? lo + t : nz - 2;  // levels past the column: discarded
-                __builtin_amdgcn_global_load_lds((const void*)(pc + dE + k), (lds_void_ptr)(stg + (0 * NP + t / 2) * 128),
-                                                 16, 0, 0);
-                __builtin_amdgcn_global_load_lds((const void*)(pc - dE + k), (lds_void_ptr)(stg + (1 * NP + t / 2) * 128),
-                                                 16, 0, 0);
-                __builtin_amdgcn_global_load_lds((const void*)(pc + dX + k), (lds_void_ptr)(stg + (2 * NP + t / 2) * 128),
-                                                 16, 0, 0);
+                const int k = lo + t;
+                if (lat_on && k < nz && k + 1 >= kw) {
+                    __builtin_amdgcn_global_load_lds((const void*)(pc + dE + k),
+                                                     (lds_void_ptr)(stg + (0 * NP + t / 2) * 128), 16, 0, 0);
+                    __builtin_amdgcn_global_load_lds((const void*)(pc - dE + k),
+                                                     (lds_void_ptr)(stg + (1 * NP + t / 2) * 128), 16, 0, 0);
+                    __builtin_amdgcn_global_load_lds((const void*)(pc + dX + k),
+                                                     (lds_void_ptr)(stg + (2 * NP + t / 2) * 128), 16, 0, 0);
+                }
             }
             const int64_t cj = lat_on ? tc.col : (int64_t)Y + 1;  // column (1, 1) when unused
             const int i = (int)(cj / Y), j = (int)(cj - (cj / Y) * Y);
