@@ -71,6 +71,10 @@ CASES = [
     ("tke", (37, 29, 80), dict(enable_tke_hor_diffusion=True)),
     ("tke", (20, 30, 33), dict(enable_eke=True, enable_idemix=True)),
     ("tke", (16, 12, 128), dict(enable_store_cabbeling_heat=True)),
+    # lateral diffusion at the deepest columns (the most LDS per block) and
+    # with lanes whose level pairs run past the column (nz 50: 8 levels per lane)
+    ("tke", (19, 23, 128), dict(enable_tke_hor_diffusion=True)),
+    ("tke", (26, 21, 50), dict(enable_tke_hor_diffusion=True, enable_eke=True)),
     ("eke", (37, 29, 80), dict(enable_eke=True)),
     ("eke", (24, 18, 17), dict(enable_eke=True, enable_store_cabbeling_heat=True)),
     ("idemix", (37, 29, 80), dict(enable_eke=True, enable_eke_diss_bottom=True, enable_idemix_hor_diffusion=True)),
