@@ -193,22 +193,24 @@ __device__ __forceinline__ void vmix_store_pairs(double* img, const SegCopyPlan<
     }
 }
 
-// LDS bytes of one wave: kappaH, temp and salt images of its 8 columns
-__host__ __device__ constexpr int vmix_lds_wave_bytes(int S) { return 3 * kSegCols * kSegP * S * 8; }
+// LDS bytes of one wave: temp and salt images of its 8 columns
+__host__ __device__ constexpr int vmix_lds_wave_bytes(int S) { return 2 * kSegCols * kSegP * S * 8; }
 
 // LDS variant (time levels outermost, nz even, kappaH, both tracer levels and
 // both tendencies 16-byte aligned): the wave copies the wet pairs of its 8
-// columns' kappaH, temp and salt into LDS, each lane reads its levels (kappaH
-// from level lo-1), and the results go back through the images: tracers on
-// the pairs that touch the wet suffix (a pair's dry level gets its old value
-// back), tendencies on every pair.
+// columns' temp and salt into LDS, each lane reads its levels there and loads
+// its kappaH pairs itself (level lo-1 from the lane below), and the results go
+// back through the two images: tracers on the pairs that touch the wet suffix
+// (a pair's dry level gets its old value back), tendencies on every pair. Two
+// images rather than three (kappaH staged too) fit six blocks per CU in LDS
+// instead of five, which is what the registers allow (three waves per SIMD).
 template <int S>
 __global__ __launch_bounds__(kSegLdsBlock) void vertmix_segl_kernel(VmixArgs p) {
     extern __shared__ double vmix_lds[];
     __shared__ double sdz[2 * kMaxNz];
     vmix_stage_dz(p, sdz);
     constexpr int IMG = kSegCols * kSegP * S;
-    double* lds = vmix_lds + (threadIdx.x / 64) * 3 * IMG;
+    double* lds = vmix_lds + (threadIdx.x / 64) * 2 * IMG;
     int64_t col;
     const SegLane L = vmix_lane(p, S, col);
     const int nz = p.nz;
@@ -217,25 +219,31 @@ __global__ __launch_bounds__(kSegLdsBlock) void vertmix_segl_kernel(VmixArgs p) 
     {
         SegCopyPlan<S> wet;
         seg_copy_plan<S>(L, threadIdx.x & 63, nz, true, wet);
-        seg_lds_copy<S>(p.kappaH, lds + 0 * IMG, wet);
-        seg_lds_copy<S>(T, lds + 1 * IMG, wet);
-        seg_lds_copy<S>(Sa, lds + 2 * IMG, wet);
+        seg_lds_copy<S>(T, lds + 0 * IMG, wet);
+        seg_lds_copy<S>(Sa, lds + 1 * IMG, wet);
     }
+    // kappaH at levels lo-1 .. lo+S-1 (its value matters on the wet levels only)
+    double kmw[S], km[S + 1];
+#pragma unroll
+    for (int t = 0; t < S; ++t) kmw[t] = 0.0;
+    col_load<S, true>(p.kappaH + L.base, 1, L.lo, nz, [&](int k) { return k + 1 >= L.k0; }, kmw);
     const bool solve = L.k0 < nz;
     const double fT = solve ? p.forc[0][col] : 0.0, fS = solve ? p.forc[1][col] : 0.0;
+    km[0] = grp_prev(L.s, kmw[S - 1]);
+#pragma unroll
+    for (int t = 0; t < S; ++t) km[t + 1] = kmw[t];
     seg_wait_vmem();
-    const double* km = lds + ((threadIdx.x & 63) / kSegP) * (kSegP * S);  // the column's kappaH
     double va[S], vb[S], vc[S], z1[S], z2[S];
-    seg_lds_read<S>(lds + 1 * IMG, L, z1);
-    seg_lds_read<S>(lds + 2 * IMG, L, z2);
-    vmix_rows<S>(p, sdz, sdz + kMaxNz, L, [&](int k) { return km[k > 0 ? k : 0]; }, fT, fS, va, vb, vc, z1, z2);
+    seg_lds_read<S>(lds + 0 * IMG, L, z1);
+    seg_lds_read<S>(lds + 1 * IMG, L, z2);
+    vmix_rows<S>(p, sdz, sdz + kMaxNz, L, [&](int k) { return km[k - L.lo + 1]; }, fT, fS, va, vb, vc, z1, z2);
     seg_solve2_f64<S>(L, va, vb, vc, z1, z2);
     // tracers: new values on the wet levels, the old ones (still in the
     // images) elsewhere
     const double dt = p.dt;
     double oT[S], oS[S];
-    seg_lds_read<S>(lds + 1 * IMG, L, oT);
-    seg_lds_read<S>(lds + 2 * IMG, L, oS);
+    seg_lds_read<S>(lds + 0 * IMG, L, oT);
+    seg_lds_read<S>(lds + 1 * IMG, L, oS);
 #pragma unroll
     for (int t = 0; t < S; ++t) {
         const int k = L.lo + t;
@@ -253,8 +261,8 @@ __global__ __launch_bounds__(kSegLdsBlock) void vertmix_segl_kernel(VmixArgs p) 
     vmix_store_pairs<S>(lds + 1 * IMG, wet, Sa, L, z2);
     SegCopyPlan<S> all;
     seg_copy_plan<S>(L, threadIdx.x & 63, nz, false, all);
-    vmix_store_pairs<S>(lds + 2 * IMG, all, p.dvmix[0], L, oT);
-    vmix_store_pairs<S>(lds + 0 * IMG, all, p.dvmix[1], L, oS);
+    vmix_store_pairs<S>(lds + 0 * IMG, all, p.dvmix[0], L, oT);
+    vmix_store_pairs<S>(lds + 1 * IMG, all, p.dvmix[1], L, oS);
 }
 
 // enforce_boundaries on one process (veros/core/utilities.py:7-21): the x
