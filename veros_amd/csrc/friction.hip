@@ -75,7 +75,7 @@ __device__ inline Face face_of(int kb0, int kb1, int nz) {
 // Every lane of the wave calls it (the solve moves data across lanes). zt and
 // zw hold dzt and dzw in LDS: read inside per-level selects, a global load
 // would be waited for at once, level after level.
-template <int S, bool VEC>
+template <int S, bool VEC, bool PV>
 __device__ __forceinline__ void fric_face(const FricArgs& p, const double* __restrict__ zt,
                                           const double* __restrict__ zw, SegLane L, int t, int64_t col, int64_t nb,
                                           bool live, bool both, bool store, double (&ds)[S]) {
@@ -88,8 +88,8 @@ __device__ __forceinline__ void fric_face(const FricArgs& p, const double* __res
     double ka[S], kn[S], x0[S], x1[S];
 #pragma unroll
     for (int q = 0; q < S; ++q) ka[q] = kn[q] = x0[q] = x1[q] = 0.0;
-    col_load<S, VEC>(p.kappaM + base, 1, lo, nz, [&](int k) { return k >= k0; }, ka);
-    col_load<S, VEC>(p.kappaM + nb * nz, 1, lo, nz, [&](int k) { return k >= k0; }, kn);
+    col_load<S, PV>(p.kappaM + base, 1, lo, nz, [&](int k) { return k >= k0; }, ka);
+    col_load<S, PV>(p.kappaM + nb * nz, 1, lo, nz, [&](int k) { return k >= k0; }, kn);
     // tau on every level, taup1 below the solve (kept there: du_mix reads it)
     const double* u0p = p.vel[t] + p.tau * p.ls + p.cs * base;
     double* u1p = p.vel[t] + p.taup1 * p.ls + p.cs * base;
@@ -147,7 +147,7 @@ __device__ __forceinline__ void fric_face(const FricArgs& p, const double* __res
     // the levels below the solve hold taup1's own values, so a pair store
     // that covers one of them writes back what is there
     col_store_if<S, VEC>(u1p, p.cs, lo, nz, x1, st);
-    col_store<S, VEC>(p.dmix[t] + base, 1, lo, nz, dm);
+    col_store<S, PV>(p.dmix[t] + base, 1, lo, nz, dm);
 }
 
 // A block of 32 face-column groups owns a strip of 31 T columns j (group g at
@@ -166,8 +166,11 @@ __device__ __forceinline__ void fric_face(const FricArgs& p, const double* __res
 // + v part on the levels
 // below the top one (which dissipates nothing). Blocks are XCD-contiguous,
 // strips fastest. Two waves per SIMD: 218 VGPRs at nz = 80.
-// VEC: levels outermost, nz even and every array 16-byte aligned.
-template <int S, bool VEC>
+// VEC: levels outermost, nz even and every array 16-byte aligned. PV (implied
+// by VEC): nz even and the arrays without time levels (kappaM, du_mix, dv_mix,
+// K_diss_v) 16-byte aligned, which then move in level pairs whatever the
+// layout of u and v.
+template <int S, bool VEC, bool PV>
 __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(2))) void friction_walk_kernel(FricArgs p, int rows, int64_t nstrips,
                                                                   int64_t nblocks) {
     __shared__ double dsu[S * kSegBlock];
@@ -210,7 +213,7 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(2))) 
             const Face f = face_of(kb, jlive ? p.kbot[nb] : 0, nz);
             L.k0 = f.k0;
             double ds[S];
-            fric_face<S, VEC>(p, zt, zw, L, 1, col, nb, jlive, f.both, jown, ds);
+            fric_face<S, VEC, PV>(p, zt, zw, L, 1, col, nb, jlive, f.both, jown, ds);
 #pragma unroll
             for (int q = 0; q < S; ++q) dsv[buf][q * kSegBlock + tid] = ds[q];
         }
@@ -222,7 +225,7 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(2))) 
             const Face f = face_of(ulive ? kb : 0, ulive ? p.kbot[nb] : 0, nz);
             L.k0 = f.k0;
             double ds[S];
-            fric_face<S, VEC>(p, zt, zw, L, 0, col, nb, ulive, f.both, !pre, ds);
+            fric_face<S, VEC, PV>(p, zt, zw, L, 0, col, nb, ulive, f.both, !pre, ds);
             const double wi = p.dxu[i], wm = p.dxu[i - 1], den = 2 * p.dxt[i];
 #pragma unroll
             for (int q = 0; q < S; ++q) {
@@ -239,7 +242,7 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(2))) 
         double* Kp = p.K_diss_v + L.base;
         double wj = 0.0, wjm = 0.0, dnj = 1.0;
         if (jown) {
-            col_load<S, VEC>(Kp, 1, L.lo, nz, [&](int k) { return k < nz - 1; }, K);
+            col_load<S, PV>(Kp, 1, L.lo, nz, [&](int k) { return k < nz - 1; }, K);
             wj = p.area_v[col], wjm = p.area_v[col - 1], dnj = 2 * p.area_t[col];
         }
         __syncthreads();
@@ -252,7 +255,7 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(2))) 
                 st[q] = L.lo + q < nz - 1;
                 K[q] = st[q] ? (K[q] + bu[q]) + bv : K[q];
             }
-            col_store_if<S, VEC>(Kp, 1, L.lo, nz, K, st);
+            col_store_if<S, PV>(Kp, 1, L.lo, nz, K, st);
         }
     }
 }
@@ -262,15 +265,20 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(2))) 
 #endif
 
 template <int S>
-void launch_fric_walk(const FricArgs& p, bool vec, hipStream_t s) {
+void launch_fric_walk(const FricArgs& p, bool vec, bool pv, hipStream_t s) {
     const int rows = VX_FRIC_ROWS;
     const int64_t nstrips = (p.Y - 3 + 30) / 31, nchunks = (p.X - 3 + rows - 1) / rows;
     const int64_t nblocks = nstrips * nchunks;
     const dim3 grid((unsigned)((nblocks + 7) / 8 * 8));
     if (vec)
-        hipLaunchKernelGGL((friction_walk_kernel<S, true>), grid, dim3(kSegBlock), 0, s, p, rows, nstrips, nblocks);
+        hipLaunchKernelGGL((friction_walk_kernel<S, true, true>), grid, dim3(kSegBlock), 0, s, p, rows, nstrips,
+                           nblocks);
+    else if (pv)
+        hipLaunchKernelGGL((friction_walk_kernel<S, false, true>), grid, dim3(kSegBlock), 0, s, p, rows, nstrips,
+                           nblocks);
     else
-        hipLaunchKernelGGL((friction_walk_kernel<S, false>), grid, dim3(kSegBlock), 0, s, p, rows, nstrips, nblocks);
+        hipLaunchKernelGGL((friction_walk_kernel<S, false, false>), grid, dim3(kSegBlock), 0, s, p, rows, nstrips,
+                           nblocks);
 }
 
 }  // namespace
@@ -321,18 +329,18 @@ extern "C" int vx_implicit_vert_friction_f64(void* stream, const vx_grid* grid, 
     const bool outer = grid->time_layout == VX_LEVELS_OUTER;
     p.cs = outer ? 1 : 3;
     p.ls = outer ? p.X * p.Y * p.nz : 1;
-    const bool vec = outer && !(p.nz & 1) && aligned16(u) && aligned16(v) && aligned16(kappaM) &&
-                     aligned16(du_mix) && aligned16(dv_mix) && aligned16(K_diss_v);
+    const bool pv = !(p.nz & 1) && aligned16(kappaM) && aligned16(du_mix) && aligned16(dv_mix) && aligned16(K_diss_v);
+    const bool vec = pv && outer && aligned16(u) && aligned16(v);
     hipStream_t s = (hipStream_t)stream;
     switch (seg_levels(p.nz)) {
-        case 2: launch_fric_walk<2>(p, vec, s); break;
-        case 4: launch_fric_walk<4>(p, vec, s); break;
-        case 6: launch_fric_walk<6>(p, vec, s); break;
-        case 8: launch_fric_walk<8>(p, vec, s); break;
-        case 10: launch_fric_walk<10>(p, vec, s); break;
-        case 12: launch_fric_walk<12>(p, vec, s); break;
-        case 14: launch_fric_walk<14>(p, vec, s); break;
-        default: launch_fric_walk<16>(p, vec, s); break;
+        case 2: launch_fric_walk<2>(p, vec, pv, s); break;
+        case 4: launch_fric_walk<4>(p, vec, pv, s); break;
+        case 6: launch_fric_walk<6>(p, vec, pv, s); break;
+        case 8: launch_fric_walk<8>(p, vec, pv, s); break;
+        case 10: launch_fric_walk<10>(p, vec, pv, s); break;
+        case 12: launch_fric_walk<12>(p, vec, pv, s); break;
+        case 14: launch_fric_walk<14>(p, vec, pv, s); break;
+        default: launch_fric_walk<16>(p, vec, pv, s); break;
     }
     return check_launch(who);
 }
