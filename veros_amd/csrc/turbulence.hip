@@ -260,8 +260,10 @@ struct TkeArgs {
 };
 
 // VEC (all three kernels): levels outermost, nz even and every array
-// 16-byte aligned; the lanes then move their levels in 16-byte pairs.
-template <int S, bool VEC>
+// 16-byte aligned; the lanes then move their levels in 16-byte pairs. PV
+// (implied by VEC): nz even and the arrays without time levels aligned, which
+// then move in pairs whatever the layout of the energy's time levels.
+template <int S, bool VEC, bool PV>
 __global__ __launch_bounds__(kTurbBlock) void tke_kernel(TkeArgs a) {
     const TurbGrid& g = a.g;
     const TurbCol tc = turb_col(g, S);
@@ -317,10 +319,10 @@ __global__ __launch_bounds__(kTurbBlock) void tke_kernel(TkeArgs a) {
         }
     }
     double km[S + 2];
-    load_halo_issue<S, VEC>(a.kappaM, base, lo, nz, km);
+    load_halo_issue<S, PV>(a.kappaM, base, lo, nz, km);
     double sq[S], ml[S];
-    load_lv<S, VEC>(a.sqrttke, base, lo, nz, sq);
-    load_lv<S, VEC>(a.mxl, base, lo, nz, ml);
+    load_lv<S, PV>(a.sqrttke, base, lo, nz, sq);
+    load_lv<S, PV>(a.mxl, base, lo, nz, ml);
     // tau on the wet levels, taup1 on the others (kept there and read by the
     // diagnostic); VEC with lateral diffusion: tau on every column's wet
     // levels, halo columns included, for the neighbours' exchange
@@ -332,10 +334,10 @@ __global__ __launch_bounds__(kTurbBlock) void tke_kernel(TkeArgs a) {
     else
         col_load<S, VEC>(E0 + base * g.cs, g.cs, lo, nz, [&](int k) { return k >= k0; }, e0);
     col_load<S, VEC>(E1 + base * g.cs, g.cs, lo, nz, [&](int k) { return tc.valid && k < k0; }, e1);
-    load_terms<S, VEC>(a.forc, base, lo, nz, [&](int k) { return k >= k0; }, fo);
+    load_terms<S, PV>(a.forc, base, lo, nz, [&](int k) { return k >= k0; }, fo);
     const double fsurf = a.forc_surf[tc.col];  // loaded with the rest, used by the top row
     const double dt = a.dt;
-    load_halo_finish<S, VEC>(tc.s, km);  // after the other loads are issued
+    load_halo_finish<S, PV>(tc.s, km);  // after the other loads are issued
     double va[S], vb[S], vc[S], vd[S];
     auto delta = [&](int t) {  // delta at level lo - 1 + t
         const int k = lo - 1 + t;
@@ -454,7 +456,7 @@ __global__ __launch_bounds__(kTurbBlock) void tke_kernel(TkeArgs a) {
         e1[t] = v;
         st[t] = store;
     }
-    col_store<S, VEC>(a.diss + base, 1, lo, nz, dg);
+    col_store<S, PV>(a.diss + base, 1, lo, nz, dg);
     col_store_if<S, VEC>(E1 + base * g.cs, g.cs, lo, nz, e1, st);
 }
 
@@ -471,7 +473,7 @@ struct EkeArgs {
     double dt, alpha, c_eps;
 };
 
-template <int S, bool VEC>
+template <int S, bool VEC, bool PV>
 __global__ __launch_bounds__(kTurbBlock) void eke_kernel(EkeArgs a) {
     const TurbGrid& g = a.g;
     const TurbCol tc = turb_col(g, S);
@@ -482,10 +484,10 @@ __global__ __launch_bounds__(kTurbBlock) void eke_kernel(EkeArgs a) {
     const int nz = g.nz, lo = tc.lo, k0 = tc.k0;
     const int64_t base = tc.col * nz;
     double km[S + 2];
-    load_halo_issue<S, VEC>(a.kappaM, base, lo, nz, km);
+    load_halo_issue<S, PV>(a.kappaM, base, lo, nz, km);
     double sq[S], el[S];
-    load_lv<S, VEC>(a.sqrteke, base, lo, nz, sq);
-    load_lv<S, VEC>(a.eke_len, base, lo, nz, el);
+    load_lv<S, PV>(a.sqrteke, base, lo, nz, sq);
+    load_lv<S, PV>(a.eke_len, base, lo, nz, el);
     double cint[S];  // eke_c_eps * sqrteke / eke_len * maskW (:121)
 #pragma unroll
     for (int t = 0; t < S; ++t) cint[t] = a.c_eps * sq[t] / el[t] * (wet_at(tc.kb, lo + t) ? 1.0 : 0.0);
@@ -496,9 +498,9 @@ __global__ __launch_bounds__(kTurbBlock) void eke_kernel(EkeArgs a) {
     for (int t = 0; t < S; ++t) e0[t] = e1[t] = 0.0;
     col_load<S, VEC>(E0 + base * g.cs, g.cs, lo, nz, [&](int k) { return k >= k0; }, e0);
     col_load<S, VEC>(E1 + base * g.cs, g.cs, lo, nz, [&](int k) { return tc.valid && k < k0; }, e1);
-    load_terms<S, VEC>(a.forc, base, lo, nz, [&](int k) { return k >= k0; }, fo);
+    load_terms<S, PV>(a.forc, base, lo, nz, [&](int k) { return k >= k0; }, fo);
     const double dt = a.dt;
-    load_halo_finish<S, VEC>(tc.s, km);  // after the other loads are issued
+    load_halo_finish<S, PV>(tc.s, km);  // after the other loads are issued
     double va[S], vb[S], vc[S], vd[S];
     auto delta = [&](int t) {
         const int k = lo - 1 + t;
@@ -573,8 +575,8 @@ __global__ __launch_bounds__(kTurbBlock) void eke_kernel(EkeArgs a) {
         }
         e1[t] = v;
     }
-    col_store<S, VEC>(a.diss_iw + base, 1, lo, nz, di);
-    col_store<S, VEC>(a.diss_tke + base, 1, lo, nz, zero);
+    col_store<S, PV>(a.diss_iw + base, 1, lo, nz, di);
+    col_store<S, PV>(a.diss_tke + base, 1, lo, nz, zero);
     col_store_if<S, VEC>(E1 + base * g.cs, g.cs, lo, nz, e1, st);
 }
 
@@ -594,7 +596,7 @@ struct IdemixArgs {
     int diss_mode;  // 0: as is; 1: column integral at the bottom cell; 2: split bottom / surface
 };
 
-template <int S, bool VEC>
+template <int S, bool VEC, bool PV>
 __global__ __launch_bounds__(kTurbBlock) void idemix_kernel(IdemixArgs a) {
     const TurbGrid& g = a.g;
     const TurbCol tc = turb_col(g, S);
@@ -605,9 +607,9 @@ __global__ __launch_bounds__(kTurbBlock) void idemix_kernel(IdemixArgs a) {
     const int nz = g.nz;
     const int64_t base = tc.col * nz;
     double c0[S + 2];
-    load_halo_issue<S, VEC>(a.c0, base, tc.lo, nz, c0);
+    load_halo_issue<S, PV>(a.c0, base, tc.lo, nz, c0);
     double al[S];
-    load_lv<S, VEC>(a.alpha_c, base, tc.lo, nz, al);
+    load_lv<S, PV>(a.alpha_c, base, tc.lo, nz, al);
     const double* __restrict__ E0 = a.E + a.tau * g.ls;
     double* __restrict__ E1 = a.E + a.taup1 * g.ls;
     const double dt = a.dt;
@@ -624,7 +626,7 @@ __global__ __launch_bounds__(kTurbBlock) void idemix_kernel(IdemixArgs a) {
     // the forcing of the lane's levels (interior columns; wet levels feed the
     // solve, the column integral sums maskW-weighted levels)
     double fo[S];
-    load_terms<S, VEC>(a.forc, base, tc.lo, nz, [&](int k) { return tc.interior && wet_at(tc.kb, k); }, fo);
+    load_terms<S, PV>(a.forc, base, tc.lo, nz, [&](int k) { return tc.interior && wet_at(tc.kb, k); }, fo);
 #pragma unroll
     for (int t = 0; t < S; ++t) fo[t] = (tc.interior && wet_at(tc.kb, tc.lo + t)) ? fo[t] : 0.0;
     if (a.diss_mode != 0) {  // (:70-103)
@@ -652,8 +654,8 @@ __global__ __launch_bounds__(kTurbBlock) void idemix_kernel(IdemixArgs a) {
         // columns are never solved
     }
     // terms added afterwards, on the levels whose rows are solved
-    add_terms<S, VEC>(a.forc_post, base, tc.lo, nz, [&](int k) { return k >= tc.k0; }, false, fo);
-    load_halo_finish<S, VEC>(tc.s, c0);  // after the other loads are issued
+    add_terms<S, PV>(a.forc_post, base, tc.lo, nz, [&](int k) { return k >= tc.k0; }, false, fo);
+    load_halo_finish<S, PV>(tc.s, c0);  // after the other loads are issued
     double va[S], vb[S], vc[S], vd[S];
     auto delta = [&](int t) {
         const int k = tc.lo - 1 + t;
@@ -738,7 +740,7 @@ __global__ __launch_bounds__(kTurbBlock) void idemix_kernel(IdemixArgs a) {
         }
         e1[t] = v;
     }
-    col_store<S, VEC>(a.iw_diss + base, 1, tc.lo, nz, di);
+    col_store<S, PV>(a.iw_diss + base, 1, tc.lo, nz, di);
     col_store_if<S, VEC>(E1 + base * g.cs, g.cs, tc.lo, nz, e1, st);
 }
 
@@ -782,17 +784,19 @@ int make_terms(const vx_terms* in, Terms& t, bool allow_empty, const char* who) 
     return VX_OK;
 }
 
-template <class Args, template <int, bool> class K>
-int launch_turb(const Args& a, const TurbGrid& g, bool vec, hipStream_t s, const char* who) {
+template <class Args, template <int, bool, bool> class K>
+int launch_turb(const Args& a, const TurbGrid& g, bool vec, bool pv, hipStream_t s, const char* who) {
     const int S = seg_levels(g.nz);
     const int64_t threads = (int64_t)g.X * g.Y * kSegP;
     const dim3 grid(blocks_mult8(threads, kTurbBlock)), block(kTurbBlock);
 #define VX_TURB_CASE(N)                                                     \
     case N:                                                                 \
         if (vec)                                                            \
-            hipLaunchKernelGGL((K<N, true>::fn), grid, block, 0, s, a);     \
+            hipLaunchKernelGGL((K<N, true, true>::fn), grid, block, 0, s, a); \
+        else if (pv)                                                        \
+            hipLaunchKernelGGL((K<N, false, true>::fn), grid, block, 0, s, a); \
         else                                                                \
-            hipLaunchKernelGGL((K<N, false>::fn), grid, block, 0, s, a);    \
+            hipLaunchKernelGGL((K<N, false, false>::fn), grid, block, 0, s, a); \
         break;
     switch (S) {
         VX_TURB_CASE(2)
@@ -809,22 +813,23 @@ int launch_turb(const Args& a, const TurbGrid& g, bool vec, hipStream_t s, const
     return check_launch(who);
 }
 
-template <int N, bool V>
+template <int N, bool V, bool P>
 struct TkeK {
-    static constexpr auto fn = tke_kernel<N, V>;
+    static constexpr auto fn = tke_kernel<N, V, P>;
 };
-template <int N, bool V>
+template <int N, bool V, bool P>
 struct EkeK {
-    static constexpr auto fn = eke_kernel<N, V>;
+    static constexpr auto fn = eke_kernel<N, V, P>;
 };
-template <int N, bool V>
+template <int N, bool V, bool P>
 struct IdemixK {
-    static constexpr auto fn = idemix_kernel<N, V>;
+    static constexpr auto fn = idemix_kernel<N, V, P>;
 };
 
 // the 16-byte pair variant: levels outermost, nz even, every array aligned
-bool turb_vec(const TurbGrid& g, std::initializer_list<const void*> arrays, const Terms& t1, const Terms* t2) {
-    if (g.cs != 1 || (g.nz & 1)) return false;
+// level pairs for the arrays listed (nz even, every one 16-byte aligned)
+bool turb_pairs(const TurbGrid& g, std::initializer_list<const void*> arrays, const Terms& t1, const Terms* t2) {
+    if (g.nz & 1) return false;
     for (const void* p : arrays)
         if (!aligned16(p)) return false;
     for (int q = 0; q < t1.n; ++q)
@@ -833,6 +838,12 @@ bool turb_vec(const TurbGrid& g, std::initializer_list<const void*> arrays, cons
         for (int q = 0; q < t2->n; ++q)
             if (!aligned16(t2->p[q])) return false;
     return true;
+}
+
+// the 16-byte pair variant for every array, the energy's time levels too
+// (they must be stored outermost)
+bool turb_vec(const TurbGrid& g, std::initializer_list<const void*> arrays, const Terms& t1, const Terms* t2) {
+    return g.cs == 1 && turb_pairs(g, arrays, t1, t2);
 }
 
 bool levels_ok(int tau, int taup1) { return tau >= 0 && tau <= 2 && taup1 >= 0 && taup1 <= 2 && tau != taup1; }
@@ -862,7 +873,9 @@ extern "C" int vx_integrate_tke_f64(void* stream, const vx_grid* grid, const vx_
     a.hor = prm->hor_diffusion != 0;
     const bool vec = turb_vec(a.g, {tke, tke + tau * a.g.ls, tke + taup1 * a.g.ls, kappaM, sqrttke, mxl, tke_diss},
                               a.forc, nullptr);
-    return launch_turb<TkeArgs, TkeK>(a, a.g, vec, (hipStream_t)stream, who);
+    // the arrays without time levels in pairs whatever the layout
+    const bool pv = vec || turb_pairs(a.g, {kappaM, sqrttke, mxl, tke_diss}, a.forc, nullptr);
+    return launch_turb<TkeArgs, TkeK>(a, a.g, vec, pv, (hipStream_t)stream, who);
 }
 
 extern "C" int vx_integrate_eke_f64(void* stream, const vx_grid* grid, const vx_eke_params* prm, double* eke,
@@ -886,7 +899,8 @@ extern "C" int vx_integrate_eke_f64(void* stream, const vx_grid* grid, const vx_
     a.dt = prm->dt_tracer, a.alpha = prm->alpha_eke, a.c_eps = prm->eke_c_eps;
     const bool vec = turb_vec(a.g, {eke, eke + tau * a.g.ls, eke + taup1 * a.g.ls, kappaM, sqrteke, eke_len,
                                     K_gm, eke_diss_iw, eke_diss_tke}, a.forc, nullptr);
-    return launch_turb<EkeArgs, EkeK>(a, a.g, vec, (hipStream_t)stream, who);
+    const bool pv = vec || turb_pairs(a.g, {kappaM, sqrteke, eke_len, eke_diss_iw, eke_diss_tke}, a.forc, nullptr);
+    return launch_turb<EkeArgs, EkeK>(a, a.g, vec, pv, (hipStream_t)stream, who);
 }
 
 extern "C" int vx_integrate_idemix_f64(void* stream, const vx_grid* grid, const vx_idemix_params* prm, double* E_iw,
@@ -916,5 +930,6 @@ extern "C" int vx_integrate_idemix_f64(void* stream, const vx_grid* grid, const 
     a.diss_mode = prm->eke_diss_mode;
     const bool vec = turb_vec(a.g, {E_iw, E_iw + tau * a.g.ls, E_iw + taup1 * a.g.ls, c0, v0, alpha_c, iw_diss},
                               a.forc, &a.forc_post);
-    return launch_turb<IdemixArgs, IdemixK>(a, a.g, vec, (hipStream_t)stream, who);
+    const bool pv = vec || turb_pairs(a.g, {c0, alpha_c, iw_diss}, a.forc, &a.forc_post);
+    return launch_turb<IdemixArgs, IdemixK>(a, a.g, vec, pv, (hipStream_t)stream, who);
 }
